@@ -1,0 +1,220 @@
+"""Throughput of the KPM Chebyshev path on one MI355X.
+
+Workload (the precision form of the KPM DOS benchmark): honeycomb lattice of
+1024 x 1024 two-atom cells (N = 2,097,152), periodic, t = -2.7 eV, onsite
+disorder U[-0.5, 0.5] eV (seeded), ELL width K = 4, spectral bound
+B = (3|t| + W/2) * 1.01; M = 2048 Chebyshev moments for R = 16 random-phase
+vectors (seeds 1000..1015), complex64 recursion with FP64 moment dots, and
+the Jackson-damped DOS on 4096 energies.
+
+One step = the whole moment recursion (M - 1 fused SpMV steps over the R
+vectors) plus the DOS reconstruction, inputs resident in HBM.  The metric is
+N*M*R units per second.  The JSON line also carries
+  roofline      the fused step kernel's algorithmic bytes per launch divided
+                by its mean launch duration (HIP events on the launch
+                stream, every `--profile-every`-th launch of the timed steps)
+                against the 8 TB/s HBM peak, plus HBM traffic per launch from
+                the committed rocprofv3 counter summary;
+  cpu_baseline  the complex128 CPU oracle (the reference algorithm, one
+                vector at a time) on a bounded sample of the same workload;
+  dos_err_vs_c128  max |DOS_c64 - DOS_c128| / max |DOS_c128| on the same
+                inputs (the complex128 GPU path is pinned to the oracle by the
+                tests).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("Chebyshev SpMV-steps/s (N·M·R/s) + % HBM roofline at DOS err "
+          "≤1e-5 vs FP64")
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = ROOT / "profiles" / "cheb_step_c64_r16_honeycomb1024_pmc.md"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(L, M, R, seed):
+    import minijclsquant_amd as jcl
+
+    H, Dx, Dy = jcl.honeycomb_lattice(L, t=-2.7, W=1.0, seed=seed)
+    A = jcl.ell_matrix(H)
+    B = (3 * 2.7 + 0.5) * 1.01
+    A.bounds = np.array([-B, B])
+    N = A.shape[0]
+    r = np.stack([jcl.random_vector_generator(N, 1000 + s) for s in range(R)])
+    return A, r, B
+
+
+def traffic_from_profile(N, K, R):
+    """HBM bytes per launch from the committed counter summary, if it was
+    measured on this exact workload shape."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    txt = TRAFFIC_FILE.read_text()
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    w = re.search(r"workload:\s*N=(\d+)\s+K=(\d+)\s+R=(\d+)", txt)
+    if not m or not w or tuple(map(int, w.groups())) != (N, K, R):
+        return None
+    return float(m.group(1))
+
+
+def cpu_baseline(A, r, target_s=15.0):
+    """complex128 oracle (reference algorithm) on host cores: one vector, a
+    bounded number of moments sized to ~target_s seconds."""
+    from oracle import oracle as O
+
+    thr = int(os.environ.get("OMP_NUM_THREADS", min(os.cpu_count() or 1, 16)))
+    Ht = A.data * (1 / A.bounds[1])
+    N = A.shape[0]
+    m_probe = 16
+    t0 = time.perf_counter()
+    O.rec_A_tab(m_probe, Ht, A.indices, A.len_row, r[0], n_threads=thr)
+    dt = time.perf_counter() - t0
+    m = int(max(m_probe, min(4096, m_probe * target_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    O.rec_A_tab(m, Ht, A.indices, A.len_row, r[0], n_threads=thr)
+    dt = time.perf_counter() - t0
+    return {
+        "value": N * m / dt,
+        "unit": "N*M*R/s",
+        "cores": thr,
+        "kind": "port",
+        "sample": f"same H, R=1 vector, M={m} moments, complex128, {dt:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=1024)
+    ap.add_argument("--moments", type=int, default=2048)
+    ap.add_argument("--vectors", type=int, default=16)
+    ap.add_argument("--energies", type=int, default=4096)
+    ap.add_argument("--dtype", default="c64")
+    ap.add_argument("--profile-every", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dos-check", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the KPM path "
+              "shards over vectors and disorder realisations with one final all-reduce "
+              "of the moments, which this build does not run", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import minijclsquant_amd as jcl
+    from minijclsquant_amd import _kpm, _native as nat
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    L, M, R = args.cells, args.moments, args.vectors
+    log(f"building workload: honeycomb {L}x{L} cells, M={M}, R={R}, {args.dtype}")
+    A, r, B = build_workload(L, M, R, seed=2024)
+    N, K = A.shape[0], A.len_row
+    h = A.device(args.dtype, 1.0 / B)
+    vr = _kpm.as_batch(r, N, args.dtype)
+    ws = torch.empty(h.workspace_bytes(R, 1), dtype=torch.uint8, device="cuda")
+    mu = torch.empty((1, M, R), dtype=torch.complex128, device="cuda")
+    E = np.linspace(-B + 1e-3, B - 1e-3, args.energies)
+    es = torch.as_tensor(E / B, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def one_step():
+        h.moments(vr, M, workspace=ws, out=mu)
+        return nat.dos(mu[0], es, nat.JKPM_KERNEL_JACKSON)
+
+    for i in range(args.warmup):
+        one_step()
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+
+    nat.profile_read(reset=True)
+    nat.profile_enable(args.profile_every)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        dos = one_step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nat.profile_enable(0)
+    step_ms_sum, n_prof = nat.profile_read(reset=True)
+    log(f"timed {args.steps} steps in {t1 - t0:.3f} s")
+    del stream
+
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    units = N * M * R
+    value = units / (ms_per_step / 1e3)
+    s_v = 8 if args.dtype == "c64" else 16
+    bytes_launch = N * K * (s_v + 4) + R * N * 4 * s_v
+    launch_ms = step_ms_sum / max(n_prof, 1)
+    achieved = bytes_launch / (launch_ms / 1e3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profile(N, K, R),
+        "kernel": "cheb_step_kernel",
+        "algorithmic_bytes_per_launch": bytes_launch,
+        "mean_launch_us": round(launch_ms * 1e3, 2),
+        "launches_timed": n_prof,
+    }
+
+    dos_err = None
+    if not args.no_dos_check and args.dtype == "c64":
+        log("complex128 reference run for the DOS error")
+        d128 = jcl.kpm_dos(A, M, r, energies=E, dtype="c128")[:, 1]
+        d64 = dos.cpu().numpy()
+        dos_err = float(np.max(np.abs(d64 - d128)) / np.max(np.abs(d128)))
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (oracle, complex128, bounded sample)")
+        cpu = cpu_baseline(A, r)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "SpMV-steps/s (N*M*R per second)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic (seeded lattice disorder and random-phase vectors)",
+        "config": {
+            "workload": (f"KPM DOS precision form: honeycomb {L}x{L} cells periodic, "
+                         f"t=-2.7, W=1 eV, K={K}, M={M}, R={R}, "
+                         f"{args.dtype} recursion + FP64 moments, DOS on "
+                         f"{args.energies} energies"),
+            "N": N, "K": K, "M": M, "R": R, "bound": B,
+        },
+        "dos_err_vs_c128": dos_err,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

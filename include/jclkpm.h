@@ -1,0 +1,145 @@
+/*
+ * jclkpm -- C-ABI of the MI355X (gfx950) Chebyshev/KPM library
+ * (libjclkpm_hip.so, built from minijclsquant_amd/csrc/).
+ *
+ * Conventions
+ *   - Every entry point returns an int status: JKPM_OK (0) or a JKPM_ERR_*
+ *     code; jkpm_last_error() gives a message for the calling thread.
+ *   - Complex numbers are interleaved (re, im) pairs: double for JKPM_C128,
+ *     float for JKPM_C64.
+ *   - Host ELL input is the reference's row-major layout: slot j of row i at
+ *     [i*len_row + j] (jclsquant/cython_modules/Extra_funs.pyx:182-219).
+ *     On the device the handle keeps a slot-major copy ([j*N + i]) scaled by
+ *     `scale` (1/B for the spectral rescale) and cast to the value type.
+ *   - Device vectors of a batch of R vectors are row-interleaved:
+ *     element (row i, vector r) at [i*R + r].
+ *   - Moments are complex128, laid out [n_left][M][R].
+ *   - `stream` is a hipStream_t (NULL = default stream).  Device-pointer
+ *     entry points are asynchronous on that stream and allocate nothing;
+ *     scratch memory comes in through `workspace`.
+ *   - A handle may be used by one thread at a time.
+ */
+#ifndef JCLKPM_H
+#define JCLKPM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JKPM_OK 0
+#define JKPM_ERR_ARG 1         /* bad argument (size, pointer, unsupported R) */
+#define JKPM_ERR_HIP 2         /* a HIP runtime call failed */
+#define JKPM_ERR_ALLOC 3       /* device allocation failed */
+#define JKPM_ERR_WORKSPACE 4   /* workspace smaller than required */
+
+#define JKPM_C128 0
+#define JKPM_C64 1
+
+#define JKPM_KERNEL_NONE 0
+#define JKPM_KERNEL_JACKSON 1
+
+typedef struct jkpm_ell jkpm_ell;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int jkpm_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *jkpm_last_error(void);
+
+/* ---------------------------------------------------------------------
+ * Reference drop-ins: host pointers in/out, synchronous, complex128.
+ * ------------------------------------------------------------------- */
+
+/* Replaces extern "C" rec_A_tab (jclsquant/cuda_cython/cuda/obv.h:20,
+ * jclsquant/cuda_cython/cuda/recursions.cu:17-98): A_vec[m] = <v|T_m(A)|v>,
+ * m < M, for the row-major ELL (data, indices, len_row) of N rows.
+ * data must already carry the 1/B rescale, as in the reference. */
+int rec_A_tab(const double *data, const int *indices, int len_row, int N,
+              double *A_vec, const double *v, int M);
+
+/* Replaces extern "C" rec_A_tab_2 (obv.h:22, recursions.cu:180-263):
+ * A_vec[m] = <v_l|T_m(A)|v_r>. */
+int rec_A_tab_2(const double *data, const int *indices, int len_row, int N,
+                double *A_vec, const double *v_l, const double *v_r, int M);
+
+/* ---------------------------------------------------------------------
+ * Device-resident ELL handle.
+ * ------------------------------------------------------------------- */
+
+/* Uploads a row-major complex128 ELL (host or device pointers, chosen by
+ * src_on_device), multiplies the values by `scale` in complex128 (as
+ * jclsquant/modifiers.py:24-26 does with fl(1/B)), casts them to value_type
+ * and stores them slot-major.  Replaces the per-call H2D upload of
+ * recursions.cu:24-28 with one upload per matrix. */
+int jkpm_ell_create(const double *data, const int *indices, int n_rows, int len_row,
+                    double scale, int value_type, int src_on_device, void *stream,
+                    jkpm_ell **out);
+
+int jkpm_ell_destroy(jkpm_ell *h);
+
+int jkpm_ell_info(const jkpm_ell *h, int *n_rows, int *len_row, int *value_type);
+
+/* Replaces the values (same indices) from a row-major complex128 array. */
+int jkpm_ell_set_values(jkpm_ell *h, const double *data, double scale,
+                        int src_on_device, void *stream);
+
+/* y <- a*A*x + b*y on R interleaved device vectors of the handle's value type.
+ * Replaces aAxby_device (jclsquant/cuda_cython/cuda/blas.cu:29-43) and the
+ * CPU aAxby (jclsquant/cython_modules/blas_funs.pyx:64-75).  x must not alias y.
+ * a and b are (re, im). */
+int jkpm_aAxby(const jkpm_ell *h, const double *a, const double *b, const void *x,
+               void *y, int R, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Chebyshev moments (the hot path).
+ * ------------------------------------------------------------------- */
+
+/* Workspace bytes jkpm_moments needs for R vectors and n_left left sets. */
+int jkpm_moments_workspace(const jkpm_ell *h, int R, int n_left, size_t *bytes);
+
+/* mu[l][m][r] = <v_l{l}[:,r] | T_m(A) v_r[:,r]>, m < M, complex128, for
+ * l < n_left (n_left = 1 if v_l1 is NULL, else 2).  The recursion runs in the
+ * handle's value type; the dots accumulate in FP64 with a fixed reduction
+ * order (bitwise reproducible).  Generalises rec_A_tab / rec_A_tab_2 /
+ * rec_A_tab2v_2 (jclsquant/recurrence_functions.py:9-111) to R vectors that
+ * share each pass over the matrix.  v_r, v_l0, v_l1: device [N][R]. */
+int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const void *v_l1,
+                 int R, int M, double *mu, void *workspace, size_t workspace_bytes,
+                 void *stream);
+
+/* ---------------------------------------------------------------------
+ * DOS reconstruction.
+ * ------------------------------------------------------------------- */
+
+/* dos[i] = sum_n g_n delta_n(e_i) Re(mean_r mu[m][r]) with
+ * delta_n(e) = (2 - [n==0]) cos(n acos e) / (pi sqrt(1 - e^2)),
+ * g = Jackson (kernel = JKPM_KERNEL_JACKSON) or 1, for rescaled energies
+ * |e_i| < 1.  mu: device complex128 [M][R]; e, dos: device float64 [n_e].
+ * Replaces the host loop of jclsquant/observables.py:49-52 over
+ * JacksonKernel / moments_delta (jclsquant/kernel_and_moments.py:19-24,72-78).
+ * workspace: >= 8*M bytes. */
+int jkpm_dos(const double *mu, int M, int R, const double *e, int n_e, int kernel,
+             double *dos, void *workspace, size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Measurement support.
+ * ------------------------------------------------------------------- */
+
+/* every > 0: jkpm_moments records a HIP event pair around every every-th
+ * Chebyshev step launch on its stream (0 = off); jkpm_profile_read sums the
+ * event durations of the recorded steps (ms) and counts them.
+ * reset != 0 clears the totals. */
+int jkpm_profile_enable(int every);
+int jkpm_profile_read(double *step_ms, long long *step_launches, int reset);
+
+/* Plain device copy dst <- src of `bytes` bytes (a multiple of 16), used to
+ * measure the achievable HBM rate and to calibrate traffic counters. */
+int jkpm_copy(void *dst, const void *src, size_t bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JCLKPM_H */

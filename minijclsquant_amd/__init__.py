@@ -1,0 +1,17 @@
+"""MI355X-native Chebyshev/KPM hot path of jclsquant.
+
+The package namespace mirrors the reference's star-import surface
+(jclsquant/__init__.py:1-10) for the KPM path: ell_matrix, recurrence
+functions, kernels and moments, modifiers, DOS observables (CPU- and
+GPU-named) and lattice builders.  The compute path is the HIP library
+lib/libjclkpm_hip.so (C-ABI in include/jclkpm.h), called through ctypes.
+"""
+from .ell_matrix import *  # noqa: F401,F403
+from .kernel_and_moments import *  # noqa: F401,F403
+from .lattice import *  # noqa: F401,F403
+from .modifiers import *  # noqa: F401,F403
+from .observables import *  # noqa: F401,F403
+from .observables_gpu import *  # noqa: F401,F403
+from .recurrence_functions import *  # noqa: F401,F403
+
+__version__ = "0.1.0"

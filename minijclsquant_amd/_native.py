@@ -1,0 +1,336 @@
+"""ctypes binding of libjclkpm_hip.so (the C-ABI declared in include/jclkpm.h).
+
+This is the only route from Python to the compute path.  There is no CPU
+fallback: when the library is missing or no HIP device is visible, every call
+raises.  Device buffers are PyTorch-ROCm tensors; the library receives their
+raw pointers and the current HIP stream.
+
+The reference binds its CUDA library the same way, through a thin extension
+over ``extern "C"`` symbols (jclsquant/cuda_cython/obv.pyx:9-49).
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libjclkpm_hip.so"
+
+JKPM_OK = 0
+JKPM_ERR_ARG = 1
+JKPM_ERR_HIP = 2
+JKPM_ERR_ALLOC = 3
+JKPM_ERR_WORKSPACE = 4
+
+JKPM_C128 = 0
+JKPM_C64 = 1
+
+JKPM_KERNEL_NONE = 0
+JKPM_KERNEL_JACKSON = 1
+
+_c = ctypes
+_vp = _c.c_void_p
+_int = _c.c_int
+_dbl = _c.c_double
+_size = _c.c_size_t
+_dp = _c.POINTER(_c.c_double)
+_ip = _c.POINTER(_c.c_int)
+
+# name -> (restype, argtypes); every symbol include/jclkpm.h declares.
+SIGNATURES = {
+    "jkpm_version": (_int, []),
+    "jkpm_last_error": (_c.c_char_p, []),
+    "rec_A_tab": (_int, [_vp, _vp, _int, _int, _vp, _vp, _int]),
+    "rec_A_tab_2": (_int, [_vp, _vp, _int, _int, _vp, _vp, _vp, _int]),
+    "jkpm_ell_create": (_int, [_vp, _vp, _int, _int, _dbl, _int, _int, _vp, _c.POINTER(_vp)]),
+    "jkpm_ell_destroy": (_int, [_vp]),
+    "jkpm_ell_info": (_int, [_vp, _ip, _ip, _ip]),
+    "jkpm_ell_set_values": (_int, [_vp, _vp, _dbl, _int, _vp]),
+    "jkpm_aAxby": (_int, [_vp, _dp, _dp, _vp, _vp, _int, _vp]),
+    "jkpm_moments_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
+    "jkpm_moments": (_int, [_vp, _vp, _vp, _vp, _int, _int, _vp, _vp, _size, _vp]),
+    "jkpm_dos": (_int, [_vp, _int, _int, _vp, _int, _int, _vp, _vp, _size, _vp]),
+    "jkpm_profile_enable": (_int, [_int]),
+    "jkpm_profile_read": (_int, [_dp, _c.POINTER(_c.c_longlong), _int]),
+    "jkpm_copy": (_int, [_vp, _vp, _size, _vp]),
+}
+
+
+class JkpmError(RuntimeError):
+    """A jclkpm call returned a non-zero status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"jclkpm error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load_library(path: Path | None = None) -> ctypes.CDLL:
+    """Loads the HIP library (no GPU needed to load it) and declares symbols."""
+    global _lib
+    with _lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = Path(path) if path is not None else LIB_PATH
+        if not p.exists():
+            raise RuntimeError(
+                f"{p} is not built: run `python -m minijclsquant_amd._build` "
+                "(or __graft_entry__.build()) first"
+            )
+        lib = ctypes.CDLL(str(p))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def lib() -> ctypes.CDLL:
+    return load_library()
+
+
+def check(rc: int) -> None:
+    if rc != JKPM_OK:
+        msg = lib().jkpm_last_error()
+        msg = msg.decode(errors="replace") if msg else ""
+        if rc == JKPM_ERR_ARG:
+            raise ValueError(f"jclkpm: {msg}")
+        raise JkpmError(rc, msg)
+
+
+# ---------------------------------------------------------------- torch glue
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def require_device():
+    """Returns torch after checking that a HIP device is visible."""
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "no HIP device visible: the jclkpm compute path runs on an MI355X GPU only"
+        )
+    lib()
+    return torch
+
+
+def stream_ptr(stream=None) -> int:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def vtype_of(dtype) -> int:
+    torch = _torch()
+    if dtype in (torch.complex128, np.complex128, "c128", "complex128"):
+        return JKPM_C128
+    if dtype in (torch.complex64, np.complex64, "c64", "complex64"):
+        return JKPM_C64
+    raise ValueError(f"unsupported value type {dtype!r}; use complex128 or complex64")
+
+
+def torch_cdtype(vtype: int):
+    torch = _torch()
+    return torch.complex128 if vtype == JKPM_C128 else torch.complex64
+
+
+class DeviceELL:
+    """Device-resident, slot-major ELL matrix behind a jkpm_ell handle.
+
+    ``data`` is the reference's row-major complex128 ELL value array
+    (``ell_matrix.data``), ``indices`` its int32 column array; both may be
+    numpy arrays (uploaded) or contiguous torch device tensors.  The values are
+    multiplied by ``scale`` in complex128 and stored in ``dtype``.
+    """
+
+    def __init__(self, data, indices, n_rows: int, len_row: int, scale: float = 1.0,
+                 dtype="c128", stream=None):
+        torch = require_device()
+        self.n = int(n_rows)
+        self.k = int(len_row)
+        self.vtype = vtype_of(dtype)
+        self.scale = float(scale)
+        self._h = _vp()
+        on_dev = isinstance(data, torch.Tensor)
+        if on_dev:
+            if not (data.is_cuda and indices.is_cuda):
+                raise ValueError("torch inputs must be device tensors")
+            if data.dtype != torch.complex128 or indices.dtype != torch.int32:
+                raise ValueError("device ELL input must be complex128 values and int32 indices")
+            data = data.contiguous()
+            indices = indices.contiguous()
+            if data.numel() != self.n * self.k or indices.numel() != self.n * self.k:
+                raise ValueError("data/indices must hold n_rows*len_row entries")
+            dp, ip = data.data_ptr(), indices.data_ptr()
+            keep = (data, indices)
+        else:
+            data = np.ascontiguousarray(data, dtype=np.complex128)
+            indices = np.ascontiguousarray(indices, dtype=np.int32)
+            if data.size != self.n * self.k or indices.size != self.n * self.k:
+                raise ValueError("data/indices must hold n_rows*len_row entries")
+            dp, ip = data.ctypes.data, indices.ctypes.data
+            keep = (data, indices)
+        check(lib().jkpm_ell_create(dp, ip, self.n, self.k, self.scale, self.vtype,
+                                    1 if on_dev else 0, stream_ptr(stream), _c.byref(self._h)))
+        del keep
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def dtype(self):
+        return torch_cdtype(self.vtype)
+
+    def set_values(self, data, scale: float | None = None, stream=None) -> None:
+        torch = _torch()
+        sc = self.scale if scale is None else float(scale)
+        if isinstance(data, torch.Tensor):
+            d = data.contiguous()
+            if d.dtype != torch.complex128 or d.numel() != self.n * self.k:
+                raise ValueError("values must be complex128 with n_rows*len_row entries")
+            check(lib().jkpm_ell_set_values(self._h, d.data_ptr(), sc, 1, stream_ptr(stream)))
+        else:
+            d = np.ascontiguousarray(data, dtype=np.complex128)
+            if d.size != self.n * self.k:
+                raise ValueError("values must hold n_rows*len_row entries")
+            check(lib().jkpm_ell_set_values(self._h, d.ctypes.data, sc, 0, stream_ptr(stream)))
+        self.scale = sc
+
+    def close(self) -> None:
+        if self._h:
+            lib().jkpm_ell_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ operations
+
+    def _check_vecs(self, *vs):
+        torch = _torch()
+        R = None
+        for v in vs:
+            if v is None:
+                continue
+            if not isinstance(v, torch.Tensor) or not v.is_cuda:
+                raise ValueError("vectors must be device tensors")
+            if v.dtype != self.dtype:
+                raise ValueError(f"vectors must be {self.dtype}, got {v.dtype}")
+            if v.dim() != 2 or v.shape[0] != self.n or not v.is_contiguous():
+                raise ValueError(f"vectors must be contiguous [N={self.n}, R]")
+            if R is None:
+                R = v.shape[1]
+            elif v.shape[1] != R:
+                raise ValueError("all vector batches must have the same R")
+        return R
+
+    def workspace_bytes(self, R: int, n_left: int = 1) -> int:
+        out = _size()
+        check(lib().jkpm_moments_workspace(self._h, int(R), int(n_left), _c.byref(out)))
+        return int(out.value)
+
+    def moments(self, v_r, M: int, v_l0=None, v_l1=None, workspace=None, out=None,
+                stream=None):
+        """mu[l, m, r] = <v_l{l}[:, r] | T_m v_r[:, r]>, complex128 [n_left, M, R].
+
+        v_r, v_l0, v_l1: device tensors [N, R] of the handle's dtype
+        (v_l0 defaults to v_r)."""
+        torch = _torch()
+        if v_l0 is None:
+            v_l0 = v_r
+        R = self._check_vecs(v_r, v_l0, v_l1)
+        M = int(M)
+        if M <= 0:
+            raise ValueError("M must be positive")
+        n_left = 2 if v_l1 is not None else 1
+        need = self.workspace_bytes(R, n_left)
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty(need, dtype=torch.uint8, device=v_r.device)
+        if out is None:
+            out = torch.empty((n_left, M, R), dtype=torch.complex128, device=v_r.device)
+        elif out.shape != (n_left, M, R) or out.dtype != torch.complex128 or not out.is_contiguous():
+            raise ValueError("out must be contiguous complex128 [n_left, M, R]")
+        check(lib().jkpm_moments(self._h, v_r.data_ptr(), v_l0.data_ptr(),
+                                 v_l1.data_ptr() if v_l1 is not None else None, R, M,
+                                 out.data_ptr(), workspace.data_ptr(), workspace.numel(),
+                                 stream_ptr(stream)))
+        return out
+
+    def aAxby(self, a: complex, b: complex, x, y, stream=None) -> None:
+        """y <- a*A*x + b*y on device tensors [N, R] (in place on y)."""
+        R = self._check_vecs(x, y)
+        if x.data_ptr() == y.data_ptr():
+            raise ValueError("x must not alias y")
+        av = (_c.c_double * 2)(complex(a).real, complex(a).imag)
+        bv = (_c.c_double * 2)(complex(b).real, complex(b).imag)
+        check(lib().jkpm_aAxby(self._h, av, bv, x.data_ptr(), y.data_ptr(), R,
+                               stream_ptr(stream)))
+
+
+def dos(mu, energies_scaled, kernel: int = JKPM_KERNEL_JACKSON, stream=None):
+    """DOS on device from complex128 moments mu [M, R] (averaged over R)."""
+    torch = _torch()
+    if mu.dtype != torch.complex128 or mu.dim() != 2 or not mu.is_contiguous():
+        raise ValueError("mu must be contiguous complex128 [M, R]")
+    e = energies_scaled.to(device=mu.device, dtype=torch.float64).contiguous()
+    if e.numel() and float(e.abs().max()) >= 1.0:
+        raise ValueError("rescaled energies must lie strictly inside (-1, 1)")
+    M, R = mu.shape
+    out = torch.empty(e.numel(), dtype=torch.float64, device=mu.device)
+    ws = torch.empty(M, dtype=torch.float64, device=mu.device)
+    check(lib().jkpm_dos(mu.data_ptr(), M, R, e.data_ptr(), e.numel(), int(kernel),
+                         out.data_ptr(), ws.data_ptr(), ws.numel() * 8, stream_ptr(stream)))
+    return out
+
+
+def profile_enable(every: int) -> None:
+    """Record HIP events around every `every`-th step launch (0 = off)."""
+    check(lib().jkpm_profile_enable(int(every)))
+
+
+def profile_read(reset: bool = True):
+    ms = _c.c_double()
+    n = _c.c_longlong()
+    check(lib().jkpm_profile_read(_c.byref(ms), _c.byref(n), 1 if reset else 0))
+    return float(ms.value), int(n.value)
+
+
+def copy(dst, src, nbytes: int, stream=None) -> None:
+    check(lib().jkpm_copy(dst.data_ptr(), src.data_ptr(), int(nbytes), stream_ptr(stream)))
+
+
+def rec_A_tab_host(data, indices, len_row: int, v, M: int, v_r=None):
+    """Calls the reference-compatible host-pointer symbols rec_A_tab /
+    rec_A_tab_2 and returns complex128 [M]."""
+    require_device()
+    data = np.ascontiguousarray(data, dtype=np.complex128)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.complex128)
+    N = v.shape[0]
+    if data.size != N * len_row or indices.size != N * len_row:
+        raise ValueError("data/indices must hold N*len_row entries")
+    out = np.zeros(M, dtype=np.complex128)
+    if v_r is None:
+        check(lib().rec_A_tab(data.ctypes.data, indices.ctypes.data, int(len_row), N,
+                              out.ctypes.data, v.ctypes.data, int(M)))
+    else:
+        v_r = np.ascontiguousarray(v_r, dtype=np.complex128)
+        if v_r.shape != v.shape:
+            raise ValueError("v_l and v_r must have the same length")
+        check(lib().rec_A_tab_2(data.ctypes.data, indices.ctypes.data, int(len_row), N,
+                                out.ctypes.data, v.ctypes.data, v_r.ctypes.data, int(M)))
+    return out

@@ -1,0 +1,495 @@
+// extern "C" entry points of libjclkpm_hip.so (declared in include/jclkpm.h).
+//
+// Host orchestration of the Chebyshev recursion: the reference drives one
+// SpMV launch plus one host-synchronising Thrust reduction per step and
+// re-uploads the matrix on every call (jclsquant/cuda_cython/cuda/
+// recursions.cu:17-98).  Here the matrix lives on the device behind a handle,
+// every step is one fused kernel launch on the caller's stream, moment
+// partials go to a ring of `kRing` steps that one reduction launch folds into
+// mu, and nothing synchronises the host until the caller does.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+struct jkpm_ell {
+    jkpm::Ell e;
+};
+
+namespace jkpm {
+
+namespace {
+thread_local std::string g_err;
+
+constexpr int kRing = 64;  // steps of partials held before one reduction launch
+
+// ---- profiling of the step kernel (events on the launch stream) ----
+struct Prof {
+    std::mutex mu;
+    int every = 0;  // record every `every`-th step (0 = off)
+    long long counter = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> free_events;
+    double ms = 0.0;
+    long long launches = 0;
+};
+Prof &prof() {
+    static Prof p;
+    return p;
+}
+
+hipEvent_t take_event(Prof &p) {
+    if (!p.free_events.empty()) {
+        hipEvent_t e = p.free_events.back();
+        p.free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Folds the durations of all recorded pairs into the totals (waits for them).
+int drain_profile(Prof &p) {
+    for (auto &pr : p.pending) {
+        JKPM_HIP(hipEventSynchronize(pr.second));
+        float ms = 0.f;
+        JKPM_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+        p.ms += ms;
+        p.launches += 1;
+        p.free_events.push_back(pr.first);
+        p.free_events.push_back(pr.second);
+    }
+    p.pending.clear();
+    return JKPM_OK;
+}
+
+__global__ void count_bad_indices(const int *__restrict__ idx, long tot, int n,
+                                  unsigned int *__restrict__ bad) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < tot) {
+        const int j = idx[t];
+        if (j < 0 || j >= n) atomicAdd(bad, 1u);
+    }
+}
+
+bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+struct MomentsLayout {
+    size_t b0 = 0, b1 = 0, ring = 0, total = 0;
+    int nchunks = 1;
+};
+
+MomentsLayout moments_layout(const Ell &H, int R, int n_left, const DeviceInfo &dev) {
+    MomentsLayout L;
+    const size_t vec = align_up((size_t)H.n * R * cplx_bytes(H.vtype), 256);
+    L.nchunks = step_chunks(H, R, dev);
+    L.b0 = 0;
+    L.b1 = vec;
+    L.ring = 2 * vec;
+    L.total = L.ring + align_up((size_t)kRing * n_left * L.nchunks * R * sizeof(double2), 256);
+    return L;
+}
+
+}  // namespace
+
+void set_error(const std::string &msg) { g_err = msg; }
+void clear_error() { g_err.clear(); }
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    g_err = std::string(what) + " failed: " + hipGetErrorString(e) + " (" + file + ":" +
+            std::to_string(line) + ")";
+    return e == hipErrorOutOfMemory ? JKPM_ERR_ALLOC : JKPM_ERR_HIP;
+}
+
+int device_info(DeviceInfo *out) {
+    static std::mutex m;
+    static std::vector<DeviceInfo> cache;
+    static std::vector<bool> have;
+    int dev = 0;
+    JKPM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(m);
+    if ((int)cache.size() <= dev) {
+        cache.resize(dev + 1);
+        have.resize(dev + 1, false);
+    }
+    if (!have[dev]) {
+        DeviceInfo d;
+        int cu = 0;
+        JKPM_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+        d.cu_count = cu > 0 ? cu : 1;
+        cache[dev] = d;
+        have[dev] = true;
+    }
+    *out = cache[dev];
+    return JKPM_OK;
+}
+
+// Runs the whole recursion for one batch; see jkpm_moments.
+static int run_moments(const Ell &H, const void *vr, const void *vl0, const void *vl1, int R,
+                       int M, double2 *mu, void *ws, size_t ws_bytes, hipStream_t s) {
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const int n_left = vl1 ? 2 : 1;
+    const MomentsLayout L = moments_layout(H, R, n_left, dev);
+    if (ws_bytes < L.total) {
+        set_error("workspace too small: need " + std::to_string(L.total) + " bytes, got " +
+                  std::to_string(ws_bytes));
+        return JKPM_ERR_WORKSPACE;
+    }
+    char *w = (char *)ws;
+    void *B[2] = {w + L.b0, w + L.b1};
+    double2 *ring = (double2 *)(w + L.ring);
+    const size_t slot = (size_t)n_left * L.nchunks * R;
+
+    // Vector groups: powers of two, largest first, so 16-byte lanes stay aligned.
+    std::vector<std::pair<int, int>> groups;  // (r0, rb)
+    for (int r0 = 0; r0 < R;) {
+        int rb = max_group(H.vtype);
+        while (rb > R - r0) rb >>= 1;
+        groups.push_back({r0, rb});
+        r0 += rb;
+    }
+
+    Prof &P = prof();
+    std::unique_lock<std::mutex> plk(P.mu);
+    const int every = P.every;
+    const bool profiling = every > 0;
+    if (!profiling) plk.unlock();
+
+    auto step = [&](int n) -> int {
+        // T_n is written to B[n & 1]; T_{n-1} is B[(n-1) & 1] (or v_r for n=1);
+        // T_{n-2} is v_r for n=2, else B[n & 1] itself (in-place update).
+        StepArgs a{};
+        a.H = &H;
+        a.first = (n == 1);
+        a.xcur = (n == 1) ? vr : B[(n - 1) & 1];
+        a.xprev = (n == 1) ? nullptr : (n == 2 ? vr : B[n & 1]);
+        a.xout = B[n & 1];
+        a.vl0 = vl0;
+        a.vl1 = vl1;
+        a.ld = R;
+        a.part = ring + (size_t)(n % kRing) * slot;
+        a.part0 = (n == 1) ? ring : nullptr;  // step 0 lives in ring slot 0
+        a.nchunks = L.nchunks;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool rec = profiling && (P.counter++ % every == 0);
+        if (rec) {
+            e0 = take_event(P);
+            e1 = take_event(P);
+            if (!e0 || !e1) return hip_fail(hipErrorOutOfMemory, "hipEventCreate", __FILE__, __LINE__);
+            JKPM_HIP(hipEventRecord(e0, s));
+        }
+        for (auto &g : groups) {
+            a.r0 = g.first;
+            a.rb = g.second;
+            int rc2 = launch_step(a, s);
+            if (rc2) return rc2;
+        }
+        if (rec) {
+            JKPM_HIP(hipEventRecord(e1, s));
+            P.pending.push_back({e0, e1});
+            if (P.pending.size() >= 8192) {
+                int rc3 = drain_profile(P);
+                if (rc3) return rc3;
+            }
+        }
+        return JKPM_OK;
+    };
+
+    // Step 1 also yields mu_0; M == 1 still runs it (T_1 is discarded).
+    int pending_lo = 0;
+    for (int n = 1; n < std::max(M, 2); ++n) {
+        rc = step(n);
+        if (rc) return rc;
+        const int last = std::min(n, M - 1);
+        if (n == std::max(M, 2) - 1 || n - pending_lo + 1 == kRing) {
+            rc = launch_reduce(ring, kRing, pending_lo, last - pending_lo + 1, n_left, L.nchunks,
+                               R, mu, M, s);
+            if (rc) return rc;
+            pending_lo = n + 1;
+        }
+    }
+    return JKPM_OK;
+}
+
+static int validate_vtype(int vtype) {
+    if (vtype != JKPM_C64 && vtype != JKPM_C128)
+        return arg_error("value_type must be JKPM_C128 (0) or JKPM_C64 (1)");
+    return JKPM_OK;
+}
+
+static int upload_values(Ell *H, const double *data, double scale, const int *idx_dev,
+                         int src_on_device, hipStream_t s) {
+    const size_t tot = (size_t)H->n * H->k;
+    const double *src = data;
+    double *staging = nullptr;
+    if (!src_on_device) {
+        JKPM_HIP(hipMalloc(&staging, tot * 16));
+        JKPM_HIP(hipMemcpyAsync(staging, data, tot * 16, hipMemcpyHostToDevice, s));
+        src = staging;
+    }
+    int rc = launch_ell_pack(src, scale, H, idx_dev, s);
+    if (staging) {
+        hipError_t e = hipStreamSynchronize(s);
+        hipFree(staging);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
+    }
+    return rc;
+}
+
+}  // namespace jkpm
+
+using namespace jkpm;
+
+extern "C" {
+
+int jkpm_version(void) { return 100; }
+
+const char *jkpm_last_error(void) { return g_err.c_str(); }
+
+int jkpm_ell_create(const double *data, const int *indices, int n_rows, int len_row,
+                    double scale, int value_type, int src_on_device, void *stream,
+                    jkpm_ell **out) {
+    clear_error();
+    if (!out) return arg_error("out is NULL");
+    *out = nullptr;
+    if (!data || !indices) return arg_error("data/indices is NULL");
+    if (n_rows <= 0 || len_row <= 0) return arg_error("n_rows and len_row must be positive");
+    if ((long long)n_rows * len_row > (1LL << 31) - 1)
+        return arg_error("n_rows*len_row exceeds 2^31-1 slots");
+    int rc = validate_vtype(value_type);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t tot = (size_t)n_rows * len_row;
+
+    if (!src_on_device) {
+        for (size_t t = 0; t < tot; ++t)
+            if (indices[t] < 0 || indices[t] >= n_rows)
+                return arg_error("column index " + std::to_string(indices[t]) + " at slot " +
+                                 std::to_string(t) + " is outside [0, n_rows)");
+    }
+
+    auto *h = new (std::nothrow) jkpm_ell;
+    if (!h) return arg_error("host allocation failed");
+    Ell &H = h->e;
+    H.n = n_rows;
+    H.k = len_row;
+    H.vtype = value_type;
+    hipGetDevice(&H.device);
+    int *idx_dev = nullptr;
+    unsigned int *bad = nullptr;
+    auto fail = [&](int code) {
+        if (idx_dev && !src_on_device) hipFree(idx_dev);
+        if (bad) hipFree(bad);
+        if (H.val) hipFree(H.val);
+        if (H.col) hipFree(H.col);
+        delete h;
+        return code;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&H.val, tot * cplx_bytes(value_type))) != hipSuccess)
+        return fail(hip_fail(e, "hipMalloc(values)", __FILE__, __LINE__));
+    if ((e = hipMalloc(&H.col, tot * sizeof(int))) != hipSuccess)
+        return fail(hip_fail(e, "hipMalloc(cols)", __FILE__, __LINE__));
+    if (src_on_device) {
+        idx_dev = const_cast<int *>(indices);
+        if ((e = hipMalloc(&bad, sizeof(unsigned int))) != hipSuccess)
+            return fail(hip_fail(e, "hipMalloc(flag)", __FILE__, __LINE__));
+        if ((e = hipMemsetAsync(bad, 0, sizeof(unsigned int), s)) != hipSuccess)
+            return fail(hip_fail(e, "hipMemsetAsync", __FILE__, __LINE__));
+        const int grid = (int)((tot + 255) / 256);
+        hipLaunchKernelGGL(count_bad_indices, dim3(grid), dim3(256), 0, s, indices, (long)tot,
+                           n_rows, bad);
+        unsigned int nbad = 0;
+        if ((e = hipMemcpyAsync(&nbad, bad, sizeof(nbad), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (e = hipStreamSynchronize(s)) != hipSuccess)
+            return fail(hip_fail(e, "index validation", __FILE__, __LINE__));
+        if (nbad) {
+            set_error(std::to_string(nbad) + " column indices are outside [0, n_rows)");
+            return fail(JKPM_ERR_ARG);
+        }
+    } else {
+        if ((e = hipMalloc(&idx_dev, tot * sizeof(int))) != hipSuccess)
+            return fail(hip_fail(e, "hipMalloc(index staging)", __FILE__, __LINE__));
+        if ((e = hipMemcpyAsync(idx_dev, indices, tot * sizeof(int), hipMemcpyHostToDevice, s)) !=
+            hipSuccess)
+            return fail(hip_fail(e, "hipMemcpyAsync(indices)", __FILE__, __LINE__));
+    }
+    rc = upload_values(&H, data, scale, idx_dev, src_on_device, s);
+    if (rc) return fail(rc);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess)
+        return fail(hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__));
+    if (!src_on_device) hipFree(idx_dev);
+    idx_dev = nullptr;
+    if (bad) hipFree(bad);
+    *out = h;
+    return JKPM_OK;
+}
+
+int jkpm_ell_destroy(jkpm_ell *h) {
+    clear_error();
+    if (!h) return JKPM_OK;
+    hipFree(h->e.val);
+    hipFree(h->e.col);
+    delete h;
+    return JKPM_OK;
+}
+
+int jkpm_ell_info(const jkpm_ell *h, int *n_rows, int *len_row, int *value_type) {
+    clear_error();
+    if (!h) return arg_error("handle is NULL");
+    if (n_rows) *n_rows = h->e.n;
+    if (len_row) *len_row = h->e.k;
+    if (value_type) *value_type = h->e.vtype;
+    return JKPM_OK;
+}
+
+int jkpm_ell_set_values(jkpm_ell *h, const double *data, double scale, int src_on_device,
+                        void *stream) {
+    clear_error();
+    if (!h || !data) return arg_error("handle/data is NULL");
+    // Values only: the slot-major column array stays as uploaded.
+    int rc = upload_values(&h->e, data, scale, nullptr, src_on_device, (hipStream_t)stream);
+    if (rc) return rc;
+    JKPM_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return JKPM_OK;
+}
+
+int jkpm_aAxby(const jkpm_ell *h, const double *a, const double *b, const void *x, void *y,
+               int R, void *stream) {
+    clear_error();
+    if (!h || !a || !b || !x || !y) return arg_error("NULL argument");
+    if (R <= 0) return arg_error("R must be positive");
+    if (x == y) return arg_error("x must not alias y");
+    return launch_aaxby(h->e, make_double2(a[0], a[1]), make_double2(b[0], b[1]), x, y, R,
+                        (hipStream_t)stream);
+}
+
+int jkpm_moments_workspace(const jkpm_ell *h, int R, int n_left, size_t *bytes) {
+    clear_error();
+    if (!h || !bytes) return arg_error("NULL argument");
+    if (R <= 0) return arg_error("R must be positive");
+    if (n_left != 1 && n_left != 2) return arg_error("n_left must be 1 or 2");
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    *bytes = moments_layout(h->e, R, n_left, dev).total;
+    return JKPM_OK;
+}
+
+int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const void *v_l1, int R,
+                 int M, double *mu, void *workspace, size_t workspace_bytes, void *stream) {
+    clear_error();
+    if (!h) return arg_error("handle is NULL");
+    if (!v_r || !v_l0 || !mu || !workspace) return arg_error("NULL vector/mu/workspace");
+    if (R <= 0) return arg_error("R must be positive");
+    if (M <= 0) return arg_error("M must be positive");
+    if (!aligned16(v_r) || !aligned16(v_l0) || (v_l1 && !aligned16(v_l1)) ||
+        !aligned16(workspace) || !aligned16(mu))
+        return arg_error("device pointers must be 16-byte aligned");
+    return run_moments(h->e, v_r, v_l0, v_l1, R, M, (double2 *)mu, workspace, workspace_bytes,
+                       (hipStream_t)stream);
+}
+
+int jkpm_dos(const double *mu, int M, int R, const double *e, int n_e, int kernel, double *dos,
+             void *workspace, size_t workspace_bytes, void *stream) {
+    clear_error();
+    if (!mu || !e || !dos || !workspace) return arg_error("NULL argument");
+    if (M <= 0 || R <= 0 || n_e < 0) return arg_error("M and R must be positive, n_e >= 0");
+    if (kernel != JKPM_KERNEL_NONE && kernel != JKPM_KERNEL_JACKSON)
+        return arg_error("kernel must be JKPM_KERNEL_NONE or JKPM_KERNEL_JACKSON");
+    if (workspace_bytes < (size_t)M * sizeof(double)) {
+        set_error("workspace too small for jkpm_dos: need 8*M bytes");
+        return JKPM_ERR_WORKSPACE;
+    }
+    return launch_dos((const double2 *)mu, M, R, e, n_e, kernel, dos, (double *)workspace,
+                      (hipStream_t)stream);
+}
+
+int jkpm_profile_enable(int every) {
+    clear_error();
+    if (every < 0) return arg_error("every must be >= 0");
+    Prof &P = prof();
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.every = every;
+    P.counter = 0;
+    return JKPM_OK;
+}
+
+int jkpm_profile_read(double *step_ms, long long *step_launches, int reset) {
+    clear_error();
+    Prof &P = prof();
+    std::lock_guard<std::mutex> lk(P.mu);
+    int rc = drain_profile(P);
+    if (rc) return rc;
+    if (step_ms) *step_ms = P.ms;
+    if (step_launches) *step_launches = P.launches;
+    if (reset) {
+        P.ms = 0.0;
+        P.launches = 0;
+    }
+    return JKPM_OK;
+}
+
+int jkpm_copy(void *dst, const void *src, size_t bytes, void *stream) {
+    clear_error();
+    if (!dst || !src) return arg_error("NULL argument");
+    if (bytes % 16 || !aligned16(dst) || !aligned16(src))
+        return arg_error("jkpm_copy needs 16-byte aligned pointers and a multiple of 16 bytes");
+    return launch_copy(dst, src, bytes, (hipStream_t)stream);
+}
+
+// ---- reference drop-ins: host pointers, synchronous, complex128 ----
+
+static int rec_host(const double *data, const int *indices, int len_row, int N, double *A_vec,
+                    const double *v_l, const double *v_r, int M) {
+    if (!data || !indices || !A_vec || !v_l || !v_r) return arg_error("NULL argument");
+    if (N <= 0 || len_row <= 0 || M <= 0) return arg_error("N, len_row and M must be positive");
+    jkpm_ell *h = nullptr;
+    int rc = jkpm_ell_create(data, indices, N, len_row, 1.0, JKPM_C128, 0, nullptr, &h);
+    if (rc) return rc;
+    size_t ws_bytes = 0;
+    rc = jkpm_moments_workspace(h, 1, 1, &ws_bytes);
+    void *ws = nullptr, *vr = nullptr, *vl = nullptr, *mu = nullptr;
+    auto cleanup = [&](int code) {
+        hipFree(ws);
+        hipFree(vr);
+        hipFree(vl);
+        hipFree(mu);
+        jkpm_ell_destroy(h);
+        return code;
+    };
+    if (rc) return cleanup(rc);
+    hipError_t e;
+    const size_t vb = (size_t)N * 16;
+    if ((e = hipMalloc(&ws, ws_bytes)) != hipSuccess || (e = hipMalloc(&vr, vb)) != hipSuccess ||
+        (e = hipMalloc(&vl, vb)) != hipSuccess || (e = hipMalloc(&mu, (size_t)M * 16)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMalloc", __FILE__, __LINE__));
+    if ((e = hipMemcpy(vr, v_r, vb, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(vl, v_l, vb, hipMemcpyHostToDevice)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMemcpy H2D", __FILE__, __LINE__));
+    rc = jkpm_moments(h, vr, vl, nullptr, 1, M, (double *)mu, ws, ws_bytes, nullptr);
+    if (rc) return cleanup(rc);
+    if ((e = hipMemcpy(A_vec, mu, (size_t)M * 16, hipMemcpyDeviceToHost)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMemcpy D2H", __FILE__, __LINE__));
+    return cleanup(JKPM_OK);
+}
+
+int rec_A_tab(const double *data, const int *indices, int len_row, int N, double *A_vec,
+              const double *v, int M) {
+    clear_error();
+    return rec_host(data, indices, len_row, N, A_vec, v, v, M);
+}
+
+int rec_A_tab_2(const double *data, const int *indices, int len_row, int N, double *A_vec,
+                const double *v_l, const double *v_r, int M) {
+    clear_error();
+    return rec_host(data, indices, len_row, N, A_vec, v_l, v_r, M);
+}
+
+}  // extern "C"

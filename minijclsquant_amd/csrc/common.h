@@ -1,0 +1,102 @@
+// Internal helpers shared by the jclkpm HIP sources (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/jclkpm.h"
+
+namespace jkpm {
+
+// Thread-local error message behind jkpm_last_error().
+void set_error(const std::string &msg);
+void clear_error();
+
+// Returns JKPM_ERR_HIP and records the message when e != hipSuccess.
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define JKPM_HIP(call)                                                        \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) return ::jkpm::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define JKPM_CHECK_LAUNCH(name)                                               \
+    do {                                                                      \
+        hipError_t e_ = hipGetLastError();                                    \
+        if (e_ != hipSuccess) return ::jkpm::hip_fail(e_, name, __FILE__, __LINE__); \
+    } while (0)
+
+inline int arg_error(const std::string &msg) {
+    set_error(msg);
+    return JKPM_ERR_ARG;
+}
+
+// Device properties of the current device, cached per device.
+struct DeviceInfo {
+    int cu_count = 256;
+    int xcd_count = 8;
+};
+int device_info(DeviceInfo *out);
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Device-resident ELL matrix: slot-major values and column indices.
+struct Ell {
+    int n = 0;        // rows (= columns)
+    int k = 0;        // slots per row (len_row)
+    int vtype = JKPM_C128;
+    void *val = nullptr;   // [k][n] complex of vtype
+    int *col = nullptr;    // [k][n]
+    int device = 0;
+};
+
+// Bytes of one complex element of a value type.
+inline size_t cplx_bytes(int vtype) { return vtype == JKPM_C64 ? 8 : 16; }
+
+// Chebyshev step launch description (one launch handles a group of RB
+// vectors of a batch of `ld` interleaved vectors, starting at vector r0).
+struct StepArgs {
+    const Ell *H;
+    const void *xcur;     // T_n          [n][ld]
+    const void *xprev;    // T_{n-1}      [n][ld] (unused when first)
+    void *xout;           // T_{n+1}      [n][ld] (may equal xprev)
+    const void *vl0;      // left vectors [n][ld]
+    const void *vl1;      // second left set or nullptr
+    int ld;
+    int r0;
+    int rb;
+    bool first;           // T_1 = A T_0 and also emit <vl|T_0>
+    double2 *part;        // partials of this step: [n_left][nchunks][ld]
+    double2 *part0;       // when first: partials of step 0 (same layout)
+    int nchunks;
+};
+
+// Number of row chunks (work-groups) used for a matrix of n rows.
+int step_chunks(const Ell &H, int rb, const DeviceInfo &dev);
+
+// Largest vector group one launch handles for a value type.
+int max_group(int vtype);
+
+int launch_step(const StepArgs &a, hipStream_t s);
+
+// mu[l][step][r] = sum over chunks of part ring slots, for steps
+// [step_first, step_first + nsteps).
+int launch_reduce(const double2 *ring, int ring_slots, int step_first, int nsteps,
+                  int n_left, int nchunks, int ld, double2 *mu, int M, hipStream_t s);
+
+int launch_aaxby(const Ell &H, double2 a, double2 b, const void *x, void *y, int R,
+                 hipStream_t s);
+
+int launch_ell_pack(const double *src_rowmajor, double scale, Ell *H, const int *idx_rowmajor,
+                    hipStream_t s);
+
+int launch_dos(const double2 *mu, int M, int R, const double *e, int n_e, int kernel,
+               double *dos, double *w, hipStream_t s);
+
+int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
+
+}  // namespace jkpm

@@ -1,0 +1,304 @@
+"""ELL sparse matrix container with the reference's interface.
+
+Mirrors jclsquant/ell_matrix.py:119-258 (class ``ell_matrix``): CSR input,
+row-major ELL arrays ``data``/``indices``/``valid_row_values`` (slot j of row i
+at ``i*len_row + j``, padding repeats the row's last column with value 0 and
+valid 0, jclsquant/cython_modules/Extra_funs.pyx:81-137,182-219), symmetric
+Gershgorin bounds (jclsquant/ell_matrix.py:83-114), ``dot`` (in-place
+y = a*A*x + b*y) and ``modifier`` (functional data modifiers).
+
+The host arrays stay in the reference layout so callers see the same object.
+Compute goes to the HIP library: ``dot`` and every recursion upload the matrix
+once into a device-resident slot-major copy (cached per object, per value
+type and scale) -- the reference instead re-uploads the whole matrix on every
+GPU call (jclsquant/cuda_cython/cuda/recursions.cu:24-28).
+"""
+from __future__ import annotations
+
+import copy
+import os
+import pickle
+
+import numpy as np
+import scipy.sparse as sp
+
+__all__ = [
+    "ell_to_csr",
+    "data_indices_cython",
+    "data_indices_cython_0",
+    "has_diagonal_elements",
+    "Gershgorin",
+    "bounds",
+    "ell_matrix",
+]
+
+
+def _csr_to_ell_arrays(len_row: int, n_rows: int, indptr, extra=()):
+    """Vectorised restatement of data_indices_c / data_indices_c_0
+    (jclsquant/cython_modules/Extra_funs.pyx:81-137, 182-219).
+
+    Returns (slot positions of the CSR entries, padding positions, the column
+    to repeat on each padded slot)."""
+    indptr = np.asarray(indptr, dtype=np.int64)
+    counts = np.diff(indptr)
+    if counts.size and counts.max() > len_row:
+        raise ValueError("len_row is smaller than the longest row")
+    nnz = int(indptr[-1])
+    rows = np.repeat(np.arange(n_rows, dtype=np.int64), counts)
+    slot = np.arange(nnz, dtype=np.int64) - indptr[rows]
+    pos = rows * len_row + slot
+    return rows, counts, pos
+
+
+def _fill(len_row, n_rows, indptr, a_indices, arrays, dtypes):
+    """Builds row-major ELL index + value arrays with the reference padding."""
+    a_indices = np.asarray(a_indices)
+    rows, counts, pos = _csr_to_ell_arrays(len_row, n_rows, indptr)
+    tot = n_rows * len_row
+    indices = np.empty(tot, dtype=np.int32)
+    valid = np.zeros(tot, dtype=np.int32)
+    # padding: repeat the row's last column index (value 0, valid 0); an empty
+    # row pads with its own index (the reference would read out of bounds).
+    indptr = np.asarray(indptr, dtype=np.int64)
+    last = np.where(counts > 0, a_indices[np.maximum(indptr[1:] - 1, 0)] if a_indices.size else 0,
+                    np.arange(n_rows))
+    indices[:] = np.repeat(last.astype(np.int32), len_row)
+    indices[pos] = a_indices
+    valid[pos] = 1
+    outs = []
+    for arr, dt in zip(arrays, dtypes):
+        o = np.zeros(tot, dtype=dt)
+        o[pos] = np.asarray(arr).astype(dt, copy=False)
+        outs.append(o)
+    return indices, valid, outs
+
+
+def data_indices_cython_0(len_row, A_shape, A_indptr, A_data, A_indices, n_threads=None):
+    """CSR -> row-major ELL (jclsquant/ell_matrix.py:50-56 over
+    Extra_funs.pyx:182-219). Returns (data, indices, valid_row_values)."""
+    indices, valid, (data,) = _fill(int(len_row), int(A_shape), A_indptr, A_indices,
+                                    [A_data], [np.complex128])
+    return data, indices, valid
+
+
+def data_indices_cython(len_row, A_shape, A_indptr, A_data, A_indices, A_dx, A_dy,
+                        A_conjugates, n_threads=None):
+    """CSR -> row-major ELL with bond vectors and conjugate flags
+    (jclsquant/ell_matrix.py:36-47 over Extra_funs.pyx:81-137). Returns
+    (data, indices, dx_vec, dy_vec, conjugates, valid_row_values)."""
+    conj = np.real(np.asarray(A_conjugates)).astype(np.int32)
+    indices, valid, (data, dx, dy, cj) = _fill(
+        int(len_row), int(A_shape), A_indptr, A_indices,
+        [A_data, A_dx, A_dy, conj], [np.complex128, np.complex128, np.complex128, np.int32])
+    return data, indices, dx, dy, cj, valid
+
+
+def ell_to_csr(A):
+    """ELL -> scipy CSR (jclsquant/ell_matrix.py:15-31): every slot becomes an
+    entry at (slot // len_row, indices[slot]); padded slots add 0."""
+    n = len(A.data)
+    rows = np.arange(n) // A.len_row
+    cols = np.asarray(A.indices, dtype=np.int64)
+    shape = tuple(int(s) for s in A.shape)
+    return sp.csr_matrix((np.asarray(A.data), (rows, cols)), shape=shape)
+
+
+def has_diagonal_elements(csr) -> bool:
+    rows, cols = csr.nonzero()
+    return bool(np.any(rows == cols))
+
+
+def Gershgorin(H):
+    """Gershgorin interval of H (jclsquant/ell_matrix.py:83-93): centres are
+    the diagonal, radii the column sums of |H| minus |diag| (axis=0)."""
+    centers = H.diagonal()
+    radii = np.asarray(abs(H).sum(axis=0)).ravel() - np.abs(centers)
+    vmin = np.amin(centers - radii)
+    vmax = np.amax(centers + radii)
+    return np.array([vmin, vmax])
+
+
+def bounds(H):
+    """Symmetric spectral bounds (jclsquant/ell_matrix.py:96-114): the
+    Gershgorin interval widened by 2.5% of its range on both sides, returned
+    as [-b, b] with b = max |end point|."""
+    if not sp.isspmatrix_csr(H):
+        H = ell_to_csr(H)
+    ger = Gershgorin(H).real
+    extra = (ger[1] - ger[0]) * 0.025
+    arr = np.array([ger[0] - extra, ger[1] + extra])
+    b = np.max(np.abs(arr))
+    return np.array([-b, b])
+
+
+class ell_matrix:
+    """ELL matrix (jclsquant/ell_matrix.py:119-258).
+
+    ``ell_matrix(A, DX=None, DY=None, CJ=None, velocities='False', space='r')``
+    with ``A`` a scipy CSR matrix.  Extra keyword ``bounds`` sets the spectral
+    bounds explicitly instead of the Gershgorin default.
+    """
+
+    def __init__(self, A, DX=None, DY=None, CJ=None, velocities='False', space='r',
+                 bounds=None):
+        self.A = A
+        self.type = None
+        self.len_row = None
+        self.len_col = None
+        self.ell_data = None
+        self.ell_indices = None
+        self.shape = None
+        self.n_threads = None
+        self.DX = DX
+        self.DY = DY
+        self.CJ = CJ
+        self.dx_vec = None
+        self.dy_vec = None
+        self.dz_vec = None
+        self.Omega = None
+        self.conjugates = None
+        self.valid_row_values = None
+        self._data = None
+        self._dev = {}
+        if DX is not None and DY is not None:
+            if len(DX.data) != len(A.data) or len(DY.data) != len(A.data):
+                print('BE CAREFULL THE NNZ OF THE DX OR DY AND H ARE NOT THE SAME')
+        self.set_n_threads()
+        self.check_format()
+        self.bounds = _bounds_default(self.A) if bounds is None else np.asarray(bounds, float)
+        self.space = space
+        self.velocities = velocities
+
+    # The value array: reassigning it drops the cached device copies.
+    @property
+    def data(self):
+        return self._data
+
+    @data.setter
+    def data(self, value):
+        self._data = value
+        self._dev = {}
+
+    def check_format(self):
+        if sp.isspmatrix_csr(self.A):
+            self.csr_to_ell()
+            self.type = 'ELLPACK(ELL) sparse matrix format'
+        elif getattr(self.A, 'type', None) == 'ELLPACK(ELL) sparse matrix format':
+            print('Matrix is already in ELL format')
+        else:
+            print("Matrix is not in CSR or in ELL")
+
+    def set_n_threads(self):
+        if self.n_threads is None:
+            env = os.getenv("OMP_NUM_THREADS")
+            self.n_threads = 8 if env is None else int(env)
+
+    def csr_to_ell(self):
+        A = self.A
+        counts = np.diff(A.indptr)
+        self.len_row = int(counts.max()) if counts.size else 0
+        self.shape = np.array([A.shape[0], A.shape[0]], dtype=np.int32)
+        self.len_col = self.shape[0]
+        if self.DX is None and self.DY is None:
+            self.data, self.indices, self.valid_row_values = data_indices_cython_0(
+                self.len_row, A.shape[0], A.indptr, A.data, A.indices, self.n_threads)
+        else:
+            cj = self.DX.data if self.CJ is None else self.CJ.data
+            (self.data, self.indices, self.dx_vec, self.dy_vec, self.conjugates,
+             self.valid_row_values) = data_indices_cython(
+                self.len_row, A.shape[0], A.indptr, A.data, A.indices, self.DX.data,
+                self.DY.data, cj, self.n_threads)
+
+    def save(self, path=None):
+        """Pickles the object (the reference uses joblib.dump,
+        jclsquant/ell_matrix.py:202-207)."""
+        if path is None:
+            print('None path selected save as ./H.json')
+            path = './H.json'
+        state = self.__getstate__()
+        with open(path, 'wb') as f:
+            pickle.dump(state, f)
+
+    def __getstate__(self):
+        st = dict(self.__dict__)
+        st['_dev'] = {}
+        return st
+
+    def __setstate__(self, st):
+        self.__dict__.update(st)
+        self._dev = {}
+
+    def deep_copy(self):
+        return copy.deepcopy(self)
+
+    def __deepcopy__(self, memo):
+        new = ell_matrix.__new__(ell_matrix)
+        for k, v in self.__dict__.items():
+            new.__dict__[k] = {} if k == '_dev' else copy.deepcopy(v, memo)
+        return new
+
+    def modifier(self, func, *args, **kwargs):
+        """new object whose data is func(self, ...) (jclsquant/ell_matrix.py:218-223)."""
+        new_data = func(self, *args, **kwargs)
+        new_matrix = self.deep_copy()
+        new_matrix.data = new_data
+        return new_matrix
+
+    def modifier_nocop(self, other, func, *args, **kwargs):
+        """In-place variant (jclsquant/ell_matrix.py:225-245)."""
+        data_new = np.ascontiguousarray(self.data.copy())
+        dx_new = np.ascontiguousarray(self.dx_vec.copy())
+        dy_new = np.ascontiguousarray(self.dy_vec.copy())
+        result = func(other, data_new, dx_new, dy_new, *args, **kwargs)
+        if isinstance(result, tuple):
+            self.data, self.dx_vec, self.dy_vec = result[0], result[1], result[2]
+        else:
+            self.data = result
+
+    # ------------------------------------------------------------ device side
+
+    def device(self, dtype='c128', scale=1.0):
+        """Device-resident slot-major copy of this matrix (values * scale, cast
+        to dtype), cached until ``data`` is reassigned or ``invalidate_device``
+        is called.  In-place edits of ``data`` need ``invalidate_device()``."""
+        from ._native import DeviceELL, vtype_of
+
+        key = (vtype_of(dtype), float(scale))
+        h = self._dev.get(key)
+        if h is None:
+            h = DeviceELL(self.data, self.indices, int(self.shape[0]), int(self.len_row),
+                          scale=scale, dtype=dtype)
+            self._dev[key] = h
+        return h
+
+    def invalidate_device(self):
+        self._dev = {}
+
+    def dot(self, a, b, x, y):
+        """y <- a*A*x + b*y in place on complex128 numpy vectors
+        (jclsquant/ell_matrix.py:209-211 over blas_funs.aAxby), computed by the
+        HIP library."""
+        from ._native import require_device
+
+        torch = require_device()
+        if not (isinstance(y, np.ndarray) and y.dtype == np.complex128 and y.flags.c_contiguous):
+            raise ValueError("y must be a C-contiguous complex128 numpy array")
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        n = int(self.shape[0])
+        if x.shape[0] != n or y.shape[0] != n:
+            raise ValueError("x and y must have one entry per row")
+        h = self.device('c128', 1.0)
+        xd = torch.from_numpy(x.reshape(n, -1)).cuda()
+        yd = torch.from_numpy(y.reshape(n, -1)).cuda()
+        h.aAxby(complex(a), complex(b), xd, yd)
+        y[...] = yd.cpu().numpy().reshape(y.shape)
+
+    def __copy__(self):
+        new = ell_matrix.__new__(ell_matrix)
+        new.__dict__.update(self.__dict__)
+        new._dev = {}
+        return new
+
+
+def _bounds_default(A):
+    return bounds(A)

@@ -1,0 +1,173 @@
+"""Host logic of the package (CPU only): ELL conversion, bounds, lattice
+builders, modifiers, kernels -- checked against the oracle and closed forms."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import minijclsquant_amd as jcl
+from oracle import oracle as O
+
+
+def ragged_csr(N=80, seed=0, kmax=7):
+    rng = np.random.default_rng(seed)
+    nnz = rng.integers(1, kmax, N)
+    rows = np.repeat(np.arange(N), nnz)
+    cols = rng.integers(0, N, rows.size)
+    H = sp.coo_matrix((rng.normal(size=rows.size) + 1j * rng.normal(size=rows.size),
+                       (rows, cols)), shape=(N, N)).tocsr()
+    H.sum_duplicates()
+    return H
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: jcl.square_lattice(9, W=1.0, seed=1)[0],
+    lambda: jcl.honeycomb_lattice(5, W=1.0, seed=2)[0],
+    lambda: ragged_csr(),
+])
+def test_ell_conversion_matches_oracle(maker):
+    H = maker()
+    A = jcl.ell_matrix(H)
+    d, i, v, K = O.csr_to_ell(H)
+    assert A.len_row == K
+    assert np.array_equal(A.data, d)
+    assert np.array_equal(A.indices, i)
+    assert np.array_equal(A.valid_row_values, v)
+    assert A.indices.dtype == np.int32 and A.data.dtype == np.complex128
+    back = jcl.ell_to_csr(A)
+    assert abs(back - H).max() == 0
+
+
+def test_ell_with_bond_vectors():
+    H, Dx, Dy = jcl.honeycomb_lattice(4, W=0.5, seed=0)
+    A = jcl.ell_matrix(H, Dx, Dy)
+    # dx_vec/dy_vec sit in the same slots as data; padding is zero
+    for M, vec in ((Dx, A.dx_vec), (Dy, A.dy_vec)):
+        rows = np.arange(A.data.size) // A.len_row
+        dense = M.toarray()
+        ok = A.valid_row_values == 1
+        assert np.allclose(vec[ok], dense[rows[ok], A.indices[ok]])
+        assert np.all(vec[~ok] == 0)
+
+
+def test_bounds_gershgorin_hand_computed():
+    # clean square lattice t=-1: centres 0, radii 4 -> [-4, 4] widened by 2.5% of 8
+    H, _ = jcl.square_lattice(8, W=0.0)
+    b = jcl.bounds(H)
+    assert np.allclose(b, [-4.2, 4.2], rtol=0, atol=1e-14)
+    # onsite +/-0.5 extremes: centre e_i, radius 4 -> [min e - 4, max e + 4]
+    eps = np.linspace(-0.5, 0.5, 64)
+    H, _ = jcl.square_lattice(8, onsite=eps)
+    lo, hi = -4.5, 4.5
+    ext = (hi - lo) * 0.025
+    assert np.allclose(jcl.bounds(H), [-(hi + ext), hi + ext], atol=1e-12)
+    # column sums (axis=0), not row sums: an asymmetric matrix tells them apart
+    A = sp.csr_matrix(np.array([[0, 3.0], [1.0, 0]], dtype=np.complex128))
+    g = jcl.Gershgorin(A)
+    # column sums are (1, 3): discs [-1, 1] and [-3, 3]; row sums would give (3, 1)
+    # in the other order, so only the per-column radii give [-3, 3] with centre 0
+    assert np.allclose(g.real, [-3.0, 3.0])
+    B = sp.csr_matrix(np.array([[1.0, 3.0], [0.0, -1.0]], dtype=np.complex128))
+    # columns: |3| in column 1 -> disc of the -1 centre is [-4, 2]; rows would
+    # put the radius 3 on the +1 centre instead ([-2, 4])
+    assert np.allclose(jcl.Gershgorin(B).real, [-4.0, 2.0])
+
+
+def test_square_lattice_spectrum():
+    L = 6
+    H, _ = jcl.square_lattice(L, t=-1.0, W=0.0)
+    k = 2 * np.pi * np.arange(L) / L
+    ek = np.sort((-2 * (np.cos(k)[:, None] + np.cos(k)[None, :])).ravel())
+    assert np.allclose(np.linalg.eigvalsh(H.toarray()), ek, atol=1e-12)
+    assert H.shape == (36, 36) and np.diff(H.indptr).max() == 5
+
+
+def test_honeycomb_spectrum_and_velocity():
+    L, t = 6, -2.7
+    H, Dx, Dy = jcl.honeycomb_lattice(L, t=t, W=0.0)
+    k = 2 * np.pi * np.arange(L) / L
+    f = 1 + np.exp(-1j * k)[:, None] + np.exp(-1j * k)[None, :]
+    e = np.sort(np.concatenate([np.abs(t * f).ravel(), -np.abs(t * f).ravel()]))
+    assert np.allclose(np.linalg.eigvalsh(H.toarray()), e, atol=1e-10)
+    assert np.diff(H.indptr).max() == 4
+    # bond vectors are antisymmetric with |d| = a_cc, and v = -i H.*Dx is Hermitian
+    assert abs(Dx + Dx.T).max() == 0 and abs(Dy + Dy.T).max() == 0
+    Dd = np.sqrt(np.abs(Dx.toarray()) ** 2 + np.abs(Dy.toarray()) ** 2)
+    off = H.toarray() != 0
+    np.fill_diagonal(off, False)
+    assert np.allclose(Dd[off], jcl.A_CC)
+    A = jcl.ell_matrix(H, Dx, Dy)
+    v = jcl.modifier_velocity(A, 'x')
+    V = sp.csr_matrix((v, (np.arange(v.size) // A.len_row, A.indices)), shape=H.shape)
+    assert abs(V - V.getH()).max() <= 1e-15
+    assert np.array_equal(v, O.vec_mul(-1j, A.data, A.dx_vec))
+
+
+def test_open_boundaries():
+    H, _ = jcl.square_lattice(5, periodic=False)
+    assert np.diff(H.indptr).min() == 3  # corner: diagonal + 2 bonds
+    Hh, Dx, _ = jcl.honeycomb_lattice(4, periodic=False)
+    assert abs(Hh - Hh.getH()).max() == 0
+    assert np.diff(Hh.indptr).min() >= 2
+
+
+def test_modifiers_match_oracle():
+    H, _ = jcl.square_lattice(7, W=1.0, seed=5)
+    A = jcl.ell_matrix(H)
+    B = A.bounds[1]
+    scaled = jcl.modifier_bounds(A)
+    ref = A.data.copy()
+    O.scal(1 / ((A.bounds[1] - A.bounds[0]) / 2), ref)
+    assert np.array_equal(scaled, ref)
+    assert np.allclose(scaled, A.data / B, rtol=1e-15)
+    diag = np.random.default_rng(1).normal(size=A.shape[0])
+    got = jcl.modifier_diagonal(A, None, diag)
+    want = O.diagonal_modifier(A.data, A.indices, A.valid_row_values, diag, A.len_row)
+    assert np.array_equal(got, want)
+    got = jcl.modifier_diagonal_anderson(A, 0.5, seed=3)
+    eps = 0.5 * (2 * np.random.default_rng(3).random(A.shape[0]) - 1)
+    want = O.diagonal_modifier(A.data, A.indices, A.valid_row_values, eps, A.len_row)
+    assert np.array_equal(got, want)
+    # asymmetric bounds: diagonal shift by -(b1+b0)/2, then scale
+    A2 = A.deep_copy()
+    A2.bounds = np.array([-3.0, 5.0])
+    got = jcl.modifier_bounds(A2)
+    shift = O.diagonal_modifier(A2.data, A2.indices, A2.valid_row_values,
+                                np.full(A2.shape[0], -1.0), A2.len_row)
+    assert np.allclose(got, shift / 4.0, rtol=1e-15, atol=0)
+
+
+def test_kernel_and_moments_match_oracle():
+    for M in (1, 3, 64, 1000):
+        assert np.max(np.abs(jcl.JacksonKernel(M) - O.jackson(M))) <= 1e-14
+    for e in (-0.9, 0.0, 0.37):
+        assert np.max(np.abs(jcl.moments_delta(e, 50) - O.moments_delta(e, 50))) <= 1e-13
+    with pytest.raises(ValueError):
+        jcl.moments_delta(1.5, 10)
+
+
+def test_random_vector_generator():
+    r = jcl.random_vector_generator(1000, 42)
+    assert np.allclose(np.abs(r), 1.0)
+    assert np.array_equal(r, jcl.random_vector_generator(1000, 42))
+    ph = np.exp(1j * np.random.default_rng(42).uniform(0, 2 * np.pi, 1000))
+    assert np.array_equal(r, ph)
+    assert jcl.random_vector_generator((3, 10), 1).shape == (3, 10)
+
+
+def test_modifier_returns_new_object_and_drops_device_cache():
+    H, _ = jcl.square_lattice(4)
+    A = jcl.ell_matrix(H)
+    A._dev[("sentinel",)] = object()
+    B = A.modifier(jcl.modifier_bounds)
+    assert B is not A and B._dev == {} and ("sentinel",) in A._dev
+    A.data = A.data.copy()
+    assert A._dev == {}
+    assert np.array_equal(B.indices, A.indices)
+
+
+def test_recursion_api_argument_behaviour(capsys):
+    H, _ = jcl.square_lattice(4)
+    A = jcl.ell_matrix(H)
+    assert jcl.rec_A_tab(10, A) is None
+    assert "must introduce" in capsys.readouterr().out
+    assert jcl.rec_A_tab2v(10, A, None, None) is None

@@ -1,0 +1,287 @@
+"""Pins the CPU oracle (oracle/kpm_oracle.c) to what the mathematics fixes.
+
+The reference ships no golden vectors or known-answer tests for this path,
+so every oracle function is checked against an independent route:
+  rec_A_tab*      eigendecomposition T_n(A) = V cos(n acos L) V^-1 (eigh for
+                  Hermitian A, eig for a non-Hermitian one), diagonal closed form
+  JacksonKernel   normalised autocorrelation of sin(pi (v+1)/(M+1))
+  moments_delta   numpy's Clenshaw chebval
+  DOS             Gauss-Chebyshev quadrature sum rule (= g_0 mu_0 exactly),
+                  positivity of the Jackson-damped estimate
+  csr_to_ell      round trip to CSR + the padding rule
+  aAxby/vdot/dot  scipy / numpy products
+  mu_nm           dense eigh brute force, Hermitian symmetry, v = I identity
+  grad_values     central finite differences in complex128
+Each check is chosen so a dropped term, a wrong sign/index or a transposed
+operand fails it (see the planted-mistake tests at the end).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from numpy.polynomial import chebyshev as C
+
+import minijclsquant_amd as jcl
+from oracle import oracle as O
+
+
+def cheb_dense(A, n_max):
+    """T_n(A) for n < n_max via eigendecomposition (complex arccos)."""
+    lam, V = np.linalg.eig(A)
+    Vi = np.linalg.inv(V)
+    th = np.arccos(lam.astype(np.complex128))
+    return [V @ np.diag(np.cos(n * th)) @ Vi for n in range(n_max)]
+
+
+def herm_lattice(L=10, W=1.5, seed=0):
+    H, _ = jcl.square_lattice(L, W=W, seed=seed)
+    B = (4 + W / 2) * 1.01
+    return H / B
+
+
+def ell_of(Hs):
+    d, i, v, K = O.csr_to_ell(sp.csr_matrix(Hs))
+    return d, i, v, K
+
+
+def test_rec_A_tab_vs_eigh():
+    Ht = herm_lattice()
+    d, i, _, K = ell_of(Ht)
+    N = Ht.shape[0]
+    r = jcl.random_vector_generator(N, 4)
+    lam, U = np.linalg.eigh(Ht.toarray())
+    w = np.abs(U.conj().T @ r) ** 2
+    M = 300
+    exact = np.array([np.sum(w * np.cos(n * np.arccos(np.clip(lam, -1, 1)))) for n in range(M)])
+    mu = O.rec_A_tab(M, d, i, K, r)
+    assert np.max(np.abs(mu - exact)) / N <= 1e-12
+
+
+def test_rec_A_tab_diagonal_closed_form():
+    N = 50
+    x = np.linspace(-0.97, 0.95, N)
+    Hs = sp.diags(x).tocsr().astype(np.complex128)
+    d, i, _, K = ell_of(Hs)
+    r = jcl.random_vector_generator(N, 1) * np.linspace(0.5, 1.5, N)
+    M = 400
+    exact = np.array([np.sum(np.abs(r) ** 2 * np.cos(n * np.arccos(x))) for n in range(M)])
+    assert np.max(np.abs(O.rec_A_tab(M, d, i, K, r) - exact)) <= 1e-11
+
+
+def test_rec_A_tab2v_non_hermitian_vs_eig():
+    rng = np.random.default_rng(3)
+    N = 30
+    S = np.eye(N) + 0.2 * (rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))) / np.sqrt(N)
+    D = np.diag(rng.uniform(-0.9, 0.9, N))
+    A = S @ D @ np.linalg.inv(S)
+    d, i, _, K = ell_of(A)
+    vl = rng.normal(size=N) + 1j * rng.normal(size=N)
+    vr = rng.normal(size=N) + 1j * rng.normal(size=N)
+    M = 60
+    T = cheb_dense(A, M)
+    exact = np.array([np.vdot(vl, T[n] @ vr) for n in range(M)])
+    mu = O.rec_A_tab2v(M, d, i, K, vl, vr)
+    assert np.max(np.abs(mu - exact)) <= 1e-9 * np.max(np.abs(exact))
+    m1, m2 = O.rec_A_tab2v_2(M, d, i, K, vl, vr)
+    assert np.max(np.abs(m1 - mu)) <= 1e-12 * np.max(np.abs(mu))
+    exact2 = np.array([np.vdot(vr, T[n] @ vr) for n in range(M)])
+    assert np.max(np.abs(m2 - exact2)) <= 1e-9 * np.max(np.abs(exact2))
+
+
+def test_jackson_autocorrelation():
+    for M in (1, 2, 7, 64, 513):
+        a = np.sin(np.pi * (np.arange(M) + 1) / (M + 1))
+        ac = np.array([np.dot(a[:M - k], a[k:]) for k in range(M)]) / np.dot(a, a)
+        assert np.max(np.abs(O.jackson(M) - ac)) <= 1e-13
+        assert abs(O.jackson(M)[0] - 1.0) <= 1e-14
+
+
+def test_moments_delta_vs_chebval():
+    rng = np.random.default_rng(0)
+    M = 80
+    c = rng.normal(size=M)
+    for x in (-0.93, -0.2, 0.0, 0.41, 0.999):
+        lhs = np.dot(O.moments_delta(x, M), c)
+        coef = np.concatenate(([c[0]], 2 * c[1:]))
+        rhs = C.chebval(x, coef) / (np.pi * np.sqrt(1 - x * x))
+        assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(rhs))
+
+
+def test_dos_sum_rule_and_positivity():
+    Ht = herm_lattice(12, W=2.0, seed=2)
+    d, i, _, K = ell_of(Ht)
+    N = Ht.shape[0]
+    r = jcl.random_vector_generator(N, 9)
+    M = 128
+    mu = O.rec_A_tab(M, d, i, K, r)
+    # Gauss-Chebyshev: int_{-1}^{1} DOS(x) dx = (pi/Q) sum_k DOS(x_k) sqrt(1-x_k^2)
+    Q = 2 * M
+    xk = np.cos(np.pi * (np.arange(Q) + 0.5) / Q)
+    dos = O.dos(mu, xk, -1.0, 1.0, True)
+    integral = np.pi / Q * np.sum(dos * np.sqrt(1 - xk ** 2))
+    assert abs(integral - mu[0].real) <= 1e-10 * N
+    assert abs(mu[0].real - N) <= 1e-10 * N
+    assert dos.min() >= -1e-9 * dos.max()  # Jackson damping keeps the estimate positive
+    # energies in physical units: DOS(E) with bounds [-B, B] equals DOS(E/B)
+    B = 3.5
+    # (x*B)/B differs from x by an ulp, which 1/sqrt(1-x^2) amplifies near |x| -> 1
+    assert np.allclose(O.dos(mu, xk * B, -B, B, True), dos, rtol=1e-9, atol=0)
+    # asymmetric bounds shift the centre: E~ = (E - c)/h
+    assert np.allclose(O.dos(mu, xk * B + 1.0, 1.0 - B, 1.0 + B, True), dos, rtol=1e-9, atol=0)
+
+
+def test_csr_to_ell_round_trip_and_padding():
+    rng = np.random.default_rng(5)
+    N = 60
+    nnz = rng.integers(1, 6, N)
+    rows = np.repeat(np.arange(N), nnz)
+    cols = rng.integers(0, N, rows.size)
+    Hs = sp.coo_matrix((rng.normal(size=rows.size) + 0j, (rows, cols)), shape=(N, N)).tocsr()
+    Hs.sum_duplicates()
+    d, idx, valid, K = ell_of(Hs)
+    counts = np.diff(Hs.indptr)
+    assert K == counts.max()
+    for row in range(N):
+        s = slice(row * K, row * K + K)
+        c = counts[row]
+        assert np.array_equal(idx[s][:c], Hs.indices[Hs.indptr[row]:Hs.indptr[row + 1]])
+        assert np.all(idx[s][c:] == Hs.indices[Hs.indptr[row + 1] - 1])
+        assert np.all(d[s][c:] == 0) and np.all(valid[s][c:] == 0) and np.all(valid[s][:c] == 1)
+    back = sp.csr_matrix((d, (np.arange(N * K) // K, idx)), shape=(N, N))
+    assert abs(back - Hs).max() == 0
+
+
+def test_blas_against_numpy():
+    rng = np.random.default_rng(1)
+    Ht = herm_lattice(9)
+    d, i, _, K = ell_of(Ht)
+    N = Ht.shape[0]
+    x = rng.normal(size=N) + 1j * rng.normal(size=N)
+    y = rng.normal(size=N) + 1j * rng.normal(size=N)
+    a, b = 0.3 - 1.1j, -0.7 + 0.25j
+    y2 = y.copy()
+    O.aAxby(a, b, x, y2, d, i, K)
+    assert np.max(np.abs(y2 - (a * (Ht @ x) + b * y))) <= 1e-13
+    assert abs(O.vdot(1, x, y) - np.vdot(x, y)) <= 1e-12
+    assert abs(O.vdot(2j, x, y) - 2j * np.vdot(x, y)) <= 1e-12
+    assert abs(O.dot(1, x, y) - np.dot(x, y)) <= 1e-12
+    z = x.copy()
+    O.scal(0.5 + 0.5j, z)
+    assert np.max(np.abs(z - (0.5 + 0.5j) * x)) <= 1e-15
+    assert np.max(np.abs(O.vec_mul(-1j, x, y) - (-1j * x * y))) <= 1e-15
+    diag = rng.normal(size=N) + 0j
+    _, _, valid, _ = ell_of(Ht)
+    out = O.diagonal_modifier(d, i, valid, diag, K)
+    Hd = Ht + sp.diags(diag)
+    d2, _, _, _ = ell_of(Hd)
+    assert np.max(np.abs(out - d2)) <= 1e-15
+
+
+def _velocity_pair(Lc=4, W=1.0, seed=0):
+    H, Dx, _ = jcl.honeycomb_lattice(Lc, t=-2.7, W=W, seed=seed)
+    B = (3 * 2.7 + W / 2) * 1.01
+    Ht = (H / B).tocsr()
+    V = (-1j * Dx.multiply(H)).tocsr()  # v_x = -i H (.) dx
+    V.sort_indices()
+    return Ht, V
+
+
+def test_mu_nm_vs_eigh():
+    Ht, V = _velocity_pair()
+    N = Ht.shape[0]
+    hd, hi, _, hk = ell_of(Ht)
+    vd, vi, _, vk = ell_of(V)
+    r = jcl.random_vector_generator(N, 2)
+    M = 24
+    mu = O.mu_nm(M, hd, hi, hk, vd, vi, vk, r)
+    lam, U = np.linalg.eigh(Ht.toarray())
+    Vd = V.toarray()
+    T = [U @ np.diag(np.cos(n * np.arccos(np.clip(lam, -1, 1)))) @ U.conj().T for n in range(M)]
+    exact = np.array([[np.vdot(r, Vd @ T[m] @ Vd @ T[n] @ r) for n in range(M)] for m in range(M)])
+    assert np.max(np.abs(mu - exact)) <= 1e-10 * np.max(np.abs(exact))
+    # The trace (sum over all basis vectors r = e_i) is Hermitian in (m, n) by
+    # cyclicity: Tr[v T_m v T_n]^* = Tr[v T_n v T_m].  Per random vector it is not.
+    Mt = 10
+    tr = sum(O.mu_nm(Mt, hd, hi, hk, vd, vi, vk, np.eye(N)[i]) for i in range(N))
+    assert np.max(np.abs(tr - tr.conj().T)) <= 1e-10 * np.max(np.abs(tr))
+    tr_exact = np.array([[np.trace(Vd @ T[m] @ Vd @ T[n]) for n in range(Mt)] for m in range(Mt)])
+    assert np.max(np.abs(tr - tr_exact)) <= 1e-10 * np.max(np.abs(tr_exact))
+
+
+def test_mu_nm_identity_velocity():
+    """v = I: mu_mn = <r|T_m T_n|r> = (mu_{m+n} + mu_{|m-n|}) / 2."""
+    Ht, _ = _velocity_pair(5)
+    N = Ht.shape[0]
+    hd, hi, _, hk = ell_of(Ht)
+    I = sp.identity(N, dtype=np.complex128, format="csr")
+    vd, vi, _, vk = ell_of(I)
+    r = jcl.random_vector_generator(N, 3)
+    M = 20
+    mu = O.mu_nm(M, hd, hi, hk, vd, vi, vk, r)
+    m1 = O.rec_A_tab(2 * M, hd, hi, hk, r)
+    for m in range(M):
+        for n in range(M):
+            assert abs(mu[m, n] - 0.5 * (m1[m + n] + m1[abs(m - n)])) <= 1e-10 * N
+
+
+def test_grad_values_vs_finite_differences():
+    Ht, _ = _velocity_pair(3, W=1.0, seed=4)
+    d, i, _, K = ell_of(Ht)
+    N = Ht.shape[0]
+    r = jcl.random_vector_generator(N, 6)
+    M = 25
+    rng = np.random.default_rng(8)
+    c = rng.normal(size=M) + 1j * rng.normal(size=M)
+
+    def J(dd):
+        return float(np.real(np.dot(c, O.rec_A_tab(M, dd, i, K, r))))
+
+    G = O.grad_values(M, d, i, K, r, c)
+    h = 1e-6
+    for s in rng.choice(d.size, 12, replace=False):
+        for unit, part in ((1.0, "re"), (1j, "im")):
+            dp, dm = d.copy(), d.copy()
+            dp[s] += unit * h
+            dm[s] -= unit * h
+            fd = (J(dp) - J(dm)) / (2 * h)
+            an = G[s].real if part == "re" else G[s].imag
+            assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd)), (s, part, fd, an)
+
+
+# ---- planted mistakes: each pin above must reject a plausible wrong oracle ----
+
+def test_pins_reject_planted_mistakes():
+    Ht = herm_lattice(8)
+    d, i, _, K = ell_of(Ht)
+    N = Ht.shape[0]
+    r = jcl.random_vector_generator(N, 4)
+    lam, U = np.linalg.eigh(Ht.toarray())
+    w = np.abs(U.conj().T @ r) ** 2
+    M = 40
+    exact = np.array([np.sum(w * np.cos(n * np.arccos(np.clip(lam, -1, 1)))) for n in range(M)])
+    good = O.rec_A_tab(M, d, i, K, r)
+    assert np.max(np.abs(good - exact)) / N <= 1e-12
+    # wrong three-term sign (T_{n+1} = 2 A T_n + T_{n-1}) fails
+    bad = np.zeros(M, complex)
+    t0, t1 = r.copy(), Ht @ r
+    bad[0], bad[1] = np.vdot(r, t0), np.vdot(r, t1)
+    for n in range(2, M):
+        t0, t1 = t1, 2 * (Ht @ t1) + t0
+        bad[n] = np.vdot(r, t1)
+    assert np.max(np.abs(bad - exact)) / N > 1e-3
+    # transposed conjugation in the dot (<T_n|r>) fails on a non-Hermitian A
+    rng = np.random.default_rng(0)
+    A = np.diag(rng.uniform(-0.5, 0.5, 12)) + 0.3j * np.triu(rng.normal(size=(12, 12)), 1)
+    dA, iA, _, KA = ell_of(A)
+    vl, vr = rng.normal(size=12) + 1j, rng.normal(size=12) - 1j
+    T = cheb_dense(A, 10)
+    exact2 = np.array([np.vdot(vl, T[n] @ vr) for n in range(10)])
+    wrong = np.array([np.vdot(T[n] @ vr, vl) for n in range(10)])
+    assert np.max(np.abs(O.rec_A_tab2v(10, dA, iA, KA, vl, vr) - exact2)) <= 1e-10
+    assert np.max(np.abs(wrong - exact2)) > 1e-3
+    # a halved g_0 (a common Jackson convention) fails the autocorrelation pin
+    g = O.jackson(16)
+    g_bad = g.copy()
+    g_bad[0] /= 2
+    a = np.sin(np.pi * (np.arange(16) + 1) / 17)
+    ac = np.array([np.dot(a[:16 - k], a[k:]) for k in range(16)]) / np.dot(a, a)
+    assert np.max(np.abs(g_bad - ac)) > 0.1
