@@ -109,6 +109,9 @@ def main():
     ap.add_argument("--profile-every", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dos-check", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for hardware counters: 3 copies of 1 GiB with the "
+                         "library's copy kernel, then 16 moments of the workload")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -139,6 +142,18 @@ def main():
     def one_step():
         h.moments(vr, M, workspace=ws, out=mu)
         return nat.dos(mu[0], es, nat.JKPM_KERNEL_JACKSON)
+
+    if args.probe:
+        nbytes = 1 << 30
+        src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            nat.copy(dst, src, nbytes)
+        h.moments(vr, 16, workspace=ws)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "done", "copy_bytes": nbytes, "N": N, "K": K, "R": R,
+                          "step_launches": 15}), flush=True)
+        return
 
     for i in range(args.warmup):
         one_step()

@@ -124,6 +124,41 @@ int jkpm_dos(const double *mu, int M, int R, const double *e, int n_e, int kerne
              double *dos, void *workspace, size_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------------------------
+ * Kubo double moments (no reference counterpart: the reference's nearest
+ * double expansion is position_operator, jclsquant/cuda_cython/cuda/
+ * obv.cu:562-644, and its coefficient table table_G_delta,
+ * jclsquant/cython_modules/Extra_funs.pyx:479-535, is never called).
+ * ------------------------------------------------------------------- */
+
+/* A handle with new values on the slot pattern of `pattern` (shares its
+ * column array): e.g. the velocity v_x = -i H .* dx of
+ * jclsquant/modifiers.py:56-79, whose slots are H's slots. */
+int jkpm_ell_create_like(const jkpm_ell *pattern, const double *data, double scale,
+                         int value_type, int src_on_device, void *stream, jkpm_ell **out);
+
+/* Workspace bytes of jkpm_kubo_mu_nm: two complex64 stacks of M*N*R,
+ * two recursion vectors and the FP32 split-K partials. */
+int jkpm_kubo_workspace(const jkpm_ell *H, int R, int M, size_t *bytes);
+
+/* mu[m][n] = (1/R) sum_r <r| v T_m(H) v T_n(H) |r>, complex128 [M][M], for
+ * Hermitian H and v (both JKPM_C64 handles, v made with jkpm_ell_create_like(H)),
+ * r: device complex64 [N][R].  Stacks L_m = T_m(H)(v r), Rk_n = v T_n(H) r in
+ * complex64; mu = L^H Rk on FP32 MFMA (split-K) with an FP64 combine. */
+int jkpm_kubo_mu_nm(const jkpm_ell *H, const jkpm_ell *V, const void *r, int R, int M,
+                    double *mu, void *workspace, size_t workspace_bytes, void *stream);
+
+/* sigma[i] = sum_mn (a_m b_n + c_m d_n) mu[m][n] (complex128) at rescaled
+ * energies e[i] (|e| < 1), M <= 2048, with g = Jackson,
+ * delta_m = (2-[m==0]) cos(m acos e)/(pi s), G_m = -2 sin(m acos e)/s,
+ * s = sqrt(1-e^2):
+ *   kind 0 (Kubo-Greenwood T=0):  a = b = g delta, c = d = 0, i.e.
+ *          Tr[v delta(e-H) v delta(e-H)] * bound^2;
+ *   kind 1 (the table_G_delta contraction, Extra_funs.pyx:430-535):
+ *          a = d = g G / bound, b = g delta, c = -g delta. */
+int jkpm_kubo_sigma(const double *mu, int M, const double *e, int n_e, int kind,
+                    double bound, double *sigma, void *stream);
+
+/* ---------------------------------------------------------------------
  * Measurement support.
  * ------------------------------------------------------------------- */
 

@@ -8,6 +8,7 @@ lib/libjclkpm_hip.so (C-ABI in include/jclkpm.h), called through ctypes.
 """
 from .ell_matrix import *  # noqa: F401,F403
 from .kernel_and_moments import *  # noqa: F401,F403
+from .kubo import *  # noqa: F401,F403
 from .lattice import *  # noqa: F401,F403
 from .modifiers import *  # noqa: F401,F403
 from .observables import *  # noqa: F401,F403

@@ -55,6 +55,10 @@ SIGNATURES = {
     "jkpm_profile_enable": (_int, [_int]),
     "jkpm_profile_read": (_int, [_dp, _c.POINTER(_c.c_longlong), _int]),
     "jkpm_copy": (_int, [_vp, _vp, _size, _vp]),
+    "jkpm_ell_create_like": (_int, [_vp, _vp, _dbl, _int, _int, _vp, _c.POINTER(_vp)]),
+    "jkpm_kubo_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
+    "jkpm_kubo_mu_nm": (_int, [_vp, _vp, _vp, _int, _int, _vp, _vp, _size, _vp]),
+    "jkpm_kubo_sigma": (_int, [_vp, _int, _vp, _int, _int, _dbl, _vp, _vp]),
 }
 
 
@@ -184,6 +188,30 @@ class DeviceELL:
                                     1 if on_dev else 0, stream_ptr(stream), _c.byref(self._h)))
         del keep
 
+    @classmethod
+    def like(cls, pattern: "DeviceELL", data, scale: float = 1.0, dtype="c64", stream=None):
+        """A handle with new row-major complex128 values on `pattern`'s slot
+        pattern (shares its column indices), e.g. a velocity operator."""
+        torch = require_device()
+        self = cls.__new__(cls)
+        self.n, self.k = pattern.n, pattern.k
+        self.vtype = vtype_of(dtype)
+        self.scale = float(scale)
+        self._h = _vp()
+        if isinstance(data, torch.Tensor):
+            d = data.contiguous()
+            if not d.is_cuda or d.dtype != torch.complex128 or d.numel() != self.n * self.k:
+                raise ValueError("values must be a complex128 device tensor of n_rows*len_row")
+            ptr, on_dev = d.data_ptr(), 1
+        else:
+            d = np.ascontiguousarray(data, dtype=np.complex128)
+            if d.size != self.n * self.k:
+                raise ValueError("values must hold n_rows*len_row entries")
+            ptr, on_dev = d.ctypes.data, 0
+        check(lib().jkpm_ell_create_like(pattern._h, ptr, self.scale, self.vtype, on_dev,
+                                         stream_ptr(stream), _c.byref(self._h)))
+        return self
+
     @property
     def handle(self):
         return self._h
@@ -294,6 +322,46 @@ def dos(mu, energies_scaled, kernel: int = JKPM_KERNEL_JACKSON, stream=None):
     ws = torch.empty(M, dtype=torch.float64, device=mu.device)
     check(lib().jkpm_dos(mu.data_ptr(), M, R, e.data_ptr(), e.numel(), int(kernel),
                          out.data_ptr(), ws.data_ptr(), ws.numel() * 8, stream_ptr(stream)))
+    return out
+
+
+def kubo_workspace_bytes(H: DeviceELL, R: int, M: int) -> int:
+    out = _size()
+    check(lib().jkpm_kubo_workspace(H.handle, int(R), int(M), _c.byref(out)))
+    return int(out.value)
+
+
+def kubo_mu_nm(H: DeviceELL, V: DeviceELL, r, M: int, workspace=None, out=None, stream=None):
+    """mu[m, n] = (1/R) sum_r <r| v T_m(H) v T_n(H) |r>, complex128 [M, M].
+    H, V: complex64 handles (V made with DeviceELL.like(H, ...)); r: device
+    complex64 [N, R]."""
+    torch = require_device()
+    if not (isinstance(r, torch.Tensor) and r.is_cuda and r.dtype == torch.complex64
+            and r.dim() == 2 and r.shape[0] == H.n and r.is_contiguous()):
+        raise ValueError(f"r must be a contiguous complex64 device tensor [N={H.n}, R]")
+    R, M = r.shape[1], int(M)
+    need = kubo_workspace_bytes(H, R, M)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=r.device)
+    if out is None:
+        out = torch.empty((M, M), dtype=torch.complex128, device=r.device)
+    check(lib().jkpm_kubo_mu_nm(H.handle, V.handle, r.data_ptr(), R, M, out.data_ptr(),
+                                workspace.data_ptr(), workspace.numel(), stream_ptr(stream)))
+    return out
+
+
+def kubo_sigma(mu, e_scaled, kind: int, bound: float, stream=None):
+    """sigma[i] = sum_mn (a_m b_n + c_m d_n) mu[m, n] at rescaled energies."""
+    torch = require_device()
+    if mu.dtype != torch.complex128 or mu.dim() != 2 or mu.shape[0] != mu.shape[1]:
+        raise ValueError("mu must be complex128 [M, M]")
+    mu = mu.contiguous()
+    e = torch.as_tensor(e_scaled, dtype=torch.float64, device=mu.device).contiguous()
+    if e.numel() and float(e.abs().max()) > 1.0:
+        raise ValueError("rescaled energies must lie in [-1, 1]")
+    out = torch.empty(e.numel(), dtype=torch.complex128, device=mu.device)
+    check(lib().jkpm_kubo_sigma(mu.data_ptr(), mu.shape[0], e.data_ptr(), e.numel(), int(kind),
+                                float(bound), out.data_ptr(), stream_ptr(stream)))
     return out
 
 

@@ -9,6 +9,7 @@
 // mu, and nothing synchronises the host until the caller does.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -17,6 +18,9 @@
 
 struct jkpm_ell {
     jkpm::Ell e;
+    // Column array ownership, shared by handles made with jkpm_ell_create_like
+    // (freed when the last handle that uses it is destroyed).
+    std::shared_ptr<int> col_owner;
 };
 
 namespace jkpm {
@@ -327,6 +331,44 @@ int jkpm_ell_create(const double *data, const int *indices, int n_rows, int len_
     if (!src_on_device) hipFree(idx_dev);
     idx_dev = nullptr;
     if (bad) hipFree(bad);
+    h->col_owner = std::shared_ptr<int>(H.col, [](int *p) { hipFree(p); });
+    *out = h;
+    return JKPM_OK;
+}
+
+int jkpm_ell_create_like(const jkpm_ell *pattern, const double *data, double scale,
+                         int value_type, int src_on_device, void *stream, jkpm_ell **out) {
+    clear_error();
+    if (!out) return arg_error("out is NULL");
+    *out = nullptr;
+    if (!pattern || !data) return arg_error("pattern/data is NULL");
+    int rc = validate_vtype(value_type);
+    if (rc) return rc;
+    auto *h = new (std::nothrow) jkpm_ell;
+    if (!h) return arg_error("host allocation failed");
+    Ell &H = h->e;
+    H.n = pattern->e.n;
+    H.k = pattern->e.k;
+    H.vtype = value_type;
+    H.device = pattern->e.device;
+    H.col = pattern->e.col;
+    h->col_owner = pattern->col_owner;
+    const size_t tot = (size_t)H.n * H.k;
+    hipError_t e = hipMalloc(&H.val, tot * cplx_bytes(value_type));
+    if (e != hipSuccess) {
+        delete h;
+        return hip_fail(e, "hipMalloc(values)", __FILE__, __LINE__);
+    }
+    rc = upload_values(&H, data, scale, nullptr, src_on_device, (hipStream_t)stream);
+    if (!rc) {
+        e = hipStreamSynchronize((hipStream_t)stream);
+        if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
+    }
+    if (rc) {
+        hipFree(H.val);
+        delete h;
+        return rc;
+    }
     *out = h;
     return JKPM_OK;
 }
@@ -335,8 +377,7 @@ int jkpm_ell_destroy(jkpm_ell *h) {
     clear_error();
     if (!h) return JKPM_OK;
     hipFree(h->e.val);
-    hipFree(h->e.col);
-    delete h;
+    delete h;  // releases this handle's share of the column array
     return JKPM_OK;
 }
 
@@ -409,6 +450,108 @@ int jkpm_dos(const double *mu, int M, int R, const double *e, int n_e, int kerne
     }
     return launch_dos((const double2 *)mu, M, R, e, n_e, kernel, dos, (double *)workspace,
                       (hipStream_t)stream);
+}
+
+// ---- Kubo double moments ----
+
+namespace {
+struct KuboLayout {
+    size_t L = 0, Rk = 0, T0 = 0, T1 = 0, part = 0, total = 0;
+    int nslices = 1, nchunks = 1;
+};
+
+KuboLayout kubo_layout(const Ell &H, int R, int M, const DeviceInfo &dev) {
+    KuboLayout K;
+    const size_t vec = (size_t)H.n * R * 8;
+    const size_t stack = align_up((size_t)M * vec, 256);
+    K.nslices = kubo_gemm_slices((long)H.n * R, M, dev);
+    K.nchunks = step_chunks(H, R, dev);
+    K.L = 0;
+    K.Rk = stack;
+    K.T0 = 2 * stack;
+    K.T1 = K.T0 + align_up(vec, 256);
+    K.part = K.T1 + align_up(vec, 256);
+    K.total = K.part + align_up((size_t)K.nslices * M * M * sizeof(float2), 256);
+    return K;
+}
+
+int kubo_check(const jkpm_ell *H, const jkpm_ell *V, int R, int M) {
+    if (!H || !V) return arg_error("H/V handle is NULL");
+    if (H->e.vtype != JKPM_C64 || V->e.vtype != JKPM_C64)
+        return arg_error("the Kubo path needs complex64 handles (JKPM_C64) for H and v");
+    if (V->e.col != H->e.col)
+        return arg_error("v must share H's slot pattern: create it with jkpm_ell_create_like(H, ...)");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    return JKPM_OK;
+}
+}  // namespace
+
+int jkpm_kubo_workspace(const jkpm_ell *H, int R, int M, size_t *bytes) {
+    clear_error();
+    if (!H || !bytes) return arg_error("NULL argument");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    *bytes = kubo_layout(H->e, R, M, dev).total;
+    return JKPM_OK;
+}
+
+int jkpm_kubo_mu_nm(const jkpm_ell *H, const jkpm_ell *V, const void *r, int R, int M, double *mu,
+                    void *workspace, size_t workspace_bytes, void *stream) {
+    clear_error();
+    int rc = kubo_check(H, V, R, M);
+    if (rc) return rc;
+    if (!r || !mu || !workspace) return arg_error("NULL vector/mu/workspace");
+    if (!aligned16(r) || !aligned16(mu) || !aligned16(workspace))
+        return arg_error("device pointers must be 16-byte aligned");
+    DeviceInfo dev;
+    rc = device_info(&dev);
+    if (rc) return rc;
+    const KuboLayout K = kubo_layout(H->e, R, M, dev);
+    if (workspace_bytes < K.total) {
+        set_error("workspace too small: need " + std::to_string(K.total) + " bytes");
+        return JKPM_ERR_WORKSPACE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    char *w = (char *)workspace;
+    const size_t vec = (size_t)H->e.n * R * 8;
+    char *Ls = w + K.L, *Rs = w + K.Rk;
+    void *Tb[2] = {w + K.T0, w + K.T1};
+    // Right recursion T_n(H) r (T_0 = r, T_n in Tb[n & 1]); the fused dual
+    // output is Rk_n = v T_n.
+    for (int n = 0; n < M; ++n) {
+        const void *x = n == 0 ? r : Tb[n & 1];
+        const void *prev = n == 1 ? r : Tb[(n + 1) & 1];
+        rc = launch_kubo_step(H->e, &V->e, x, n == 0 ? nullptr : prev, Tb[(n + 1) & 1],
+                              Rs + (size_t)n * vec, R, n == 0 ? 1.f : 2.f, n == 0 ? 0.f : -1.f,
+                              K.nchunks, s);
+        if (rc) return rc;
+    }
+    // Left recursion L_m = T_m(H) (v r), L_0 = Rk_0, out of place into the stack.
+    JKPM_HIP(hipMemcpyAsync(Ls, Rs, vec, hipMemcpyDeviceToDevice, s));
+    for (int m = 0; m + 1 < M; ++m) {
+        rc = launch_kubo_step(H->e, nullptr, Ls + (size_t)m * vec,
+                              m == 0 ? nullptr : Ls + (size_t)(m - 1) * vec, Ls + (size_t)(m + 1) * vec,
+                              nullptr, R, m == 0 ? 1.f : 2.f, m == 0 ? 0.f : -1.f, K.nchunks, s);
+        if (rc) return rc;
+    }
+    rc = launch_kubo_gemm((const float2 *)Ls, (const float2 *)Rs, (long)H->e.n * R, M, K.nslices,
+                          (float2 *)(w + K.part), s);
+    if (rc) return rc;
+    return launch_kubo_reduce((const float2 *)(w + K.part), K.nslices, M, 1.0 / R, (double2 *)mu, s);
+}
+
+int jkpm_kubo_sigma(const double *mu, int M, const double *e, int n_e, int kind, double bound,
+                    double *sigma, void *stream) {
+    clear_error();
+    if (!mu || !e || !sigma) return arg_error("NULL argument");
+    if (M <= 0 || M > 2048) return arg_error("M must be in [1, 2048]");
+    if (n_e < 0) return arg_error("n_e must be >= 0");
+    if (kind != 0 && kind != 1) return arg_error("kind must be 0 (Kubo-Greenwood) or 1 (G-delta table)");
+    if (!(bound > 0.0)) return arg_error("bound must be positive");
+    return launch_kubo_sigma((const double2 *)mu, M, e, n_e, kind, bound, (double2 *)sigma,
+                             (hipStream_t)stream);
 }
 
 int jkpm_profile_enable(int every) {

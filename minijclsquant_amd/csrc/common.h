@@ -99,4 +99,17 @@ int launch_dos(const double2 *mu, int M, int R, const double *e, int n_e, int ke
 
 int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 
+// ---- Kubo double moments (kubo.hip) ----
+int kubo_gemm_slices(long K, int M, const DeviceInfo &dev);
+// out = alpha*H*x + beta*prev (prev unused when beta == 0); when Vop is set,
+// also wout = Vop*x on the same gathered x (Vop shares H's columns).
+int launch_kubo_step(const Ell &H, const Ell *Vop, const void *x, const void *prev, void *out,
+                     void *wout, int R, float alpha, float beta, int nchunks, hipStream_t s);
+int launch_kubo_gemm(const float2 *L, const float2 *Rk, long K, int M, int nslices, float2 *part,
+                     hipStream_t s);
+int launch_kubo_reduce(const float2 *part, int nslices, int M, double scale, double2 *mu,
+                       hipStream_t s);
+int launch_kubo_sigma(const double2 *mu, int M, const double *e, int n_e, int kind, double bound,
+                      double2 *sigma, hipStream_t s);
+
 }  // namespace jkpm

@@ -330,3 +330,51 @@ int orc_grad_values(int M, const zc *data, const int *indices, int len_row, int 
     free(lb);
     return 0;
 }
+
+/* jclsquant/cython_modules/Extra_funs.pyx:430-535 (moments_Gmin_real_c,
+ * moments_Gmas_real_c, moments_delta_c, table_G_delta_c) for an explicit list
+ * of rescaled energies (the reference uses linspace(-alpha, alpha, 2M)):
+ *   bigmas[m][n][i] = (Gmas_m(E_i) g_m / B) * (delta_n(E_i) g_n)
+ *   bigmin[m][n][i] = -(delta_m(E_i) g_m) * (Gmin_n(E_i) g_n / B)
+ * Gmas_m = 2 sin(-m acos E)/s, Gmin_m = -2 sin(m acos E)/s (index 0: 0),
+ * delta_m = [1, 2cos(m acos E)...]/(pi s), s = sqrt(1-E^2) (1e-16 if 0). */
+void orc_table_G_delta(int M, const double *E, int n_e, double bound, double *bigmas,
+                       double *bigmin) {
+    double *g = (double *)malloc(sizeof(double) * (size_t)M);
+    double *gmas = (double *)malloc(sizeof(double) * (size_t)M);
+    double *gmin = (double *)malloc(sizeof(double) * (size_t)M);
+    double *dl = (double *)malloc(sizeof(double) * (size_t)M);
+    int i, m, n;
+    if (!g || !gmas || !gmin || !dl) { free(g); free(gmas); free(gmin); free(dl); return; }
+    orc_jackson(M, g);
+    for (i = 0; i < n_e; ++i) {
+        double e = E[i];
+        double s = sqrt(1.0 - e * e), f;
+        if (s == 0) s = 1e-16;
+        f = M_PI * sqrt(1.0 - e * e);
+        if (f == 0) f = 1e-16;
+        gmin[0] = 0.0;
+        gmas[0] = 0.0;
+        dl[0] = 1.0 / f;
+        for (m = 1; m < M; ++m) {
+            gmin[m] = -2.0 * sin(m * acos(e)) / s;
+            gmas[m] = 2.0 * sin(-m * acos(e)) / s;
+            dl[m] = 2.0 * cos(m * acos(e)) / f;
+        }
+        for (m = 0; m < M; ++m) {
+            gmin[m] = gmin[m] * g[m] / bound;
+            gmas[m] = gmas[m] * g[m] / bound;
+            dl[m] = dl[m] * g[m];
+        }
+        for (n = 0; n < M; ++n)
+            for (m = 0; m < M; ++m) {
+                size_t o = ((size_t)m * M + n) * (size_t)n_e + i;
+                bigmas[o] = gmas[m] * dl[n];
+                bigmin[o] = -dl[m] * gmin[n];
+            }
+    }
+    free(g);
+    free(gmas);
+    free(gmin);
+    free(dl);
+}

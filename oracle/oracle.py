@@ -53,6 +53,7 @@ def lib():
             "orc_dos": (None, [_vp, i, _vp, i, d, d, i, _vp]),
             "orc_mu_nm": (i, [i, _vp, _vp, i, _vp, _vp, i, i, _vp, _vp, i]),
             "orc_grad_values": (i, [i, _vp, _vp, i, i, _vp, _vp, _vp, i]),
+            "orc_table_G_delta": (None, [i, _vp, i, d, _vp, _vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -202,6 +203,33 @@ def mu_nm(M, h_data, h_idx, h_k, v_data, v_idx, v_k, r, n_threads=None):
                          int(v_k), r.size, _p(r), _p(out), threads(n_threads))
     assert rc == 0
     return out
+
+
+def table_G_delta(M, E, bound):
+    """(bigmas, bigmin), each float64 [M, M, len(E)], at rescaled energies E."""
+    E = np.ascontiguousarray(E, np.float64)
+    bm = np.zeros((M, M, E.size))
+    bn = np.zeros((M, M, E.size))
+    lib().orc_table_G_delta(int(M), _p(E), E.size, float(bound), _p(bm), _p(bn))
+    return bm, bn
+
+
+def kubo_sigma(mu, E, bound, kind="greenwood"):
+    """sigma(E_i) from double moments mu [M, M] (complex).  'greenwood':
+    sum_mn a_m a_n mu_mn with a = Jackson * moments_delta; 'g_delta':
+    sum_mn (bigmas + bigmin)[m, n, i] mu_mn with the materialised table."""
+    mu = np.asarray(mu, np.complex128)
+    M = mu.shape[0]
+    E = np.asarray(E, np.float64)
+    if kind == "greenwood":
+        g = jackson(M)
+        out = np.zeros(E.size, np.complex128)
+        for i, e in enumerate(E):
+            a = moments_delta(e, M) * g
+            out[i] = a @ mu @ a
+        return out
+    bm, bn = table_G_delta(M, E, bound)
+    return np.einsum("mni,mn->i", bm + bn, mu)
 
 
 def grad_values(M, data, indices, len_row, r, c, n_threads=None):

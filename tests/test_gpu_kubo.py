@@ -1,0 +1,121 @@
+"""GPU parity of the Kubo double moments mu_mn (complex64 stacks, FP32 MFMA
+GEMM, FP64 combine) and of the sigma(E) contraction.
+
+Tolerances: mu_mn against the complex128 oracle, max |dmu| <= 2e-5 * max |mu|
+(complex64 recursion + FP32 accumulation over N*R terms); sigma kernel
+against the oracle's FP64 contraction of the same mu <= 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+import minijclsquant_amd as jcl
+from minijclsquant_amd import _native as nat
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_MU = 2e-5
+
+
+def lattice(L, W=1.0, seed=0):
+    H, Dx, Dy = jcl.honeycomb_lattice(L, t=-2.7, W=W, seed=seed)
+    A = jcl.ell_matrix(H, Dx, Dy)
+    B = (3 * 2.7 + W / 2) * 1.01
+    A.bounds = np.array([-B, B])
+    return A, B
+
+
+def vectors(N, R, seed0=0):
+    return np.stack([jcl.random_vector_generator(N, seed0 + s) for s in range(R)])
+
+
+def oracle_mu(A, B, v_data, r, M):
+    hd = A.data * (1 / B)
+    return np.mean([O.mu_nm(M, hd, A.indices, A.len_row, v_data, A.indices, A.len_row, v, )
+                    for v in r], axis=0)
+
+
+@pytest.mark.parametrize("L,M,R", [(8, 32, 2), (6, 40, 1), (10, 130, 3), (16, 64, 4)])
+def test_mu_nm_vs_oracle(torch_gpu, L, M, R):
+    A, B = lattice(L, seed=L)
+    r = vectors(A.shape[0], R, 10 * L)
+    mu = jcl.kpm_kubo_moments(A, M, r, "x").cpu().numpy()
+    ref = oracle_mu(A, B, jcl.modifier_velocity(A, "x"), r, M)
+    err = np.max(np.abs(mu - ref)) / np.max(np.abs(ref))
+    assert err <= TOL_MU, err
+
+
+def test_mu_nm_y_direction_and_single_vector(torch_gpu):
+    A, B = lattice(8, seed=3)
+    r = jcl.random_vector_generator(A.shape[0], 5)
+    mu = jcl.kpm_kubo_moments(A, 24, r, "y").cpu().numpy()
+    ref = oracle_mu(A, B, jcl.modifier_velocity(A, "y"), r[None, :], 24)
+    assert np.max(np.abs(mu - ref)) / np.max(np.abs(ref)) <= TOL_MU
+
+
+def identity_velocity(A):
+    rows = np.arange(A.data.size) // A.len_row
+    return np.where((A.indices == rows) & (A.valid_row_values == 1), 1.0 + 0j, 0.0 + 0j)
+
+
+@pytest.mark.parametrize("L,M,R", [(32, 96, 4), (512, 512, 4)])
+def test_mu_nm_identity_velocity(torch_gpu, L, M, R):
+    """v = I: mu_mn = (mu_{m+n} + mu_{|m-n|}) / 2 from the single moments.
+    (512, 512, 4) is the Kubo workload's full size: honeycomb 512 x 512 cells,
+    N = 524,288, M = 512, R = 4."""
+    A, B = lattice(L, seed=L)
+    N = A.shape[0]
+    r = vectors(N, R, 77)
+    mu2 = jcl.kpm_kubo_moments(A, M, r, v_data=identity_velocity(A)).cpu().numpy()
+    single = jcl.kpm_moments(A, 2 * M, r, dtype="c128").cpu().numpy().mean(axis=0)
+    m = np.arange(M)
+    want = 0.5 * (single[m[:, None] + m[None, :]] + single[np.abs(m[:, None] - m[None, :])])
+    err = np.max(np.abs(mu2 - want)) / N
+    assert err <= TOL_MU, err
+
+
+@pytest.mark.parametrize("kind", ["greenwood", "g_delta"])
+def test_sigma_kernel_vs_oracle(torch_gpu, kind):
+    torch = torch_gpu
+    rng = np.random.default_rng(1)
+    M, B = 96, 3.3
+    mu = rng.normal(size=(M, M)) + 1j * rng.normal(size=(M, M))
+    E = np.linspace(-0.98, 0.98, 37)
+    got = nat.kubo_sigma(torch.from_numpy(mu).cuda(), E, jcl.kubo.KINDS[kind], B).cpu().numpy()
+    want = O.kubo_sigma(mu, E, B, kind)
+    assert np.max(np.abs(got - want)) / np.max(np.abs(want)) <= 1e-10
+
+
+def test_kpm_sigma_end_to_end(torch_gpu):
+    A, B = lattice(12, seed=9)
+    r = vectors(A.shape[0], 2, 4)
+    M = 64
+    E = np.linspace(-B + 0.05, B - 0.05, 101)
+    s = jcl.kpm_sigma(A, M, r, "x", energies=E)
+    ref_mu = oracle_mu(A, B, jcl.modifier_velocity(A, "x"), r, M)
+    want = O.kubo_sigma(ref_mu, E / B, B, "greenwood").real
+    assert s.shape == (E.size, 2) and np.array_equal(s[:, 0], E)
+    assert np.max(np.abs(s[:, 1] - want)) / np.max(np.abs(want)) <= 1e-4
+    s2 = jcl.kpm_sigma(A, M, r, "x", energies=E, kind="g_delta")
+    want2 = O.kubo_sigma(ref_mu, E / B, B, "g_delta").real
+    assert np.max(np.abs(s2[:, 1] - want2)) / np.max(np.abs(want2)) <= 1e-4
+
+
+def test_kubo_argument_errors(torch_gpu):
+    torch = torch_gpu
+    A, B = lattice(4)
+    H128 = A.device("c128", 1 / B)
+    H64 = A.device("c64", 1 / B)
+    V64 = nat.DeviceELL.like(H64, jcl.modifier_velocity(A, "x"), 1.0, "c64")
+    other = nat.DeviceELL(A.data, A.indices, A.shape[0], A.len_row, 1 / B, "c64")
+    r = torch.zeros((A.shape[0], 1), dtype=torch.complex64, device="cuda")
+    V128 = nat.DeviceELL.like(H128, jcl.modifier_velocity(A, "x"), 1.0, "c128")
+    with pytest.raises(ValueError):
+        nat.kubo_mu_nm(H128, V128, r, 8)  # complex128 handles are refused
+    with pytest.raises(ValueError):
+        nat.kubo_mu_nm(H64, other, r, 8)  # v not on H's slot pattern
+    with pytest.raises(ValueError):
+        nat.kubo_sigma(torch.zeros((4, 4), dtype=torch.complex128, device="cuda"),
+                       np.array([1.5]), 0, B)
+    out = nat.kubo_mu_nm(H64, V64, r + 1, 8)
+    assert out.shape == (8, 8)

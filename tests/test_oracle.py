@@ -247,6 +247,34 @@ def test_grad_values_vs_finite_differences():
             assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd)), (s, part, fd, an)
 
 
+def test_table_G_delta_vs_chebyshev_u():
+    """G_m = -2 sin(m acos E)/sqrt(1-E^2) = -2 U_{m-1}(E) (scipy's eval_chebyu);
+    bigmas[m,n] = G_m g_m/B * delta_n g_n is rank-1 per energy and
+    bigmin[m,n] = -bigmas[n,m]."""
+    from scipy.special import eval_chebyu
+
+    M, B = 12, 2.5
+    E = np.array([-0.9, -0.31, 0.0, 0.5, 0.97])
+    bm, bn = O.table_G_delta(M, E, B)
+    g = O.jackson(M)
+    for i, e in enumerate(E):
+        G = np.array([0.0] + [-2 * eval_chebyu(m - 1, e) for m in range(1, M)])
+        dl = O.moments_delta(e, M)
+        want = np.outer(G * g / B, dl * g)
+        assert np.max(np.abs(bm[:, :, i] - want)) <= 1e-12 * np.max(np.abs(want))
+        assert np.max(np.abs(bn[:, :, i] + bm[:, :, i].T)) <= 1e-12 * np.max(np.abs(want))
+    # kubo_sigma's two routes agree: materialised table vs rank-1 forms
+    rng = np.random.default_rng(2)
+    mu = rng.normal(size=(M, M)) + 1j * rng.normal(size=(M, M))
+    s_table = O.kubo_sigma(mu, E, B, "g_delta")
+    s_rank = []
+    for i, e in enumerate(E):
+        G = np.array([0.0] + [-2 * eval_chebyu(m - 1, e) for m in range(1, M)]) * g / B
+        dl = O.moments_delta(e, M) * g
+        s_rank.append(G @ mu @ dl - dl @ mu @ G)
+    assert np.max(np.abs(s_table - np.array(s_rank))) <= 1e-11 * np.max(np.abs(s_rank))
+
+
 # ---- planted mistakes: each pin above must reject a plausible wrong oracle ----
 
 def test_pins_reject_planted_mistakes():
