@@ -53,7 +53,7 @@ def build_workload(L, M, R, seed):
     A = jcl.ell_matrix(H)
     B = (3 * 2.7 + 0.5) * 1.01
     A.bounds = np.array([-B, B])
-    N = A.shape[0]
+    N = int(A.shape[0])
     r = np.stack([jcl.random_vector_generator(N, 1000 + s) for s in range(R)])
     return A, r, B
 
@@ -78,7 +78,7 @@ def cpu_baseline(A, r, target_s=15.0):
 
     thr = int(os.environ.get("OMP_NUM_THREADS", min(os.cpu_count() or 1, 16)))
     Ht = A.data * (1 / A.bounds[1])
-    N = A.shape[0]
+    N = int(A.shape[0])
     m_probe = 16
     t0 = time.perf_counter()
     O.rec_A_tab(m_probe, Ht, A.indices, A.len_row, r[0], n_threads=thr)
@@ -130,7 +130,7 @@ def main():
     L, M, R = args.cells, args.moments, args.vectors
     log(f"building workload: honeycomb {L}x{L} cells, M={M}, R={R}, {args.dtype}")
     A, r, B = build_workload(L, M, R, seed=2024)
-    N, K = A.shape[0], A.len_row
+    N, K = int(A.shape[0]), int(A.len_row)
     h = A.device(args.dtype, 1.0 / B)
     vr = _kpm.as_batch(r, N, args.dtype)
     ws = torch.empty(h.workspace_bytes(R, 1), dtype=torch.uint8, device="cuda")

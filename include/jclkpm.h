@@ -110,6 +110,42 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
                  void *stream);
 
 /* ---------------------------------------------------------------------
+ * Batches of disorder realizations (no reference counterpart: reference
+ * callers loop over realizations and vectors serially).
+ * ------------------------------------------------------------------- */
+
+typedef struct jkpm_batch jkpm_batch;
+
+/* Describes the handle h as a block-diagonal batch of G realizations with
+ * rows [row_offsets[g], row_offsets[g+1]) (host array of G+1, strictly
+ * increasing, 0 .. N); column indices of realization g stay inside its rows.
+ * The rows are cut into work-group chunks proportional to each realization's
+ * size that never straddle two realizations.  h must outlive the batch. */
+int jkpm_batch_create(const jkpm_ell *h, int G, const long long *row_offsets,
+                      jkpm_batch **out);
+int jkpm_batch_destroy(jkpm_batch *b);
+int jkpm_batch_workspace(const jkpm_batch *b, int R, int n_left, size_t *bytes);
+
+/* jkpm_moments for every realization at once, one kernel launch per
+ * Chebyshev step for the whole batch: mu[l][g][m][r] (complex128). */
+int jkpm_batch_moments(const jkpm_batch *b, const void *v_r, const void *v_l0,
+                       const void *v_l1, int R, int M, double *mu, void *workspace,
+                       size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------
+ * HIP graphs: capture the launches of asynchronous entry points (moments,
+ * batch moments, DOS, ...) issued on `stream` between begin and end, then
+ * replay them with one launch.  Pointers and sizes are frozen at capture.
+ * ------------------------------------------------------------------- */
+
+typedef struct jkpm_graph jkpm_graph;
+
+int jkpm_graph_begin(void *stream);
+int jkpm_graph_end(void *stream, jkpm_graph **out);
+int jkpm_graph_launch(const jkpm_graph *g, void *stream);
+int jkpm_graph_destroy(jkpm_graph *g);
+
+/* ---------------------------------------------------------------------
  * DOS reconstruction.
  * ------------------------------------------------------------------- */
 
@@ -122,6 +158,33 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
  * workspace: >= 8*M bytes. */
 int jkpm_dos(const double *mu, int M, int R, const double *e, int n_e, int kernel,
              double *dos, void *workspace, size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Gradients (no reference counterpart: the reference computes no
+ * derivatives).
+ * ------------------------------------------------------------------- */
+
+/* Workspace bytes of jkpm_moments_grad for segment length seg (<= 0:
+ * ceil(sqrt(M-1))): 2*ceil((M-1)/seg) checkpoint vectors, 2*seg segment
+ * vectors, 4 rolling vectors (batches of R) and the N*K slot gradient. */
+int jkpm_grad_workspace(const jkpm_ell *h, int R, int M, int seg, size_t *bytes);
+
+/* Adjoint of jkpm_moments (one left set, v_l = v_r = r): given
+ * grad_mu[m][r] = dL/dRe(mu) + i dL/dIm(mu) (device complex128 [M][R]),
+ * writes grad_values = dL/dRe(data) + i dL/dIm(data) for the row-major
+ * complex128 values the handle was created from (device, N*K).  Reverse
+ * Chebyshev recursion lambda_k = g_k r + 2 A lambda_{k+1} - lambda_{k+2}
+ * (A^H = A: Hermitian matrices) with checkpoints every seg steps and
+ * recomputation of each segment.  r: device [N][R] of the handle's type. */
+int jkpm_moments_grad(const jkpm_ell *h, const void *r, int R, int M, const double *grad_mu,
+                      int seg, double *grad_values, void *workspace, size_t workspace_bytes,
+                      void *stream);
+
+/* Transpose of jkpm_dos: grad_mu[n][r] = (1/R) g_n (2-[n==0])
+ * sum_i w[i] cos(n acos e_i) / (pi sqrt(1-e_i^2)) (imaginary part 0), for
+ * w = dL/d dos.  grad_mu: device complex128 [M][R]. */
+int jkpm_dos_grad(const double *w, int M, int R, const double *e, int n_e, int kernel,
+                  double *grad_mu, void *stream);
 
 /* ---------------------------------------------------------------------
  * Kubo double moments (no reference counterpart: the reference's nearest

@@ -16,3 +16,4 @@ from .observables_gpu import *  # noqa: F401,F403
 from .recurrence_functions import *  # noqa: F401,F403
 
 __version__ = "0.1.0"
+from .batch import *  # noqa: F401,F403,E402

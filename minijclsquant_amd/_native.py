@@ -59,6 +59,17 @@ SIGNATURES = {
     "jkpm_kubo_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
     "jkpm_kubo_mu_nm": (_int, [_vp, _vp, _vp, _int, _int, _vp, _vp, _size, _vp]),
     "jkpm_kubo_sigma": (_int, [_vp, _int, _vp, _int, _int, _dbl, _vp, _vp]),
+    "jkpm_grad_workspace": (_int, [_vp, _int, _int, _int, _c.POINTER(_size)]),
+    "jkpm_moments_grad": (_int, [_vp, _vp, _int, _int, _vp, _int, _vp, _vp, _size, _vp]),
+    "jkpm_dos_grad": (_int, [_vp, _int, _int, _vp, _int, _int, _vp, _vp]),
+    "jkpm_batch_create": (_int, [_vp, _int, _vp, _c.POINTER(_vp)]),
+    "jkpm_batch_destroy": (_int, [_vp]),
+    "jkpm_batch_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
+    "jkpm_batch_moments": (_int, [_vp, _vp, _vp, _vp, _int, _int, _vp, _vp, _size, _vp]),
+    "jkpm_graph_begin": (_int, [_vp]),
+    "jkpm_graph_end": (_int, [_vp, _c.POINTER(_vp)]),
+    "jkpm_graph_launch": (_int, [_vp, _vp]),
+    "jkpm_graph_destroy": (_int, [_vp]),
 }
 
 
@@ -298,6 +309,28 @@ class DeviceELL:
                                  stream_ptr(stream)))
         return out
 
+    def grad_workspace_bytes(self, R: int, M: int, seg: int = 0) -> int:
+        out = _size()
+        check(lib().jkpm_grad_workspace(self._h, int(R), int(M), int(seg), _c.byref(out)))
+        return int(out.value)
+
+    def grad_values(self, r, M: int, grad_mu, seg: int = 0, stream=None):
+        """dL/dRe(values) + i dL/dIm(values) for the row-major complex128
+        values this handle was made from, given grad_mu [M, R] (complex128)
+        of mu = moments(r, M)[0].  seg: checkpoint spacing (0: sqrt(M-1))."""
+        torch = _torch()
+        R = self._check_vecs(r)
+        M = int(M)
+        g = grad_mu.to(dtype=torch.complex128).contiguous()
+        if g.shape != (M, R) or not g.is_cuda:
+            raise ValueError(f"grad_mu must be a complex128 device tensor [M={M}, R={R}]")
+        ws = torch.empty(self.grad_workspace_bytes(R, M, seg), dtype=torch.uint8, device=r.device)
+        out = torch.empty(self.n * self.k, dtype=torch.complex128, device=r.device)
+        check(lib().jkpm_moments_grad(self._h, r.data_ptr(), R, M, g.data_ptr(), int(seg),
+                                      out.data_ptr(), ws.data_ptr(), ws.numel(),
+                                      stream_ptr(stream)))
+        return out
+
     def aAxby(self, a: complex, b: complex, x, y, stream=None) -> None:
         """y <- a*A*x + b*y on device tensors [N, R] (in place on y)."""
         R = self._check_vecs(x, y)
@@ -322,6 +355,114 @@ def dos(mu, energies_scaled, kernel: int = JKPM_KERNEL_JACKSON, stream=None):
     ws = torch.empty(M, dtype=torch.float64, device=mu.device)
     check(lib().jkpm_dos(mu.data_ptr(), M, R, e.data_ptr(), e.numel(), int(kernel),
                          out.data_ptr(), ws.data_ptr(), ws.numel() * 8, stream_ptr(stream)))
+    return out
+
+
+class Batch:
+    """A DeviceELL seen as a block-diagonal batch of G realizations with rows
+    [offsets[g], offsets[g+1]) (jkpm_batch_*)."""
+
+    def __init__(self, H: DeviceELL, offsets):
+        require_device()
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.H = H
+        self.G = off.size - 1
+        self.offsets = off
+        self._b = _vp()
+        check(lib().jkpm_batch_create(H.handle, self.G, off.ctypes.data, _c.byref(self._b)))
+
+    def workspace_bytes(self, R: int, n_left: int = 1) -> int:
+        out = _size()
+        check(lib().jkpm_batch_workspace(self._b, int(R), int(n_left), _c.byref(out)))
+        return int(out.value)
+
+    def moments(self, v_r, M: int, v_l0=None, workspace=None, out=None, stream=None):
+        """mu [G, M, R] complex128 of every realization (v_l0 defaults to v_r)."""
+        torch = _torch()
+        if v_l0 is None:
+            v_l0 = v_r
+        R = self.H._check_vecs(v_r, v_l0)
+        M = int(M)
+        need = self.workspace_bytes(R, 1)
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty(need, dtype=torch.uint8, device=v_r.device)
+        if out is None:
+            out = torch.empty((self.G, M, R), dtype=torch.complex128, device=v_r.device)
+        elif out.shape != (self.G, M, R) or out.dtype != torch.complex128 or not out.is_contiguous():
+            raise ValueError("out must be contiguous complex128 [G, M, R]")
+        check(lib().jkpm_batch_moments(self._b, v_r.data_ptr(), v_l0.data_ptr(), None, R, M,
+                                       out.data_ptr(), workspace.data_ptr(), workspace.numel(),
+                                       stream_ptr(stream)))
+        return out
+
+    def close(self):
+        if self._b:
+            lib().jkpm_batch_destroy(self._b)
+            self._b = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """HIP graph of the library launches issued on `stream` inside a
+    ``with Graph.capture(stream) as g:`` block; ``g.replay()`` relaunches
+    them (same pointers and sizes)."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self._g = _vp()
+
+    @classmethod
+    def capture(cls, stream):
+        return _Capture(cls(stream))
+
+    def replay(self, stream=None):
+        check(lib().jkpm_graph_launch(self._g, stream_ptr(stream or self.stream)))
+
+    def close(self):
+        if self._g:
+            lib().jkpm_graph_destroy(self._g)
+            self._g = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Capture:
+    def __init__(self, g: Graph):
+        self.g = g
+
+    def __enter__(self):
+        torch = require_device()
+        if self.g.stream is None or self.g.stream == torch.cuda.default_stream():
+            raise ValueError("graph capture needs a non-default stream")
+        check(lib().jkpm_graph_begin(stream_ptr(self.g.stream)))
+        return self.g
+
+    def __exit__(self, et, ev, tb):
+        rc = lib().jkpm_graph_end(stream_ptr(self.g.stream), _c.byref(self.g._g))
+        if et is None:
+            check(rc)
+        return False
+
+
+def dos_grad(w, M: int, R: int, energies_scaled, kernel: int = JKPM_KERNEL_JACKSON, stream=None):
+    """Transpose of dos(): complex128 [M, R] gradient of mu from w = dL/d dos."""
+    torch = require_device()
+    w = w.to(dtype=torch.float64).contiguous()
+    e = energies_scaled.to(device=w.device, dtype=torch.float64).contiguous()
+    if w.numel() != e.numel():
+        raise ValueError("w and energies must have the same length")
+    out = torch.empty((int(M), int(R)), dtype=torch.complex128, device=w.device)
+    check(lib().jkpm_dos_grad(w.data_ptr(), int(M), int(R), e.data_ptr(), e.numel(), int(kernel),
+                              out.data_ptr(), stream_ptr(stream)))
     return out
 
 

@@ -1,0 +1,138 @@
+"""Many disorder realizations (of uneven sizes) in one batch.
+
+The reference computes one realization and one vector at a time (callers
+loop over ``kpm_dos``, jclsquant/observables.py:26-54).  Here G
+realizations are stacked into one block-diagonal ELL (rows of realization g
+at [offsets[g], offsets[g+1]), its columns shifted by offsets[g], its values
+pre-scaled by its own 1/B_g), so every Chebyshev step of all G realizations
+and all R vectors is one kernel launch; moments come out per realization.
+The whole M-step recursion can be captured into one HIP graph.
+
+``square_lattice_ell`` builds the row-major ELL of a periodic square lattice
+directly (no CSR), with the same slots, order and values as
+``ell_matrix(square_lattice(...)[0])``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _kpm
+from . import _native as nat
+
+__all__ = ["square_lattice_ell", "block_diagonal", "BatchKPM", "kpm_dos_batch"]
+
+
+def square_lattice_ell(L, t=-1.0, W=0.0, seed=None):
+    """(data c128 [N*5], indices int32 [N*5]) of the periodic L x L square
+    lattice with onsite U[-W/2, W/2] from default_rng(seed); columns sorted
+    ascending per row, as in the CSR -> ELL path.  L >= 3."""
+    L = int(L)
+    if L < 3:
+        raise ValueError("L >= 3 (smaller periodic lattices have coinciding bonds)")
+    N = L * L
+    rng = np.random.default_rng(seed)
+    eps = (-W / 2 + W * rng.random(N)) if W else np.zeros(N)
+    i = np.arange(N, dtype=np.int64)
+    x, y = i % L, i // L
+    cols = np.stack([((y - 1) % L) * L + x, y * L + (x - 1) % L, i,
+                     y * L + (x + 1) % L, ((y + 1) % L) * L + x], 1)
+    vals = np.full((N, 5), complex(t))
+    vals[:, 2] = eps
+    order = np.argsort(cols, axis=1, kind="stable")
+    cols = np.take_along_axis(cols, order, 1)
+    vals = np.take_along_axis(vals, order, 1)
+    return vals.reshape(-1), cols.reshape(-1).astype(np.int32)
+
+
+def block_diagonal(blocks):
+    """Stack (data, indices, n_rows, len_row, scale) blocks into one
+    row-major ELL of width K = max len_row; returns (data, indices, offsets, K).
+    Narrower rows are padded like the reference (last column repeated, 0)."""
+    K = max(int(b[3]) for b in blocks)
+    offsets = np.zeros(len(blocks) + 1, dtype=np.int64)
+    for g, b in enumerate(blocks):
+        offsets[g + 1] = offsets[g] + int(b[2])
+    data = np.zeros(int(offsets[-1]) * K, dtype=np.complex128)
+    idx = np.zeros(int(offsets[-1]) * K, dtype=np.int32)
+    for g, (d, ix, n, k, s) in enumerate(blocks):
+        n, k = int(n), int(k)
+        d2 = np.asarray(d).reshape(n, k) * s
+        i2 = np.asarray(ix, dtype=np.int64).reshape(n, k) + offsets[g]
+        if k < K:
+            d2 = np.concatenate([d2, np.zeros((n, K - k), d2.dtype)], 1)
+            i2 = np.concatenate([i2, np.repeat(i2[:, -1:], K - k, 1)], 1)
+        o0, o1 = int(offsets[g]) * K, int(offsets[g + 1]) * K
+        data[o0:o1] = d2.reshape(-1)
+        idx[o0:o1] = i2.reshape(-1)
+    return data, idx, offsets, K
+
+
+class BatchKPM:
+    """Device state for the moments of G realizations x R vectors.
+
+    blocks: list of (data, indices, n_rows, len_row, scale) -- row-major ELL
+    of each realization and the scale 1/B_g of its spectral rescale.
+    vectors: list of [R, N_g] arrays (same R for every realization)."""
+
+    def __init__(self, blocks, vectors, M, dtype="c64"):
+        torch = nat.require_device()
+        data, idx, offsets, K = block_diagonal(blocks)
+        self.offsets = offsets
+        self.M = int(M)
+        self.H = nat.DeviceELL(data, idx, int(offsets[-1]), K, 1.0, dtype)
+        self.batch = nat.Batch(self.H, offsets)
+        v = np.concatenate([np.asarray(vg).reshape(-1, int(b[2])) for vg, b in zip(vectors, blocks)],
+                           axis=1)
+        self.R = v.shape[0]
+        self.vr = _kpm.as_batch(v, int(offsets[-1]), dtype)
+        self.ws = torch.empty(self.batch.workspace_bytes(self.R), dtype=torch.uint8, device="cuda")
+        self.mu = torch.empty((self.batch.G, self.M, self.R), dtype=torch.complex128, device="cuda")
+        self.graph = None
+        self.stream = None
+
+    def run(self, stream=None):
+        """Eager launch of the whole batch; returns mu [G, M, R] (device)."""
+        return self.batch.moments(self.vr, self.M, workspace=self.ws, out=self.mu, stream=stream)
+
+    def capture(self):
+        """Captures the recursion into a HIP graph on a private stream."""
+        torch = nat.require_device()
+        self.stream = torch.cuda.Stream()
+        self.stream.wait_stream(torch.cuda.current_stream())
+        with nat.Graph.capture(self.stream) as g:
+            self.batch.moments(self.vr, self.M, workspace=self.ws, out=self.mu, stream=self.stream)
+        self.graph = g
+        return g
+
+    def replay(self):
+        """Replays the captured graph; returns mu [G, M, R] after the replay
+        has finished on the capture stream."""
+        torch = nat.require_device()
+        if self.graph is None:
+            self.capture()
+        self.stream.wait_stream(torch.cuda.current_stream())
+        self.graph.replay(self.stream)
+        torch.cuda.current_stream().wait_stream(self.stream)
+        return self.mu
+
+
+def kpm_dos_batch(mats, M, random_vectors, energies=None, dtype="c64", kernel="jackson",
+                  graph=True):
+    """DOS tables [E, 2] for a list of ell_matrix realizations, computed in
+    one batch (random_vectors: list of [R, N_g] or [N_g] arrays)."""
+    blocks = []
+    for A in mats:
+        s = _kpm.bounds_scale(A.bounds)
+        if s is None:
+            raise ValueError("batched realizations need symmetric bounds")
+        blocks.append((A.data, A.indices, int(A.shape[0]), int(A.len_row), s))
+    vecs = [np.atleast_2d(v) for v in random_vectors]
+    bk = BatchKPM(blocks, vecs, M, dtype)
+    mu = bk.replay() if graph else bk.run()
+    out = []
+    for g, A in enumerate(mats):
+        E = (np.linspace(A.bounds[0] + 1e-3, A.bounds[1] - 1e-3, 2 * M + 1)
+             if energies is None else np.asarray(energies, dtype=np.float64))
+        d = _kpm.dos_device(mu[g], _kpm.rescale_energies(E, A.bounds), kernel)
+        out.append(np.stack([E, d.cpu().numpy()], 1))
+    return out

@@ -87,10 +87,21 @@ struct MomentsLayout {
     int nchunks = 1;
 };
 
-MomentsLayout moments_layout(const Ell &H, int R, int n_left, const DeviceInfo &dev) {
+// Row-chunk table of a block-diagonal batch of G realizations sharing one
+// handle: chunks never straddle two realizations, and chunks of group g are
+// group_chunks[g] .. group_chunks[g+1].
+struct Groups {
+    const long *chunk_rows = nullptr;  // device [nchunks + 1]
+    const int *group_chunks = nullptr; // device [G + 1]
+    int G = 1;
+    int nchunks = 0;
+};
+
+MomentsLayout moments_layout(const Ell &H, int R, int n_left, const DeviceInfo &dev,
+                             const Groups *grp = nullptr) {
     MomentsLayout L;
     const size_t vec = align_up((size_t)H.n * R * cplx_bytes(H.vtype), 256);
-    L.nchunks = step_chunks(H, R, dev);
+    L.nchunks = grp ? grp->nchunks : step_chunks(H, R, dev);
     L.b0 = 0;
     L.b1 = vec;
     L.ring = 2 * vec;
@@ -132,14 +143,16 @@ int device_info(DeviceInfo *out) {
     return JKPM_OK;
 }
 
-// Runs the whole recursion for one batch; see jkpm_moments.
+// Runs the whole recursion for one batch; see jkpm_moments.  With grp, the
+// handle is a block-diagonal batch and mu is [n_left][G][M][R].
 static int run_moments(const Ell &H, const void *vr, const void *vl0, const void *vl1, int R,
-                       int M, double2 *mu, void *ws, size_t ws_bytes, hipStream_t s) {
+                       int M, double2 *mu, void *ws, size_t ws_bytes, hipStream_t s,
+                       const Groups *grp = nullptr) {
     DeviceInfo dev;
     int rc = device_info(&dev);
     if (rc) return rc;
     const int n_left = vl1 ? 2 : 1;
-    const MomentsLayout L = moments_layout(H, R, n_left, dev);
+    const MomentsLayout L = moments_layout(H, R, n_left, dev, grp);
     if (ws_bytes < L.total) {
         set_error("workspace too small: need " + std::to_string(L.total) + " bytes, got " +
                   std::to_string(ws_bytes));
@@ -159,10 +172,13 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
         r0 += rb;
     }
 
+    // No event pairs while the stream is being captured into a graph.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    JKPM_HIP(hipStreamIsCapturing(s, &cap));
     Prof &P = prof();
     std::unique_lock<std::mutex> plk(P.mu);
     const int every = P.every;
-    const bool profiling = every > 0;
+    const bool profiling = every > 0 && cap == hipStreamCaptureStatusNone;
     if (!profiling) plk.unlock();
 
     auto step = [&](int n) -> int {
@@ -180,6 +196,7 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
         a.part = ring + (size_t)(n % kRing) * slot;
         a.part0 = (n == 1) ? ring : nullptr;  // step 0 lives in ring slot 0
         a.nchunks = L.nchunks;
+        a.chunk_rows = grp ? grp->chunk_rows : nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         const bool rec = profiling && (P.counter++ % every == 0);
         if (rec) {
@@ -213,7 +230,7 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
         const int last = std::min(n, M - 1);
         if (n == std::max(M, 2) - 1 || n - pending_lo + 1 == kRing) {
             rc = launch_reduce(ring, kRing, pending_lo, last - pending_lo + 1, n_left, L.nchunks,
-                               R, mu, M, s);
+                               R, mu, M, grp ? grp->group_chunks : nullptr, grp ? grp->G : 1, s);
             if (rc) return rc;
             pending_lo = n + 1;
         }
@@ -230,6 +247,7 @@ static int validate_vtype(int vtype) {
 static int upload_values(Ell *H, const double *data, double scale, const int *idx_dev,
                          int src_on_device, hipStream_t s) {
     const size_t tot = (size_t)H->n * H->k;
+    H->scale = scale;
     const double *src = data;
     double *staging = nullptr;
     if (!src_on_device) {
@@ -437,6 +455,163 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
                        (hipStream_t)stream);
 }
 
+// ---- block-diagonal batches of realizations ----
+
+struct jkpm_batch {
+    const jkpm_ell *h = nullptr;  // borrowed: must outlive the batch
+    long *chunk_rows = nullptr;   // device
+    int *group_chunks = nullptr;  // device
+    int G = 0, nchunks = 0;
+    Groups grp() const {
+        Groups g;
+        g.chunk_rows = chunk_rows;
+        g.group_chunks = group_chunks;
+        g.G = G;
+        g.nchunks = nchunks;
+        return g;
+    }
+};
+
+int jkpm_batch_create(const jkpm_ell *h, int G, const long long *row_offsets, jkpm_batch **out) {
+    clear_error();
+    if (!out) return arg_error("out is NULL");
+    *out = nullptr;
+    if (!h || !row_offsets) return arg_error("NULL argument");
+    if (G <= 0) return arg_error("G must be positive");
+    if (row_offsets[0] != 0 || row_offsets[G] != h->e.n)
+        return arg_error("row_offsets must start at 0 and end at the handle's row count");
+    for (int g = 0; g < G; ++g)
+        if (row_offsets[g + 1] <= row_offsets[g])
+            return arg_error("row_offsets must be strictly increasing");
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const double target = (double)dev.cu_count * 8;
+    std::vector<long> rows{0};
+    std::vector<int> gch{0};
+    for (int g = 0; g < G; ++g) {
+        const long n0 = row_offsets[g], ng = row_offsets[g + 1] - n0;
+        long c = (long)std::llround(target * (double)ng / (double)h->e.n);
+        c = std::max(1L, std::min(c, (ng + 63) / 64));
+        long per = (ng + c - 1) / c;
+        per = (per + 63) / 64 * 64;
+        for (long r0 = 0; r0 < ng; r0 += per) rows.push_back(n0 + std::min(ng, r0 + per));
+        gch.push_back((int)rows.size() - 1);
+    }
+    // Pad with empty chunks to a multiple of the XCD count; they belong to no group.
+    while ((rows.size() - 1) % 8 != 0 && rows.size() - 1 >= 8) rows.push_back(h->e.n);
+    auto *b = new (std::nothrow) jkpm_batch;
+    if (!b) return arg_error("host allocation failed");
+    b->h = h;
+    b->G = G;
+    b->nchunks = (int)rows.size() - 1;
+    hipError_t e;
+    if ((e = hipMalloc(&b->chunk_rows, rows.size() * sizeof(long))) != hipSuccess ||
+        (e = hipMalloc(&b->group_chunks, gch.size() * sizeof(int))) != hipSuccess ||
+        (e = hipMemcpy(b->chunk_rows, rows.data(), rows.size() * sizeof(long),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->group_chunks, gch.data(), gch.size() * sizeof(int),
+                       hipMemcpyHostToDevice)) != hipSuccess) {
+        hipFree(b->chunk_rows);
+        hipFree(b->group_chunks);
+        delete b;
+        return hip_fail(e, "batch tables", __FILE__, __LINE__);
+    }
+    *out = b;
+    return JKPM_OK;
+}
+
+int jkpm_batch_destroy(jkpm_batch *b) {
+    clear_error();
+    if (!b) return JKPM_OK;
+    hipFree(b->chunk_rows);
+    hipFree(b->group_chunks);
+    delete b;
+    return JKPM_OK;
+}
+
+int jkpm_batch_workspace(const jkpm_batch *b, int R, int n_left, size_t *bytes) {
+    clear_error();
+    if (!b || !bytes) return arg_error("NULL argument");
+    if (R <= 0) return arg_error("R must be positive");
+    if (n_left != 1 && n_left != 2) return arg_error("n_left must be 1 or 2");
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const Groups g = b->grp();
+    *bytes = moments_layout(b->h->e, R, n_left, dev, &g).total;
+    return JKPM_OK;
+}
+
+int jkpm_batch_moments(const jkpm_batch *b, const void *v_r, const void *v_l0, const void *v_l1,
+                       int R, int M, double *mu, void *workspace, size_t workspace_bytes,
+                       void *stream) {
+    clear_error();
+    if (!b) return arg_error("batch is NULL");
+    if (!v_r || !v_l0 || !mu || !workspace) return arg_error("NULL vector/mu/workspace");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    if (!aligned16(v_r) || !aligned16(v_l0) || (v_l1 && !aligned16(v_l1)) ||
+        !aligned16(workspace) || !aligned16(mu))
+        return arg_error("device pointers must be 16-byte aligned");
+    const Groups g = b->grp();
+    return run_moments(b->h->e, v_r, v_l0, v_l1, R, M, (double2 *)mu, workspace, workspace_bytes,
+                       (hipStream_t)stream, &g);
+}
+
+// ---- HIP graph capture of asynchronous entry points ----
+
+struct jkpm_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int jkpm_graph_begin(void *stream) {
+    clear_error();
+    if (!stream) return arg_error("capture needs a non-default stream");
+    JKPM_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return JKPM_OK;
+}
+
+int jkpm_graph_end(void *stream, jkpm_graph **out) {
+    clear_error();
+    if (!out) return arg_error("out is NULL");
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    JKPM_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    auto *gr = new (std::nothrow) jkpm_graph;
+    if (!gr) {
+        hipGraphExecDestroy(ex);
+        hipGraphDestroy(g);
+        return arg_error("host allocation failed");
+    }
+    gr->graph = g;
+    gr->exec = ex;
+    *out = gr;
+    return JKPM_OK;
+}
+
+int jkpm_graph_launch(const jkpm_graph *g, void *stream) {
+    clear_error();
+    if (!g) return arg_error("graph is NULL");
+    JKPM_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return JKPM_OK;
+}
+
+int jkpm_graph_destroy(jkpm_graph *g) {
+    clear_error();
+    if (!g) return JKPM_OK;
+    hipGraphExecDestroy(g->exec);
+    hipGraphDestroy(g->graph);
+    delete g;
+    return JKPM_OK;
+}
+
 int jkpm_dos(const double *mu, int M, int R, const double *e, int n_e, int kernel, double *dos,
              void *workspace, size_t workspace_bytes, void *stream) {
     clear_error();
@@ -450,6 +625,148 @@ int jkpm_dos(const double *mu, int M, int R, const double *e, int n_e, int kerne
     }
     return launch_dos((const double2 *)mu, M, R, e, n_e, kernel, dos, (double *)workspace,
                       (hipStream_t)stream);
+}
+
+// ---- adjoint gradient ----
+
+namespace {
+struct GradLayout {
+    size_t ckpt = 0, F0 = 0, F1 = 0, tseg = 0, lam = 0, C1 = 0, C2 = 0, G = 0, total = 0;
+    size_t vec = 0;
+    int S = 1, nseg = 0, nchunks = 1;
+};
+
+GradLayout grad_layout(const Ell &H, int R, int M, int seg, const DeviceInfo &dev) {
+    GradLayout L;
+    L.vec = align_up((size_t)H.n * R * cplx_bytes(H.vtype), 256);
+    int S = seg > 0 ? seg : 1;
+    if (seg <= 0)
+        while ((long)S * S < (long)(M - 1)) ++S;  // ceil(sqrt(M - 1))
+    L.S = S;
+    L.nseg = M > 1 ? (M - 1 + S - 1) / S : 0;
+    L.nchunks = step_chunks(H, R, dev);
+    size_t o = 0;
+    L.ckpt = o;
+    o += (size_t)2 * L.nseg * L.vec;
+    L.F0 = o;
+    o += L.vec;
+    L.F1 = o;
+    o += L.vec;
+    L.tseg = o;
+    o += (size_t)S * L.vec;
+    L.lam = o;
+    o += (size_t)S * L.vec;
+    L.C1 = o;
+    o += L.vec;
+    L.C2 = o;
+    o += L.vec;
+    L.G = o;
+    o += align_up((size_t)H.n * H.k * sizeof(double2), 256);
+    L.total = o;
+    return L;
+}
+}  // namespace
+
+int jkpm_grad_workspace(const jkpm_ell *h, int R, int M, int seg, size_t *bytes) {
+    clear_error();
+    if (!h || !bytes) return arg_error("NULL argument");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    *bytes = grad_layout(h->e, R, M, seg, dev).total;
+    return JKPM_OK;
+}
+
+int jkpm_moments_grad(const jkpm_ell *h, const void *r, int R, int M, const double *grad_mu,
+                      int seg, double *grad_values, void *workspace, size_t workspace_bytes,
+                      void *stream) {
+    clear_error();
+    if (!h) return arg_error("handle is NULL");
+    if (!r || !grad_mu || !grad_values || !workspace) return arg_error("NULL argument");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    if (!aligned16(r) || !aligned16(grad_mu) || !aligned16(grad_values) || !aligned16(workspace))
+        return arg_error("device pointers must be 16-byte aligned");
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const Ell &H = h->e;
+    const GradLayout L = grad_layout(H, R, M, seg, dev);
+    if (workspace_bytes < L.total) {
+        set_error("workspace too small: need " + std::to_string(L.total) + " bytes");
+        return JKPM_ERR_WORKSPACE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    char *w = (char *)workspace;
+    const size_t vb = (size_t)H.n * R * cplx_bytes(H.vtype);  // bytes of one vector batch
+    const int S = L.S;
+    auto ck = [&](int i) { return (void *)(w + L.ckpt + (size_t)i * L.vec); };
+    auto ts = [&](int q) { return (void *)(w + L.tseg + (size_t)q * vb); };
+    auto lm = [&](int q) { return (void *)(w + L.lam + (size_t)q * vb); };
+    void *F[2] = {w + L.F0, w + L.F1};
+    void *C1 = w + L.C1, *C2 = w + L.C2;
+    double2 *G = (double2 *)(w + L.G);
+    const double2 *gmu = (const double2 *)grad_mu;
+    const long gcount = (long)H.n * H.k;
+
+    rc = launch_zero(G, gcount, s);
+    if (rc) return rc;
+    if (M >= 2) {
+        // Forward pass: T_0 = r, T_n (n >= 1) in F[n & 1]; keep T_{cS}, T_{cS+1}.
+        const int last = M - 2;  // highest T index the gradient needs
+        JKPM_HIP(hipMemcpyAsync(ck(0), r, vb, hipMemcpyDeviceToDevice, s));
+        for (int n = 0; n < last; ++n) {
+            const void *x = n == 0 ? r : F[n & 1];
+            const void *prev = n == 0 ? nullptr : (n == 1 ? r : F[(n + 1) & 1]);
+            rc = launch_plain_step(H, x, prev, F[(n + 1) & 1], nullptr, nullptr, R,
+                                   n == 0 ? 1.0 : 2.0, n == 0 ? 0.0 : -1.0, L.nchunks, s);
+            if (rc) return rc;
+            const int m = n + 1;
+            if (m % S <= 1 && 2 * (m / S) + (m % S) < 2 * L.nseg)
+                JKPM_HIP(hipMemcpyAsync(ck(2 * (m / S) + (m % S)), F[m & 1], vb,
+                                        hipMemcpyDeviceToDevice, s));
+        }
+        // Reverse sweep over segments of k = 1 .. M-1.
+        rc = launch_zero((double2 *)C1, (long)(vb / 16), s);
+        if (!rc) rc = launch_zero((double2 *)C2, (long)(vb / 16), s);
+        if (rc) return rc;
+        for (int c = L.nseg - 1; c >= 0; --c) {
+            const int lo = 1 + c * S, hi = std::min((c + 1) * S, M - 1);
+            const int count = hi - lo + 1;  // T_{cS} .. T_{hi-1} and lambda_lo .. lambda_hi
+            JKPM_HIP(hipMemcpyAsync(ts(0), ck(2 * c), vb, hipMemcpyDeviceToDevice, s));
+            if (count >= 2)
+                JKPM_HIP(hipMemcpyAsync(ts(1), ck(2 * c + 1), vb, hipMemcpyDeviceToDevice, s));
+            for (int q = 2; q < count; ++q) {
+                rc = launch_plain_step(H, ts(q - 1), ts(q - 2), ts(q), nullptr, nullptr, R, 2.0,
+                                       -1.0, L.nchunks, s);
+                if (rc) return rc;
+            }
+            for (int k = hi; k >= lo; --k) {
+                const void *x = (k + 1 <= hi) ? lm(k + 1 - lo) : C1;
+                const void *prev = (k + 2 <= hi) ? lm(k + 2 - lo) : (k + 2 == hi + 1 ? C1 : C2);
+                rc = launch_plain_step(H, x, prev, lm(k - lo), gmu + (size_t)k * R, r, R, 2.0,
+                                       -1.0, L.nchunks, s);
+                if (rc) return rc;
+            }
+            rc = launch_grad_fold(H, lm(0), ts(0), count, lo, G, R, L.nchunks, s);
+            if (rc) return rc;
+            // carry lambda_lo -> C1 and lambda_{lo+1} -> C2 for the next segment down
+            JKPM_HIP(hipMemcpyAsync(C2, count >= 2 ? lm(1) : C1, vb, hipMemcpyDeviceToDevice, s));
+            JKPM_HIP(hipMemcpyAsync(C1, lm(0), vb, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    // the handle holds scale * values, so dL/dvalues = scale * dL/d(handle values)
+    return launch_grad_out(G, H.n, H.k, H.scale, (double2 *)grad_values, s);
+}
+
+int jkpm_dos_grad(const double *w, int M, int R, const double *e, int n_e, int kernel,
+                  double *grad_mu, void *stream) {
+    clear_error();
+    if (!w || !e || !grad_mu) return arg_error("NULL argument");
+    if (M <= 0 || R <= 0 || n_e < 0) return arg_error("M and R must be positive, n_e >= 0");
+    if (kernel != JKPM_KERNEL_NONE && kernel != JKPM_KERNEL_JACKSON)
+        return arg_error("kernel must be JKPM_KERNEL_NONE or JKPM_KERNEL_JACKSON");
+    return launch_dos_grad(w, M, R, e, n_e, kernel, (double2 *)grad_mu, (hipStream_t)stream);
 }
 
 // ---- Kubo double moments ----

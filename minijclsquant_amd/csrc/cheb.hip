@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const Real *__restrict__ xcur, const Real *xprev, Real *xout,
     const Real *__restrict__ vl0, const Real *__restrict__ vl1, int ld,
     double2 *__restrict__ part, double2 *__restrict__ part0, int first,
-    int rows_per_chunk, int nchunks) {
+    int rows_per_chunk, int nchunks, const long *__restrict__ chunk_rows) {
     using V = vec_t<Real, 2 * VPL>;
     using C2 = vec_t<Real, 2>;
     constexpr int RPW = 64 / LPR;
@@ -55,8 +55,11 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
 
     const int b = blockIdx.x;
     const int chunk = (nchunks % kXcd == 0) ? (b % kXcd) * (nchunks / kXcd) + b / kXcd : b;
-    const long row_begin = (long)chunk * rows_per_chunk;
-    const long row_end = min((long)n, row_begin + rows_per_chunk);
+    // Uniform chunks, or an explicit row table (chunks that never straddle
+    // two realizations of a block-diagonal batch).
+    const long row_begin = chunk_rows ? chunk_rows[chunk] : (long)chunk * rows_per_chunk;
+    const long row_end =
+        chunk_rows ? chunk_rows[chunk + 1] : min((long)n, row_begin + rows_per_chunk);
 
     double ar[NL][VPL], ai[NL][VPL];
     double a0r[NL][VPL], a0i[NL][VPL];
@@ -198,17 +201,23 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     }
 }
 
-// One work-group per (vector, left set, step): fixed-order sum of the chunk
-// partials of that step.
+// One work-group per (vector, left set x group, step): fixed-order sum of the
+// chunk partials of that step over the group's chunks (all chunks when there
+// are no groups).
 __global__ __launch_bounds__(kBlock) void reduce_kernel(const double2 *__restrict__ ring,
                                                          int ring_slots, int step_first,
                                                          int n_left, int nchunks, int ld,
-                                                         double2 *__restrict__ mu, int M) {
-    const int r = blockIdx.x, l = blockIdx.y, step = step_first + blockIdx.z;
+                                                         double2 *__restrict__ mu, int M,
+                                                         const int *__restrict__ group_chunks,
+                                                         int G) {
+    const int r = blockIdx.x, l = blockIdx.y / G, g = blockIdx.y % G;
+    const int step = step_first + blockIdx.z;
     const double2 *p =
         ring + ((long)(step % ring_slots) * n_left + l) * (long)nchunks * ld + r;
+    const int c_begin = group_chunks ? group_chunks[g] : 0;
+    const int c_end = group_chunks ? group_chunks[g + 1] : nchunks;
     double sr = 0.0, si = 0.0;
-    for (int c = threadIdx.x; c < nchunks; c += kBlock) {
+    for (int c = c_begin + threadIdx.x; c < c_end; c += kBlock) {
         const double2 q = p[(long)c * ld];
         sr += q.x;
         si += q.y;
@@ -224,7 +233,8 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(const double2 *__restric
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) mu[((long)l * M + step) * ld + r] = make_double2(shr[0], shi[0]);
+    if (threadIdx.x == 0)
+        mu[(((long)l * G + g) * M + step) * ld + r] = make_double2(shr[0], shi[0]);
 }
 
 // Row-major complex128 (device) -> slot-major value type, times `scale`.
@@ -374,18 +384,21 @@ int launch_step(const StepArgs &a, hipStream_t s) {
     double2 *p_part = a.part + a.r0;
     double2 *p_part0 = a.part0 ? a.part0 + a.r0 : nullptr;
     int p_first = a.first ? 1 : 0, p_rpc = (int)rpc, p_nch = a.nchunks;
+    const long *p_rows = a.chunk_rows;
     void *args[] = {&p_val,  &p_col,  &p_n,    &p_k,     &p_xcur,  &p_xprev, &p_xout, &p_vl0,
-                    &p_vl1,  &p_ld,   &p_part, &p_part0, &p_first, &p_rpc,   &p_nch};
+                    &p_vl1,  &p_ld,   &p_part, &p_part0, &p_first, &p_rpc,   &p_nch,  &p_rows};
     JKPM_HIP(hipLaunchKernel(e.fn, dim3(a.nchunks), dim3(kBlock), args, 0, s));
     JKPM_CHECK_LAUNCH("cheb_step_kernel");
     return JKPM_OK;
 }
 
 int launch_reduce(const double2 *ring, int ring_slots, int step_first, int nsteps, int n_left,
-                  int nchunks, int ld, double2 *mu, int M, hipStream_t s) {
+                  int nchunks, int ld, double2 *mu, int M, const int *group_chunks, int G,
+                  hipStream_t s) {
     if (nsteps <= 0) return JKPM_OK;
-    hipLaunchKernelGGL(reduce_kernel, dim3(ld, n_left, nsteps), dim3(kBlock), 0, s, ring,
-                       ring_slots, step_first, n_left, nchunks, ld, mu, M);
+    if (G < 1) G = 1;
+    hipLaunchKernelGGL(reduce_kernel, dim3(ld, n_left * G, nsteps), dim3(kBlock), 0, s, ring,
+                       ring_slots, step_first, n_left, nchunks, ld, mu, M, group_chunks, G);
     JKPM_CHECK_LAUNCH("reduce_kernel");
     return JKPM_OK;
 }

@@ -49,6 +49,7 @@ struct Ell {
     int n = 0;        // rows (= columns)
     int k = 0;        // slots per row (len_row)
     int vtype = JKPM_C128;
+    double scale = 1.0;    // the stored values are scale * (uploaded values)
     void *val = nullptr;   // [k][n] complex of vtype
     int *col = nullptr;    // [k][n]
     int device = 0;
@@ -73,6 +74,7 @@ struct StepArgs {
     double2 *part;        // partials of this step: [n_left][nchunks][ld]
     double2 *part0;       // when first: partials of step 0 (same layout)
     int nchunks;
+    const long *chunk_rows = nullptr;  // device [nchunks+1] row starts, or uniform chunks
 };
 
 // Number of row chunks (work-groups) used for a matrix of n rows.
@@ -83,10 +85,12 @@ int max_group(int vtype);
 
 int launch_step(const StepArgs &a, hipStream_t s);
 
-// mu[l][step][r] = sum over chunks of part ring slots, for steps
-// [step_first, step_first + nsteps).
+// mu[l][g][step][r] = sum over the chunks of group g (group_chunks[g] ..
+// group_chunks[g+1]; all chunks and G = 1 when group_chunks is null) of the
+// ring partials, for steps [step_first, step_first + nsteps).
 int launch_reduce(const double2 *ring, int ring_slots, int step_first, int nsteps,
-                  int n_left, int nchunks, int ld, double2 *mu, int M, hipStream_t s);
+                  int n_left, int nchunks, int ld, double2 *mu, int M,
+                  const int *group_chunks, int G, hipStream_t s);
 
 int launch_aaxby(const Ell &H, double2 a, double2 b, const void *x, void *y, int R,
                  hipStream_t s);
@@ -98,6 +102,21 @@ int launch_dos(const double2 *mu, int M, int R, const double *e, int n_e, int ke
                double *dos, double *w, hipStream_t s);
 
 int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
+
+// ---- adjoint recursion (adjoint.hip) ----
+// out = alpha*H*x + beta*prev + coef[r]*u (prev unused when beta == 0,
+// u and coef unused when coef is null); vectors [n][R] of H's value type.
+int launch_plain_step(const Ell &H, const void *x, const void *prev, void *out,
+                      const double2 *coef, const void *u, int R, double alpha, double beta,
+                      int nchunks, hipStream_t s);
+// G[s][i] += sum_{q<count} f_{k_lo+q} sum_r lam_q[i,r] conj(tprev_q[col(i,s),r]),
+// lam_q = lam + q*N*R, tprev_q = tprev + q*N*R, f_1 = 1, f_k = 2 otherwise.
+int launch_grad_fold(const Ell &H, const void *lam, const void *tprev, int count, int k_lo,
+                     double2 *G, int R, int nchunks, hipStream_t s);
+int launch_grad_out(const double2 *G, int n, int k, double scale, double2 *out, hipStream_t s);
+int launch_zero(double2 *p, long cnt, hipStream_t s);
+int launch_dos_grad(const double *w, int M, int R, const double *e, int n_e, int kernel,
+                    double2 *grad_mu, hipStream_t s);
 
 // ---- Kubo double moments (kubo.hip) ----
 int kubo_gemm_slices(long K, int M, const DeviceInfo &dev);

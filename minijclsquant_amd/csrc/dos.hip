@@ -52,7 +52,45 @@ __global__ __launch_bounds__(kBlock) void dos_kernel(const double *__restrict__ 
     if (threadIdx.x == 0) dos[i] = sh[0] / (M_PI * sqrt(1.0 - ei * ei));
 }
 
+// Transpose of the DOS map for gradients: given w_i = dL/d dos_i,
+// grad_mu[n][r] = (1/R) g_n (2 - [n==0]) sum_i w_i cos(n theta_i) / (pi s_i)
+// (real; dos depends on Re mu only).  One work-group per moment n.
+__global__ __launch_bounds__(kBlock) void dos_grad_kernel(const double *__restrict__ w, int M,
+                                                           int R, const double *__restrict__ e,
+                                                           int n_e, int kernel,
+                                                           double2 *__restrict__ grad_mu) {
+    const int n = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n_e; i += kBlock) {
+        const double ei = e[i];
+        s += w[i] * cos(n * acos(ei)) / (M_PI * sqrt(1.0 - ei * ei));
+    }
+    __shared__ double sh[kBlock];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    double g = 1.0;
+    if (kernel == JKPM_KERNEL_JACKSON) {
+        const double q = M_PI / (M + 1.0);
+        g = ((M - n + 1.0) * cos(q * n) + sin(q * n) / tan(q)) / (M + 1.0);
+    }
+    const double val = sh[0] * g * (n == 0 ? 1.0 : 2.0) / (double)R;
+    for (int r = threadIdx.x; r < R; r += kBlock) grad_mu[(long)n * R + r] = make_double2(val, 0.0);
+}
+
 }  // namespace
+
+int launch_dos_grad(const double *w, int M, int R, const double *e, int n_e, int kernel,
+                    double2 *grad_mu, hipStream_t s) {
+    if (M <= 0) return JKPM_OK;
+    hipLaunchKernelGGL(dos_grad_kernel, dim3(M), dim3(kBlock), 0, s, w, M, R, e, n_e, kernel,
+                       grad_mu);
+    JKPM_CHECK_LAUNCH("dos_grad_kernel");
+    return JKPM_OK;
+}
 
 int launch_dos(const double2 *mu, int M, int R, const double *e, int n_e, int kernel,
                double *dos, double *w, hipStream_t s) {

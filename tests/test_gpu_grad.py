@@ -1,0 +1,143 @@
+"""Adjoint Chebyshev gradients on the GPU.
+
+* jkpm_moments_grad vs the oracle's FP64 reverse-mode restatement
+  (oracle.grad_values, itself pinned by finite differences): <= 1e-10 relative.
+* Independence of the checkpoint spacing (sqrt(M) default, 1, 3, M).
+* torch.autograd through chebyshev_moments + kpm_dos_torch on real onsite
+  energies and bond hoppings vs central finite differences (h = 1e-6,
+  complex128) -- on a small lattice and at the differentiation workload's
+  size (square 512 x 512, M = 1024, R = 8, 64 coordinates):
+  |g - g_FD| / |g_FD| <= 1e-4.
+"""
+import numpy as np
+import pytest
+
+import minijclsquant_amd as jcl
+from minijclsquant_amd import _native as nat
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def square_params(L, seed, eps_half=1.0, sig=0.05):
+    H, hop = jcl.square_lattice(L, t=-1.0, seed=seed, W=2 * eps_half, bond_noise=sig)
+    A = jcl.ell_matrix(H)
+    B = (4 * (1 + 6 * sig) + eps_half) * 1.01
+    A.bounds = np.array([-B, B])
+    return A, B
+
+
+def test_grad_values_vs_oracle(torch_gpu):
+    torch = torch_gpu
+    H, Dx, Dy = jcl.honeycomb_lattice(6, t=-2.7, W=1.0, seed=3)
+    A = jcl.ell_matrix(H)
+    B = jcl.bounds(H)[1]
+    N, K, M, R = A.shape[0], A.len_row, 40, 3
+    rng = np.random.default_rng(0)
+    r = np.stack([jcl.random_vector_generator(N, 50 + s) for s in range(R)])
+    g = rng.normal(size=(M, R)) + 1j * rng.normal(size=(M, R))
+    h = nat.DeviceELL(A.data, A.indices, N, K, 1 / B, "c128")
+    rd = torch.from_numpy(r.T.copy()).cuda()
+    gd = torch.from_numpy(g).cuda()
+    want = sum(O.grad_values(M, A.data * (1 / B), A.indices, K, r[k], np.conj(g[:, k]))
+               for k in range(R)) * (1 / B)
+    for seg in (0, 1, 3, M):
+        got = h.grad_values(rd, M, gd, seg=seg).cpu().numpy()
+        err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+        assert err <= 1e-10, (seg, err)
+
+
+@pytest.mark.parametrize("M", [1, 2, 3, 17])
+def test_grad_small_M(torch_gpu, M):
+    torch = torch_gpu
+    A, B = square_params(5, seed=M)
+    N, K = A.shape[0], A.len_row
+    r = jcl.random_vector_generator(N, 1)
+    g = np.random.default_rng(M).normal(size=(M, 1)) + 0j
+    h = nat.DeviceELL(A.data, A.indices, N, K, 1 / B, "c128")
+    got = h.grad_values(torch.from_numpy(r.reshape(N, 1)).cuda(), M,
+                        torch.from_numpy(g).cuda()).cpu().numpy()
+    want = O.grad_values(M, A.data / B, A.indices, K, r, np.conj(g[:, 0])) / B
+    scale = max(np.max(np.abs(want)), 1e-300)
+    assert np.max(np.abs(got - want)) / scale <= 1e-10 or np.max(np.abs(want)) == 0
+
+
+def _loss_setup(torch, L, M, R, n_e, seed):
+    from minijclsquant_amd.autograd import param_slots
+
+    A, B = square_params(L, seed)
+    N = A.shape[0]
+    slot_param, pairs = param_slots(A)
+    rows = np.arange(A.data.size) // A.len_row
+    eps0 = np.real(A.data[(rows == A.indices) & (A.valid_row_values == 1)])
+    # bond values from the matrix itself, in bond order
+    Hd = A.A.tocsr()
+    t0 = np.real(np.asarray(Hd[pairs[:, 0], pairs[:, 1]]).ravel())
+    r = np.stack([jcl.random_vector_generator(N, 300 + s) for s in range(R)])
+    rd = torch.from_numpy(r.T.copy()).cuda()
+    idx = torch.from_numpy(A.indices).cuda()
+    E = np.linspace(-B + 0.05, B - 0.05, n_e)
+    es = torch.as_tensor(E / B, device="cuda")
+    w = torch.as_tensor(np.random.default_rng(seed + 1).normal(size=n_e), device="cuda")
+    return A, B, slot_param, eps0, t0, rd, idx, es, w
+
+
+def _loss(torch, eps, hop, slot_param, idx, rd, M, B, es, w):
+    from minijclsquant_amd.autograd import chebyshev_moments, kpm_dos_torch, lattice_values
+
+    vals = lattice_values(eps, hop, slot_param)
+    mu = chebyshev_moments(vals, idx, rd, M, scale=1.0 / B)
+    dos = kpm_dos_torch(mu, es)
+    return (w * dos).sum()
+
+
+def _check_fd(torch, L, M, R, n_e, n_coord, seed):
+    A, B, sp_, eps0, t0, rd, idx, es, w = _loss_setup(torch, L, M, R, n_e, seed)
+    N = A.shape[0]
+    eps = torch.tensor(eps0, device="cuda", requires_grad=True)
+    hop = torch.tensor(t0, device="cuda", requires_grad=True)
+    loss = _loss(torch, eps, hop, sp_, idx, rd, M, B, es, w)
+    loss.backward()
+    g_all = torch.cat([eps.grad, hop.grad]).cpu().numpy()
+    rng = np.random.default_rng(seed + 2)
+    coords = rng.choice(g_all.size, n_coord, replace=False)
+    h = 1e-6
+    fd = []
+    with torch.no_grad():
+        base = torch.cat([eps, hop]).detach()
+        for c in coords:
+            vals = []
+            for sgn in (1.0, -1.0):
+                p = base.clone()
+                p[c] += sgn * h
+                vals.append(float(_loss(torch, p[:N], p[N:], sp_, idx, rd, M, B, es, w)))
+            fd.append((vals[0] - vals[1]) / (2 * h))
+    fd = np.array(fd)
+    g = g_all[coords]
+    return np.linalg.norm(g - fd) / np.linalg.norm(fd), g, fd
+
+
+def test_autograd_matches_fd_small(torch_gpu):
+    err, g, fd = _check_fd(torch_gpu, L=12, M=64, R=2, n_e=64, n_coord=16, seed=4)
+    assert err <= 1e-4, (err, g[:4], fd[:4])
+
+
+def test_autograd_matches_fd_workload_size(torch_gpu):
+    """Square 512 x 512, onsite U[-1, 1], t_ij = -1 + N(0, 0.05^2), M = 1024,
+    R = 8, loss = sum_E w(E) DOS(E) with seeded Gaussian w, 64 coordinates."""
+    err, g, fd = _check_fd(torch_gpu, L=512, M=1024, R=8, n_e=256, n_coord=64, seed=2024)
+    assert err <= 1e-4, (err, g[:4], fd[:4])
+
+
+def test_dos_grad_is_transpose(torch_gpu):
+    torch = torch_gpu
+    rng = np.random.default_rng(3)
+    M, R, n_e = 50, 3, 40
+    e = torch.as_tensor(np.linspace(-0.95, 0.95, n_e), device="cuda")
+    mu = torch.as_tensor(rng.normal(size=(M, R)) + 1j * rng.normal(size=(M, R)), device="cuda")
+    w = torch.as_tensor(rng.normal(size=n_e), device="cuda")
+    # <w, D mu> == Re <D^T w, mu> for the linear DOS map D
+    lhs = float((w * nat.dos(mu, e)).sum())
+    gm = nat.dos_grad(w, M, R, e)
+    rhs = float((gm.real * mu.real).sum() + (gm.imag * mu.imag).sum())
+    assert abs(lhs - rhs) <= 1e-10 * max(1.0, abs(lhs))

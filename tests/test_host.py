@@ -165,6 +165,29 @@ def test_modifier_returns_new_object_and_drops_device_cache():
     assert np.array_equal(B.indices, A.indices)
 
 
+def test_square_lattice_ell_matches_csr_path():
+    for L in (3, 5, 16):
+        d, i = jcl.square_lattice_ell(L, t=-1.0, W=2.0, seed=L)
+        A = jcl.ell_matrix(jcl.square_lattice(L, t=-1.0, W=2.0, seed=L)[0])
+        assert np.array_equal(d, A.data) and np.array_equal(i, A.indices)
+
+
+def test_block_diagonal_layout():
+    mats = [jcl.ell_matrix(jcl.square_lattice(L, W=1.0, seed=L)[0]) for L in (3, 4)]
+    Hh, _, _ = jcl.honeycomb_lattice(2, W=1.0, seed=1)
+    mats.append(jcl.ell_matrix(Hh))  # K = 4 < 5: padded
+    blocks = [(A.data, A.indices, A.shape[0], A.len_row, 0.5) for A in mats]
+    data, idx, off, K = jcl.block_diagonal(blocks)
+    assert K == 5 and list(off) == [0, 9, 25, 33]
+    big = sp.csr_matrix((data, (np.arange(data.size) // K, idx)), shape=(33, 33))
+    want = sp.block_diag([0.5 * A.A for A in mats]).tocsr()
+    assert abs(big - want).max() == 0
+    # padded slots of the narrower block repeat their row's last column
+    last = idx.reshape(33, K)[25:, 3]
+    assert np.array_equal(idx.reshape(33, K)[25:, 4], last)
+    assert np.all(data.reshape(33, K)[25:, 4] == 0)
+
+
 def test_recursion_api_argument_behaviour(capsys):
     H, _ = jcl.square_lattice(4)
     A = jcl.ell_matrix(H)
