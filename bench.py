@@ -71,6 +71,23 @@ def traffic_from_profile(N, K, R):
     return float(m.group(1))
 
 
+def copy_ceiling(torch, nat, nbytes=2 << 30, reps=5):
+    """HBM rate of the library's plain copy kernel (bytes read + written per
+    second), timed with HIP events on the current stream."""
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    nat.copy(dst, src, nbytes)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        nat.copy(dst, src, nbytes)
+    t1.record()
+    t1.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    del src, dst
+    return 2 * nbytes / (ms / 1e3) / 1e9
+
+
 def cpu_baseline(A, r, target_s=15.0):
     """complex128 oracle (reference algorithm) on host cores: one vector, a
     bounded number of moments sized to ~target_s seconds."""
@@ -159,6 +176,8 @@ def main():
         one_step()
         log(f"warmup step {i + 1}/{args.warmup} done")
     torch.cuda.synchronize()
+    copy_gbs = copy_ceiling(torch, nat)
+    log(f"copy kernel: {copy_gbs:.0f} GB/s (read + write)")
 
     nat.profile_read(reset=True)
     nat.profile_enable(args.profile_every)
@@ -190,6 +209,8 @@ def main():
         "algorithmic_bytes_per_launch": bytes_launch,
         "mean_launch_us": round(launch_ms * 1e3, 2),
         "launches_timed": n_prof,
+        "copy_kernel_GBs": round(copy_gbs, 1),
+        "frac_of_copy": round(achieved / copy_gbs, 4),
     }
 
     dos_err = None

@@ -110,6 +110,26 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
                  void *stream);
 
 /* ---------------------------------------------------------------------
+ * Launch shape of the Chebyshev step kernel and its autotuner (no reference
+ * counterpart: the reference launches <<<ceil(N/1024), 1024>>> always,
+ * jclsquant/cuda_cython/cuda/recursions.cu:19-20).
+ * ------------------------------------------------------------------- */
+
+/* variant: 0..3 = (rows per lane, non-temporal loads of the read-once
+ * streams) in {(1,no), (2,no), (1,yes), (2,yes)} (variants exist for
+ * single-left-set kernels of ELL width 4 and 5; other shapes use 0);
+ * chunk_mult: work-groups per CU (1..64, default 8). */
+int jkpm_ell_set_launch(jkpm_ell *h, int variant, int chunk_mult);
+int jkpm_ell_get_launch(const jkpm_ell *h, int *variant, int *chunk_mult);
+
+/* Times `steps` fused steps of jkpm_moments on v_r (device [N][R]) for each
+ * variant x chunk_mult in {4, 8, 16, 32}, keeps the fastest on the handle and
+ * returns its mean step time in microseconds.  Synchronises `stream`. */
+int jkpm_autotune_workspace(const jkpm_ell *h, int R, int steps, size_t *bytes);
+int jkpm_autotune(jkpm_ell *h, const void *v_r, int R, int steps, void *workspace,
+                  size_t workspace_bytes, void *stream, double *best_us);
+
+/* ---------------------------------------------------------------------
  * Batches of disorder realizations (no reference counterpart: reference
  * callers loop over realizations and vectors serially).
  * ------------------------------------------------------------------- */

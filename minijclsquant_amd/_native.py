@@ -70,6 +70,10 @@ SIGNATURES = {
     "jkpm_graph_end": (_int, [_vp, _c.POINTER(_vp)]),
     "jkpm_graph_launch": (_int, [_vp, _vp]),
     "jkpm_graph_destroy": (_int, [_vp]),
+    "jkpm_ell_set_launch": (_int, [_vp, _int, _int]),
+    "jkpm_ell_get_launch": (_int, [_vp, _ip, _ip]),
+    "jkpm_autotune_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
+    "jkpm_autotune": (_int, [_vp, _vp, _int, _int, _vp, _size, _vp, _dp]),
 }
 
 
@@ -308,6 +312,27 @@ class DeviceELL:
                                  out.data_ptr(), workspace.data_ptr(), workspace.numel(),
                                  stream_ptr(stream)))
         return out
+
+    def set_launch(self, variant: int, chunk_mult: int) -> None:
+        check(lib().jkpm_ell_set_launch(self._h, int(variant), int(chunk_mult)))
+
+    def get_launch(self):
+        v, m = _c.c_int(), _c.c_int()
+        check(lib().jkpm_ell_get_launch(self._h, _c.byref(v), _c.byref(m)))
+        return int(v.value), int(m.value)
+
+    def autotune(self, v_r, steps: int = 8, stream=None) -> float:
+        """Picks the fastest step-kernel launch shape for these vectors;
+        returns its mean step time (us)."""
+        torch = _torch()
+        R = self._check_vecs(v_r)
+        need = _size()
+        check(lib().jkpm_autotune_workspace(self._h, R, int(steps), _c.byref(need)))
+        ws = torch.empty(int(need.value), dtype=torch.uint8, device=v_r.device)
+        best = _c.c_double()
+        check(lib().jkpm_autotune(self._h, v_r.data_ptr(), R, int(steps), ws.data_ptr(),
+                                  ws.numel(), stream_ptr(stream), _c.byref(best)))
+        return float(best.value)
 
     def grad_workspace_bytes(self, R: int, M: int, seg: int = 0) -> int:
         out = _size()

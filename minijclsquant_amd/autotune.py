@@ -1,0 +1,79 @@
+"""Launch-shape autotuner with a persistent table.
+
+The step kernel has four variants (rows per lane x non-temporal streams) and
+a work-group count per CU; jkpm_autotune times them on the caller's matrix
+and vectors.  Results are cached in a JSON table keyed by device, matrix
+shape (N, K), value type and batch size R, so a shape is tuned once per
+machine.  ``JKPM_LAUNCH="variant,chunk_mult"`` overrides the table.
+
+The reference has a single fixed launch (<<<ceil(N/1024), 1024>>>,
+jclsquant/cuda_cython/cuda/recursions.cu:19-20).
+"""
+from __future__ import annotations
+
+import json
+import os
+import threading
+from pathlib import Path
+
+from . import _native as nat
+
+__all__ = ["tuned", "table_path"]
+
+_lock = threading.Lock()
+
+
+def table_path() -> Path:
+    env = os.environ.get("JKPM_TUNE_TABLE")
+    if env:
+        return Path(env)
+    return Path(os.path.expanduser("~")) / ".cache" / "jclkpm" / "launch_table.json"
+
+
+def _key(h: "nat.DeviceELL", R: int) -> str:
+    import torch
+
+    dev = torch.cuda.get_device_name(torch.cuda.current_device())
+    return f"{dev}|N={h.n}|K={h.k}|vtype={h.vtype}|R={R}"
+
+
+def _load():
+    p = table_path()
+    try:
+        return json.loads(p.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(tab):
+    p = table_path()
+    try:
+        p.parent.mkdir(parents=True, exist_ok=True)
+        tmp = p.with_suffix(".tmp")
+        tmp.write_text(json.dumps(tab, indent=1, sort_keys=True))
+        os.replace(tmp, p)
+    except OSError:
+        pass  # a read-only home just means re-tuning next time
+
+
+def tuned(h: "nat.DeviceELL", v_r, steps: int = 8, retune: bool = False):
+    """Applies the cached (or freshly tuned) launch shape to handle h for the
+    batch v_r [N, R]; returns (variant, chunk_mult)."""
+    env = os.environ.get("JKPM_LAUNCH")
+    if env:
+        v, m = (int(x) for x in env.split(","))
+        h.set_launch(v, m)
+        return v, m
+    R = int(v_r.shape[1])
+    key = _key(h, R)
+    with _lock:
+        tab = _load()
+        if not retune and key in tab:
+            v, m = tab[key]["variant"], tab[key]["chunk_mult"]
+            h.set_launch(v, m)
+            return v, m
+        us = h.autotune(v_r, steps)
+        v, m = h.get_launch()
+        tab[key] = {"variant": v, "chunk_mult": m, "step_us": us}
+        _store(tab)
+        return v, m

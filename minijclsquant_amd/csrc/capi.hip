@@ -455,6 +455,101 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
                        (hipStream_t)stream);
 }
 
+// ---- launch shape and autotuner ----
+
+int jkpm_ell_set_launch(jkpm_ell *h, int variant, int chunk_mult) {
+    clear_error();
+    if (!h) return arg_error("handle is NULL");
+    if (variant < 0 || variant > 3) return arg_error("variant must be 0..3");
+    if (chunk_mult < 1 || chunk_mult > 64) return arg_error("chunk_mult must be 1..64");
+    h->e.variant = variant;
+    h->e.chunk_mult = chunk_mult;
+    return JKPM_OK;
+}
+
+int jkpm_ell_get_launch(const jkpm_ell *h, int *variant, int *chunk_mult) {
+    clear_error();
+    if (!h) return arg_error("handle is NULL");
+    if (variant) *variant = h->e.variant;
+    if (chunk_mult) *chunk_mult = h->e.chunk_mult;
+    return JKPM_OK;
+}
+
+namespace {
+const int kTuneMult[] = {4, 8, 16, 32};
+}
+
+int jkpm_autotune_workspace(const jkpm_ell *h, int R, int steps, size_t *bytes) {
+    clear_error();
+    if (!h || !bytes) return arg_error("NULL argument");
+    if (R <= 0 || steps <= 0) return arg_error("R and steps must be positive");
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    Ell probe = h->e;
+    size_t need = 0;
+    for (int m : kTuneMult) {
+        probe.chunk_mult = m;
+        need = std::max(need, moments_layout(probe, R, 1, dev).total);
+    }
+    *bytes = need + align_up((size_t)(steps + 1) * R * sizeof(double2), 256);
+    return JKPM_OK;
+}
+
+int jkpm_autotune(jkpm_ell *h, const void *v_r, int R, int steps, void *workspace,
+                  size_t workspace_bytes, void *stream, double *best_us) {
+    clear_error();
+    if (!h || !v_r || !workspace) return arg_error("NULL argument");
+    if (R <= 0 || steps <= 0) return arg_error("R and steps must be positive");
+    size_t need = 0;
+    int rc = jkpm_autotune_workspace(h, R, steps, &need);
+    if (rc) return rc;
+    if (workspace_bytes < need) {
+        set_error("workspace too small: need " + std::to_string(need) + " bytes");
+        return JKPM_ERR_WORKSPACE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const size_t mu_bytes = align_up((size_t)(steps + 1) * R * sizeof(double2), 256);
+    double2 *mu = (double2 *)workspace;
+    char *ws = (char *)workspace + mu_bytes;
+    const size_t ws_bytes = workspace_bytes - mu_bytes;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    JKPM_HIP(hipEventCreate(&e0));
+    JKPM_HIP(hipEventCreate(&e1));
+    const int keep_v = h->e.variant, keep_m = h->e.chunk_mult;
+    double best = 1e300;
+    int bv = keep_v, bm = keep_m;
+    for (int v = 0; v < 4 && !rc; ++v)
+        for (int m : kTuneMult) {
+            h->e.variant = v;
+            h->e.chunk_mult = m;
+            // one untimed pass, then the timed pass of `steps` fused steps
+            rc = run_moments(h->e, v_r, v_r, nullptr, R, steps + 1, mu, ws, ws_bytes, s);
+            if (!rc) rc = hipEventRecord(e0, s) == hipSuccess ? 0 : JKPM_ERR_HIP;
+            if (!rc) rc = run_moments(h->e, v_r, v_r, nullptr, R, steps + 1, mu, ws, ws_bytes, s);
+            if (!rc) rc = hipEventRecord(e1, s) == hipSuccess ? 0 : JKPM_ERR_HIP;
+            if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? 0 : JKPM_ERR_HIP;
+            if (rc) break;
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) {
+                best = ms;
+                bv = v;
+                bm = m;
+            }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    h->e.variant = rc ? keep_v : bv;
+    h->e.chunk_mult = rc ? keep_m : bm;
+    if (rc) {
+        if (g_err.empty()) set_error("autotune launch failed");
+        return rc;
+    }
+    if (best_us) *best_us = best * 1e3 / steps;
+    return JKPM_OK;
+}
+
 // ---- block-diagonal batches of realizations ----
 
 struct jkpm_batch {

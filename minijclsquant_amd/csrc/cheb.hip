@@ -35,7 +35,19 @@ constexpr int kXcd = 8;
 template <typename Real, int N>
 using vec_t = Real __attribute__((ext_vector_type(N)));
 
-template <typename Real, int VPL, int LPR, int NL, int KT>
+// Loads of data read once per step (T_{n-1}, v_l): non-temporal when NT, so
+// they do not displace the gather window of T_n from the caches.
+template <bool NT, typename V>
+__device__ __forceinline__ V load_stream(const V *p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+// UNR: rows per lane per loop iteration (all their loads are issued before
+// any is used, for more bytes in flight per wave).
+template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT>
 __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const Real *__restrict__ val, const int *__restrict__ col, int n, int k_rt,
     const Real *__restrict__ xcur, const Real *xprev, Real *xout,
@@ -71,85 +83,115 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const int voff = lv * VPL * 2;
     const Real *vls[2] = {vl0, vl1};
 
-    for (long row = row_begin + wave * RPW + lr; row < row_end; row += kWaves * RPW) {
-        Real sr[VPL], si[VPL];
+    constexpr int STRIDE = kWaves * RPW;  // rows between a lane's consecutive rows
+    for (long base = row_begin + wave * RPW + lr; base < row_end; base += (long)STRIDE * UNR) {
+        // Rows past the chunk end are clamped to `base` (a valid row) for the
+        // loads and their results are dropped.
+        long rows[UNR];
+        bool ok[UNR];
 #pragma unroll
-        for (int v = 0; v < VPL; ++v) sr[v] = si[v] = Real(0);
+        for (int u = 0; u < UNR; ++u) {
+            const long rw = base + (long)u * STRIDE;
+            ok[u] = rw < row_end;
+            rows[u] = ok[u] ? rw : base;
+        }
+        Real sr[UNR][VPL], si[UNR][VPL];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) sr[u][v] = si[u][v] = Real(0);
 
         if constexpr (KT > 0) {
-            int j[KT];
-            C2 h[KT];
+            int j[UNR][KT];
+            C2 h[UNR][KT];
 #pragma unroll
-            for (int s = 0; s < KT; ++s) {
-                const long e = (long)s * n + row;
-                j[s] = col[e];
-                h[s] = *reinterpret_cast<const C2 *>(val + 2 * e);
-            }
-            V x[KT];
+            for (int u = 0; u < UNR; ++u)
 #pragma unroll
-            for (int s = 0; s < KT; ++s)
-                x[s] = *reinterpret_cast<const V *>(xcur + (long)j[s] * ld * 2 + voff);
-#pragma unroll
-            for (int s = 0; s < KT; ++s)
-#pragma unroll
-                for (int v = 0; v < VPL; ++v) {
-                    sr[v] += h[s].x * x[s][2 * v] - h[s].y * x[s][2 * v + 1];
-                    si[v] += h[s].x * x[s][2 * v + 1] + h[s].y * x[s][2 * v];
+                for (int s = 0; s < KT; ++s) {
+                    const long e = (long)s * n + rows[u];
+                    j[u][s] = col[e];
+                    h[u][s] = *reinterpret_cast<const C2 *>(val + 2 * e);
                 }
+            V x[UNR][KT];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int s = 0; s < KT; ++s)
+                    x[u][s] = *reinterpret_cast<const V *>(xcur + (long)j[u][s] * ld * 2 + voff);
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int s = 0; s < KT; ++s)
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) {
+                        sr[u][v] += h[u][s].x * x[u][s][2 * v] - h[u][s].y * x[u][s][2 * v + 1];
+                        si[u][v] += h[u][s].x * x[u][s][2 * v + 1] + h[u][s].y * x[u][s][2 * v];
+                    }
         } else {
-            for (int s = 0; s < K; ++s) {
-                const long e = (long)s * n + row;
-                const int j = col[e];
-                const C2 h = *reinterpret_cast<const C2 *>(val + 2 * e);
-                const V x = *reinterpret_cast<const V *>(xcur + (long)j * ld * 2 + voff);
 #pragma unroll
-                for (int v = 0; v < VPL; ++v) {
-                    sr[v] += h.x * x[2 * v] - h.y * x[2 * v + 1];
-                    si[v] += h.x * x[2 * v + 1] + h.y * x[2 * v];
+            for (int u = 0; u < UNR; ++u)
+                for (int s = 0; s < K; ++s) {
+                    const long e = (long)s * n + rows[u];
+                    const int j = col[e];
+                    const C2 h = *reinterpret_cast<const C2 *>(val + 2 * e);
+                    const V x = *reinterpret_cast<const V *>(xcur + (long)j * ld * 2 + voff);
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) {
+                        sr[u][v] += h.x * x[2 * v] - h.y * x[2 * v + 1];
+                        si[u][v] += h.x * x[2 * v + 1] + h.y * x[2 * v];
+                    }
                 }
-            }
         }
 
-        const long vo = row * ld * 2 + voff;
-        V y;
-        if (first) {
+        // T_{n-1} and v_l of every row first, then the arithmetic.
+        V p[UNR], q[UNR][NL], t0[UNR];
 #pragma unroll
-            for (int v = 0; v < VPL; ++v) {
-                y[2 * v] = sr[v];
-                y[2 * v + 1] = si[v];
-            }
-            const V t0 = *reinterpret_cast<const V *>(xcur + vo);
+        for (int u = 0; u < UNR; ++u) {
+            const long vo = rows[u] * ld * 2 + voff;
+            if (first)
+                t0[u] = *reinterpret_cast<const V *>(xcur + vo);
+            else
+                p[u] = load_stream<NT>(reinterpret_cast<const V *>(xprev + vo));
 #pragma unroll
-            for (int l = 0; l < NL; ++l) {
-                const V q = *reinterpret_cast<const V *>(vls[l] + vo);
+            for (int l = 0; l < NL; ++l) q[u][l] = load_stream<NT>(reinterpret_cast<const V *>(vls[l] + vo));
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if (!ok[u]) continue;
+            const long vo = rows[u] * ld * 2 + voff;
+            V y;
+            if (first) {
 #pragma unroll
                 for (int v = 0; v < VPL; ++v) {
-                    const double qr = q[2 * v], qi = q[2 * v + 1];
-                    const double tr = t0[2 * v], ti = t0[2 * v + 1];
-                    a0r[l][v] += qr * tr + qi * ti;
-                    a0i[l][v] += qr * ti - qi * tr;
+                    y[2 * v] = sr[u][v];
+                    y[2 * v + 1] = si[u][v];
+                }
+#pragma unroll
+                for (int l = 0; l < NL; ++l)
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) {
+                        const double qr = q[u][l][2 * v], qi = q[u][l][2 * v + 1];
+                        const double tr = t0[u][2 * v], ti = t0[u][2 * v + 1];
+                        a0r[l][v] += qr * tr + qi * ti;
+                        a0i[l][v] += qr * ti - qi * tr;
+                    }
+            } else {
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    y[2 * v] = Real(2) * sr[u][v] - p[u][2 * v];
+                    y[2 * v + 1] = Real(2) * si[u][v] - p[u][2 * v + 1];
                 }
             }
-        } else {
-            const V p = *reinterpret_cast<const V *>(xprev + vo);
+            *reinterpret_cast<V *>(xout + vo) = y;
 #pragma unroll
-            for (int v = 0; v < VPL; ++v) {
-                y[2 * v] = Real(2) * sr[v] - p[2 * v];
-                y[2 * v + 1] = Real(2) * si[v] - p[2 * v + 1];
-            }
-        }
-        *reinterpret_cast<V *>(xout + vo) = y;
-
+            for (int l = 0; l < NL; ++l)
 #pragma unroll
-        for (int l = 0; l < NL; ++l) {
-            const V q = *reinterpret_cast<const V *>(vls[l] + vo);
-#pragma unroll
-            for (int v = 0; v < VPL; ++v) {
-                const double qr = q[2 * v], qi = q[2 * v + 1];
-                const double yr = y[2 * v], yi = y[2 * v + 1];
-                ar[l][v] += qr * yr + qi * yi;
-                ai[l][v] += qr * yi - qi * yr;
-            }
+                for (int v = 0; v < VPL; ++v) {
+                    const double qr = q[u][l][2 * v], qi = q[u][l][2 * v + 1];
+                    const double yr = y[2 * v], yi = y[2 * v + 1];
+                    ar[l][v] += qr * yr + qi * yi;
+                    ai[l][v] += qr * yi - qi * yr;
+                }
         }
     }
 
@@ -287,61 +329,78 @@ __global__ void copy_kernel(const int4 *__restrict__ src, int4 *__restrict__ dst
 // ---- dispatch table for the step kernel ----
 struct StepEntry {
     const void *fn;
-    int vpl, lpr;
+    int vpl, lpr, unr;
 };
 
-template <typename Real, int VPL, int LPR, int NL, int KT>
+template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT>
 const void *step_ptr() {
-    return reinterpret_cast<const void *>(&cheb_step_kernel<Real, VPL, LPR, NL, KT>);
+    return reinterpret_cast<const void *>(&cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT>);
 }
 
-template <typename Real, int VPL, int LPR, int NL>
-const void *pick_k(int k) {
-    if (k == 4) return step_ptr<Real, VPL, LPR, NL, 4>();
-    if (k == 5) return step_ptr<Real, VPL, LPR, NL, 5>();
-    return step_ptr<Real, VPL, LPR, NL, 0>();
+// Variants (UNR, NT) exist for the single-left-set kernels of K = 4 and 5
+// (the lattice workloads); every other shape has the (1, plain) kernel.
+template <typename Real, int VPL, int LPR, int KT>
+const void *pick_variant(int variant) {
+    switch (variant) {
+        case 1: return step_ptr<Real, VPL, LPR, 1, KT, 2, false>();
+        case 2: return step_ptr<Real, VPL, LPR, 1, KT, 1, true>();
+        case 3: return step_ptr<Real, VPL, LPR, 1, KT, 2, true>();
+        default: return step_ptr<Real, VPL, LPR, 1, KT, 1, false>();
+    }
 }
 
 template <typename Real, int VPL, int LPR>
-const void *pick_nl(int nl, int k) {
-    return nl == 2 ? pick_k<Real, VPL, LPR, 2>(k) : pick_k<Real, VPL, LPR, 1>(k);
+const void *pick_nl(int nl, int k, int variant) {
+    if (nl == 2) {
+        if (k == 4) return step_ptr<Real, VPL, LPR, 2, 4, 1, false>();
+        if (k == 5) return step_ptr<Real, VPL, LPR, 2, 5, 1, false>();
+        return step_ptr<Real, VPL, LPR, 2, 0, 1, false>();
+    }
+    if (k == 4) return pick_variant<Real, VPL, LPR, 4>(variant);
+    if (k == 5) return pick_variant<Real, VPL, LPR, 5>(variant);
+    return step_ptr<Real, VPL, LPR, 1, 0, 1, false>();
+}
+
+int variant_unr(int nl, int k, int variant) {
+    return (nl == 1 && (k == 4 || k == 5) && (variant == 1 || variant == 3)) ? 2 : 1;
 }
 
 // Kernel for a group of rb vectors.  wide = 16-byte lanes for c64 (needs
 // 16-byte aligned rows: ld and r0 even).
-StepEntry pick_step(int vtype, int rb, bool wide, int nl, int k) {
+StepEntry pick_step(int vtype, int rb, bool wide, int nl, int k, int variant) {
+    const int u = variant_unr(nl, k, variant);
     if (vtype == JKPM_C64) {
         if (wide) {
             switch (rb) {
-                case 2: return {pick_nl<float, 2, 1>(nl, k), 2, 1};
-                case 4: return {pick_nl<float, 2, 2>(nl, k), 2, 2};
-                case 8: return {pick_nl<float, 2, 4>(nl, k), 2, 4};
-                case 16: return {pick_nl<float, 2, 8>(nl, k), 2, 8};
-                case 32: return {pick_nl<float, 2, 16>(nl, k), 2, 16};
+                case 2: return {pick_nl<float, 2, 1>(nl, k, variant), 2, 1, u};
+                case 4: return {pick_nl<float, 2, 2>(nl, k, variant), 2, 2, u};
+                case 8: return {pick_nl<float, 2, 4>(nl, k, variant), 2, 4, u};
+                case 16: return {pick_nl<float, 2, 8>(nl, k, variant), 2, 8, u};
+                case 32: return {pick_nl<float, 2, 16>(nl, k, variant), 2, 16, u};
                 default: break;
             }
         }
         switch (rb) {
-            case 1: return {pick_nl<float, 1, 1>(nl, k), 1, 1};
-            case 2: return {pick_nl<float, 1, 2>(nl, k), 1, 2};
-            case 4: return {pick_nl<float, 1, 4>(nl, k), 1, 4};
-            case 8: return {pick_nl<float, 1, 8>(nl, k), 1, 8};
-            case 16: return {pick_nl<float, 1, 16>(nl, k), 1, 16};
-            case 32: return {pick_nl<float, 1, 32>(nl, k), 1, 32};
+            case 1: return {pick_nl<float, 1, 1>(nl, k, variant), 1, 1, u};
+            case 2: return {pick_nl<float, 1, 2>(nl, k, variant), 1, 2, u};
+            case 4: return {pick_nl<float, 1, 4>(nl, k, variant), 1, 4, u};
+            case 8: return {pick_nl<float, 1, 8>(nl, k, variant), 1, 8, u};
+            case 16: return {pick_nl<float, 1, 16>(nl, k, variant), 1, 16, u};
+            case 32: return {pick_nl<float, 1, 32>(nl, k, variant), 1, 32, u};
             default: break;
         }
     } else {
         switch (rb) {
-            case 1: return {pick_nl<double, 1, 1>(nl, k), 1, 1};
-            case 2: return {pick_nl<double, 1, 2>(nl, k), 1, 2};
-            case 4: return {pick_nl<double, 1, 4>(nl, k), 1, 4};
-            case 8: return {pick_nl<double, 1, 8>(nl, k), 1, 8};
-            case 16: return {pick_nl<double, 1, 16>(nl, k), 1, 16};
-            case 32: return {pick_nl<double, 1, 32>(nl, k), 1, 32};
+            case 1: return {pick_nl<double, 1, 1>(nl, k, variant), 1, 1, u};
+            case 2: return {pick_nl<double, 1, 2>(nl, k, variant), 1, 2, u};
+            case 4: return {pick_nl<double, 1, 4>(nl, k, variant), 1, 4, u};
+            case 8: return {pick_nl<double, 1, 8>(nl, k, variant), 1, 8, u};
+            case 16: return {pick_nl<double, 1, 16>(nl, k, variant), 1, 16, u};
+            case 32: return {pick_nl<double, 1, 32>(nl, k, variant), 1, 32, u};
             default: break;
         }
     }
-    return {nullptr, 0, 0};
+    return {nullptr, 0, 0, 1};
 }
 
 }  // namespace
@@ -349,9 +408,9 @@ StepEntry pick_step(int vtype, int rb, bool wide, int nl, int k) {
 int max_group(int) { return 32; }
 
 int step_chunks(const Ell &H, int, const DeviceInfo &dev) {
-    // Enough work-groups to give every CU several (8 per CU at most), never
-    // fewer rows per chunk than one wave-row sweep.
-    long target = (long)dev.cu_count * 8;
+    // chunk_mult work-groups per CU (tunable, 8 by default), never fewer rows
+    // per chunk than one wave-row sweep.
+    long target = (long)dev.cu_count * (H.chunk_mult > 0 ? H.chunk_mult : 8);
     long by_rows = (H.n + 63) / 64;
     long c = target < by_rows ? target : by_rows;
     if (c >= kXcd) c = c / kXcd * kXcd;
@@ -364,9 +423,9 @@ int launch_step(const StepArgs &a, hipStream_t s) {
     const int nl = a.vl1 ? 2 : 1;
     const size_t cb = cplx_bytes(H.vtype);
     const bool wide = (a.ld % 2 == 0) && (a.r0 % 2 == 0);
-    StepEntry e = pick_step(H.vtype, a.rb, wide, nl, H.k);
+    StepEntry e = pick_step(H.vtype, a.rb, wide, nl, H.k, H.variant);
     if (!e.fn) return arg_error("unsupported vector group size " + std::to_string(a.rb));
-    const int rpb = kWaves * (64 / e.lpr);
+    const int rpb = kWaves * (64 / e.lpr) * e.unr;
     long rpc = ((long)H.n + a.nchunks - 1) / a.nchunks;
     rpc = (rpc + rpb - 1) / rpb * rpb;
     const size_t off = (size_t)a.r0 * cb;

@@ -50,6 +50,11 @@ struct Ell {
     int k = 0;        // slots per row (len_row)
     int vtype = JKPM_C128;
     double scale = 1.0;    // the stored values are scale * (uploaded values)
+    // Launch shape of the step kernel (jkpm_ell_set_launch / the autotuner):
+    // variant 0..3 = (rows per lane, non-temporal streams) in
+    // {(1,no), (2,no), (1,yes), (2,yes)}; chunk_mult work-groups per CU.
+    int variant = 0;
+    int chunk_mult = 8;
     void *val = nullptr;   // [k][n] complex of vtype
     int *col = nullptr;    // [k][n]
     int device = 0;

@@ -248,6 +248,34 @@ def test_argument_errors(torch_gpu):
         nat.DeviceELL(dd, di, A.shape[0], A.len_row)
 
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_launch_variants_and_autotune(torch_gpu, dtype, tmp_path, monkeypatch):
+    """Every launch shape gives the same moments (same per-row arithmetic;
+    only the FP64 order of the chunk partials changes), and the autotuner
+    keeps one of them and records it in its table."""
+    from minijclsquant_amd import _kpm, autotune
+
+    A = square(40, seed=2)  # K = 5 (variants exist)
+    Ht = A.modifier(jcl.modifier_bounds)
+    r = vectors(A.shape[0], 16, 5)
+    h = nat.DeviceELL(Ht.data, Ht.indices, A.shape[0], A.len_row, 1.0, dtype)
+    vr = _kpm.as_batch(r, A.shape[0], dtype)
+    ref = h.moments(vr, 200)[0].cpu().numpy()
+    for v in range(4):
+        for m in (1, 4, 8, 32):
+            h.set_launch(v, m)
+            got = h.moments(vr, 200)[0].cpu().numpy()
+            assert np.max(np.abs(got - ref)) / abs(ref[0, 0]) <= 1e-12, (v, m)
+    monkeypatch.setenv("JKPM_TUNE_TABLE", str(tmp_path / "t.json"))
+    v, m = autotune.tuned(h, vr, steps=4)
+    assert 0 <= v <= 3 and m in (4, 8, 16, 32)
+    assert (tmp_path / "t.json").exists()
+    h.set_launch(0, 8)
+    assert autotune.tuned(h, vr) == (v, m)  # from the table
+    with pytest.raises(ValueError):
+        h.set_launch(4, 8)
+
+
 def test_precision_form_full_size(torch_gpu):
     """Honeycomb 1024x1024 cells (N = 2,097,152), K = 4, M = 2048, R = 16,
     complex64 recursion + FP64 dots: size-independent properties, and the
