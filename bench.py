@@ -126,6 +126,9 @@ def main():
     ap.add_argument("--profile-every", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dos-check", action="store_true")
+    ap.add_argument("--no-doubling", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shape instead of the tuned one")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware counters: 3 copies of 1 GiB with the "
                          "library's copy kernel, then 16 moments of the workload")
@@ -154,10 +157,12 @@ def main():
     mu = torch.empty((1, M, R), dtype=torch.complex128, device="cuda")
     E = np.linspace(-B + 1e-3, B - 1e-3, args.energies)
     es = torch.as_tensor(E / B, device="cuda")
-    stream = torch.cuda.current_stream()
 
-    def one_step():
-        h.moments(vr, M, workspace=ws, out=mu)
+    def one_step(doubling=False):
+        if doubling:
+            nat.moments_ex(h, None, vr, M, True, workspace=ws_dbl, out=mu[0])
+        else:
+            h.moments(vr, M, workspace=ws, out=mu)
         return nat.dos(mu[0], es, nat.JKPM_KERNEL_JACKSON)
 
     if args.probe:
@@ -172,6 +177,14 @@ def main():
                           "step_launches": 15}), flush=True)
         return
 
+    if not args.no_tune:
+        from minijclsquant_amd import autotune
+
+        v, m = autotune.tuned(h, vr)
+        log(f"launch shape: variant {v}, {m} work-groups per CU")
+        ws = torch.empty(h.workspace_bytes(R, 1), dtype=torch.uint8, device="cuda")
+    ws_dbl = torch.empty(nat.moments_ex_workspace_bytes(h, None, R, M, True), dtype=torch.uint8,
+                         device="cuda")
     for i in range(args.warmup):
         one_step()
         log(f"warmup step {i + 1}/{args.warmup} done")
@@ -179,24 +192,25 @@ def main():
     copy_gbs = copy_ceiling(torch, nat)
     log(f"copy kernel: {copy_gbs:.0f} GB/s (read + write)")
 
-    nat.profile_read(reset=True)
-    nat.profile_enable(args.profile_every)
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        dos = one_step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    nat.profile_enable(0)
-    step_ms_sum, n_prof = nat.profile_read(reset=True)
-    log(f"timed {args.steps} steps in {t1 - t0:.3f} s")
-    del stream
+    def timed(doubling):
+        nat.profile_read(reset=True)
+        nat.profile_enable(args.profile_every)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            d = one_step(doubling)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        nat.profile_enable(0)
+        ms_sum, n_prof = nat.profile_read(reset=True)
+        return (t1 - t0) * 1e3 / args.steps, ms_sum / max(n_prof, 1), n_prof, d
 
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    ms_per_step, launch_ms, n_prof, dos = timed(False)
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms per step")
+
     units = N * M * R
     value = units / (ms_per_step / 1e3)
     s_v = 8 if args.dtype == "c64" else 16
     bytes_launch = N * K * (s_v + 4) + R * N * 4 * s_v
-    launch_ms = step_ms_sum / max(n_prof, 1)
     achieved = bytes_launch / (launch_ms / 1e3) / 1e9
     roofline = {
         "bound": "hbm",
@@ -213,12 +227,35 @@ def main():
         "frac_of_copy": round(achieved / copy_gbs, 4),
     }
 
+    d128 = None
     dos_err = None
     if not args.no_dos_check and args.dtype == "c64":
         log("complex128 reference run for the DOS error")
         d128 = jcl.kpm_dos(A, M, r, energies=E, dtype="c128")[:, 1]
-        d64 = dos.cpu().numpy()
-        dos_err = float(np.max(np.abs(d64 - d128)) / np.max(np.abs(d128)))
+        dos_err = float(np.max(np.abs(dos.cpu().numpy() - d128)) / np.max(np.abs(d128)))
+
+    dbl = None
+    if not args.no_doubling:
+        one_step(True)
+        torch.cuda.synchronize()
+        ms_d, launch_d, n_d, dos_d = timed(True)
+        log(f"doubling form: {ms_d:.2f} ms per step")
+        bytes_d = N * K * (s_v + 4) + R * N * 3 * s_v
+        ach_d = bytes_d / (launch_d / 1e3) / 1e9
+        dbl = {
+            "value": units / (ms_d / 1e3),
+            "ms_per_step": ms_d,
+            "steps_per_recursion": max(1, M // 2),
+            "note": ("same M moments from M/2 Chebyshev steps: mu_2k = 2<T_k|T_k> - mu_0, "
+                     "mu_2k+1 = 2<T_k|T_k+1> - mu_1 (Hermitian H); no v_l stream"),
+            "roofline": {"bound": "hbm", "achieved": round(ach_d, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach_d / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes_per_launch": bytes_d,
+                         "mean_launch_us": round(launch_d * 1e3, 2), "launches_timed": n_d},
+            "dos_err_vs_c128": (None if d128 is None else
+                                float(np.max(np.abs(dos_d.cpu().numpy() - d128))
+                                      / np.max(np.abs(d128)))),
+        }
 
     cpu = None
     if not args.no_cpu:
@@ -248,6 +285,7 @@ def main():
         "dos_err_vs_c128": dos_err,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "doubling": dbl,
     }
     print(json.dumps(out), flush=True)
 

@@ -153,6 +153,46 @@ int jkpm_batch_moments(const jkpm_batch *b, const void *v_r, const void *v_l0,
                        size_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------------------------
+ * Chebyshev series application.
+ * ------------------------------------------------------------------- */
+
+int jkpm_series_workspace(const jkpm_ell *h, int R, int comm, size_t *bytes);
+
+/* v_op == NULL: out = sum_{m<M} c_m T_m(A) v -- replaces rec_A_vec
+ *   (jclsquant/recurrence_functions.py:114-143) and the GPU rec_A
+ *   (jclsquant/cuda_cython/cuda/recursions.cu:267-317).
+ * v_op != NULL (made with jkpm_ell_create_like(h, ...)): the commutator
+ *   recursion C_0 = 0, C_1 = V v, C_m = 2 A C_{m-1} - C_{m-2} + 2 V T_{m-1};
+ *   out = sum_m c_m C_m -- rec_com_A_vec (recurrence_functions.py:146-196,
+ *   GPU rec_comm_A_vec, recursions.cu:319-388) -- and/or the table
+ *   tab[m] = c_m C_m -- rec_com_A_vec_tab (recurrence_functions.py:202-250).
+ * c: host complex128 [M] (re, im); v, out: device [N][R]; tab: device
+ * [M][N][R]; all of the handle's value type.  Asynchronous on stream. */
+int jkpm_series(const jkpm_ell *h, const jkpm_ell *v_op, const void *v, int R, int M,
+                const double *c, void *out, void *tab, void *workspace,
+                size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Single-left-set moments mu_m = <r|T_m(A)|r> with options.
+ * ------------------------------------------------------------------- */
+
+/* Doubling form: for Hermitian A only (not checked), the M moments come from
+ * S = max(1, M/2) Chebyshev steps through mu_2k = 2<T_k|T_k> - mu_0 and
+ * mu_2k+1 = 2<T_k|T_{k+1}> - mu_1 (Weisse et al., Rev. Mod. Phys. 78, 275
+ * (2006)); each step reads no left vectors.  Same moments as the plain
+ * recursion up to rounding.  No reference counterpart: the reference runs
+ * M-1 steps with a dot against the stored vector
+ * (jclsquant/recurrence_functions.py:31-38). */
+#define JKPM_MOMENTS_DOUBLING 1
+
+/* Either a handle (mu [M][R]) or a batch (mu [G][M][R]); v_r device [N][R]. */
+int jkpm_moments_ex_workspace(const jkpm_ell *h, const jkpm_batch *b, int R, int M, int flags,
+                              size_t *bytes);
+int jkpm_moments_ex(const jkpm_ell *h, const jkpm_batch *b, const void *v_r, int R, int M,
+                    int flags, double *mu, void *workspace, size_t workspace_bytes,
+                    void *stream);
+
+/* ---------------------------------------------------------------------
  * HIP graphs: capture the launches of asynchronous entry points (moments,
  * batch moments, DOS, ...) issued on `stream` between begin and end, then
  * replay them with one launch.  Pointers and sizes are frozen at capture.

@@ -34,12 +34,18 @@ def as_batch(v, n: int, dtype) -> "torch.Tensor":
     return t.to(device="cuda", dtype=cd).transpose(0, 1).contiguous()
 
 
-def moments(A, M: int, v_r, v_l=None, v_l2=None, dtype="c128", scale: float = 1.0):
+def moments(A, M: int, v_r, v_l=None, v_l2=None, dtype="c128", scale: float = 1.0,
+            doubling: bool = False):
     """mu[l, r, m] = <v_l{l}[r] | T_m(scale*A) | v_r[r]> as a device tensor
-    complex128 [n_left, R, M]; A is an ``ell_matrix``."""
+    complex128 [n_left, R, M]; A is an ``ell_matrix``.  doubling (v_l = v_r,
+    Hermitian A): two moments per Chebyshev step (jkpm_moments_ex)."""
     n = int(A.shape[0])
     h = A.device(dtype, scale)
     vr = as_batch(v_r, n, dtype)
+    if doubling:
+        if v_l is not None or v_l2 is not None:
+            raise ValueError("the doubling form needs v_l = v_r")
+        return nat.moments_ex(h, None, vr, M, True).transpose(0, 1)[None]
     vl = vr if v_l is None else as_batch(v_l, n, dtype)
     vl2 = None if v_l2 is None else as_batch(v_l2, n, dtype)
     for w in (vl, vl2):

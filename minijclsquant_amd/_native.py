@@ -74,7 +74,71 @@ SIGNATURES = {
     "jkpm_ell_get_launch": (_int, [_vp, _ip, _ip]),
     "jkpm_autotune_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
     "jkpm_autotune": (_int, [_vp, _vp, _int, _int, _vp, _size, _vp, _dp]),
+    "jkpm_moments_ex_workspace": (_int, [_vp, _vp, _int, _int, _int, _c.POINTER(_size)]),
+    "jkpm_moments_ex": (_int, [_vp, _vp, _vp, _int, _int, _int, _vp, _vp, _size, _vp]),
+    "jkpm_series_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
+    "jkpm_series": (_int, [_vp, _vp, _vp, _int, _int, _vp, _vp, _vp, _vp, _size, _vp]),
 }
+
+
+def series(h, v_op, v, M: int, c, want_sum: bool = True, want_table: bool = False,
+           stream=None):
+    """sum_m c_m X_m ([N, R]) and/or the table c_m X_m ([M, N, R]) with
+    X = T_m(H) v (v_op None) or the commutator recursion (v_op on H's slots)."""
+    torch = _torch()
+    R = h._check_vecs(v)
+    M = int(M)
+    cc = np.ascontiguousarray(np.asarray(c, dtype=np.complex128).reshape(-1))
+    if cc.size < M or M <= 0:
+        raise ValueError("need M > 0 coefficients")
+    ws_n = _size()
+    check(lib().jkpm_series_workspace(h.handle, R, 1 if v_op is not None else 0,
+                                      _c.byref(ws_n)))
+    ws = torch.empty(int(ws_n.value), dtype=torch.uint8, device=v.device)
+    out = torch.empty_like(v) if want_sum else None
+    tab = torch.empty((M,) + tuple(v.shape), dtype=v.dtype, device=v.device) if want_table else None
+    check(lib().jkpm_series(h.handle, v_op.handle if v_op is not None else None, v.data_ptr(), R,
+                            M, cc.ctypes.data, out.data_ptr() if out is not None else None,
+                            tab.data_ptr() if tab is not None else None, ws.data_ptr(),
+                            ws.numel(), stream_ptr(stream)))
+    return out, tab
+
+JKPM_MOMENTS_DOUBLING = 1
+
+
+def moments_ex_workspace_bytes(h, b, R: int, M: int, doubling: bool) -> int:
+    need = _size()
+    check(lib().jkpm_moments_ex_workspace(h.handle if h is not None else None,
+                                          b._b if b is not None else None, int(R), int(M),
+                                          JKPM_MOMENTS_DOUBLING if doubling else 0,
+                                          _c.byref(need)))
+    return int(need.value)
+
+
+def moments_ex(h, b, v_r, M: int, doubling: bool, workspace=None, out=None, stream=None):
+    """Single-left-set moments through jkpm_moments_ex: handle h ([M, R]) or
+    batch b ([G, M, R]); doubling selects the two-moments-per-step form."""
+    torch = _torch()
+    H = b.H if b is not None else h
+    R = H._check_vecs(v_r)
+    M = int(M)
+    if M <= 0:
+        raise ValueError("M must be positive")
+    flags = JKPM_MOMENTS_DOUBLING if doubling else 0
+    hp = h.handle if h is not None else None
+    bp = b._b if b is not None else None
+    need = _size()
+    check(lib().jkpm_moments_ex_workspace(hp, bp, R, M, flags, _c.byref(need)))
+    if workspace is None or workspace.numel() < need.value:
+        workspace = torch.empty(int(need.value), dtype=torch.uint8, device=v_r.device)
+    shape = (b.G, M, R) if b is not None else (M, R)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.complex128, device=v_r.device)
+    elif tuple(out.shape) != shape or out.dtype != torch.complex128 or not out.is_contiguous():
+        raise ValueError(f"out must be contiguous complex128 {list(shape)}")
+    check(lib().jkpm_moments_ex(hp, bp, v_r.data_ptr(), R, M, flags, out.data_ptr(),
+                                workspace.data_ptr(), workspace.numel(), stream_ptr(stream)))
+    return out
 
 
 class JkpmError(RuntimeError):

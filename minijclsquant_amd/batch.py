@@ -74,33 +74,45 @@ class BatchKPM:
     of each realization and the scale 1/B_g of its spectral rescale.
     vectors: list of [R, N_g] arrays (same R for every realization)."""
 
-    def __init__(self, blocks, vectors, M, dtype="c64"):
+    def __init__(self, blocks, vectors, M, dtype="c64", doubling=False):
         torch = nat.require_device()
         data, idx, offsets, K = block_diagonal(blocks)
         self.offsets = offsets
         self.M = int(M)
+        self.doubling = bool(doubling)
         self.H = nat.DeviceELL(data, idx, int(offsets[-1]), K, 1.0, dtype)
         self.batch = nat.Batch(self.H, offsets)
         v = np.concatenate([np.asarray(vg).reshape(-1, int(b[2])) for vg, b in zip(vectors, blocks)],
                            axis=1)
         self.R = v.shape[0]
         self.vr = _kpm.as_batch(v, int(offsets[-1]), dtype)
-        self.ws = torch.empty(self.batch.workspace_bytes(self.R), dtype=torch.uint8, device="cuda")
+        self.ws = None
         self.mu = torch.empty((self.batch.G, self.M, self.R), dtype=torch.complex128, device="cuda")
         self.graph = None
         self.stream = None
 
+    def _launch(self, stream):
+        if self.ws is None:
+            import torch
+
+            need = nat.moments_ex_workspace_bytes(None, self.batch, self.R, self.M, self.doubling)
+            self.ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+        return nat.moments_ex(None, self.batch, self.vr, self.M, self.doubling,
+                              workspace=self.ws, out=self.mu, stream=stream)
+
     def run(self, stream=None):
         """Eager launch of the whole batch; returns mu [G, M, R] (device)."""
-        return self.batch.moments(self.vr, self.M, workspace=self.ws, out=self.mu, stream=stream)
+        return self._launch(stream)
 
     def capture(self):
         """Captures the recursion into a HIP graph on a private stream."""
         torch = nat.require_device()
         self.stream = torch.cuda.Stream()
         self.stream.wait_stream(torch.cuda.current_stream())
+        self._launch(torch.cuda.current_stream())  # allocates the workspace outside capture
+        torch.cuda.synchronize()
         with nat.Graph.capture(self.stream) as g:
-            self.batch.moments(self.vr, self.M, workspace=self.ws, out=self.mu, stream=self.stream)
+            self._launch(self.stream)
         self.graph = g
         return g
 
@@ -117,7 +129,7 @@ class BatchKPM:
 
 
 def kpm_dos_batch(mats, M, random_vectors, energies=None, dtype="c64", kernel="jackson",
-                  graph=True):
+                  graph=True, doubling=False):
     """DOS tables [E, 2] for a list of ell_matrix realizations, computed in
     one batch (random_vectors: list of [R, N_g] or [N_g] arrays)."""
     blocks = []
@@ -127,7 +139,7 @@ def kpm_dos_batch(mats, M, random_vectors, energies=None, dtype="c64", kernel="j
             raise ValueError("batched realizations need symmetric bounds")
         blocks.append((A.data, A.indices, int(A.shape[0]), int(A.len_row), s))
     vecs = [np.atleast_2d(v) for v in random_vectors]
-    bk = BatchKPM(blocks, vecs, M, dtype)
+    bk = BatchKPM(blocks, vecs, M, dtype, doubling)
     mu = bk.replay() if graph else bk.run()
     out = []
     for g, A in enumerate(mats):

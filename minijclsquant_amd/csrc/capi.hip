@@ -83,9 +83,12 @@ __global__ void count_bad_indices(const int *__restrict__ idx, long tot, int n,
 bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 struct MomentsLayout {
-    size_t b0 = 0, b1 = 0, ring = 0, total = 0;
+    size_t b0 = 0, b1 = 0, ring = 0, raw = 0, total = 0;
     int nchunks = 1;
 };
+
+// Steps of the doubling form for M moments (T_1 .. T_S, S = max(1, M/2)).
+int doubling_steps(int M) { return std::max(1, M / 2); }
 
 // Row-chunk table of a block-diagonal batch of G realizations sharing one
 // handle: chunks never straddle two realizations, and chunks of group g are
@@ -97,15 +100,23 @@ struct Groups {
     int nchunks = 0;
 };
 
+// doubling_M > 0: the doubling form for that many moments (two partial sets
+// and a raw-dot buffer [2][G][S+1][R]).
 MomentsLayout moments_layout(const Ell &H, int R, int n_left, const DeviceInfo &dev,
-                             const Groups *grp = nullptr) {
+                             const Groups *grp = nullptr, int doubling_M = 0) {
     MomentsLayout L;
     const size_t vec = align_up((size_t)H.n * R * cplx_bytes(H.vtype), 256);
+    if (doubling_M > 0) n_left = 2;
     L.nchunks = grp ? grp->nchunks : step_chunks(H, R, dev);
     L.b0 = 0;
     L.b1 = vec;
     L.ring = 2 * vec;
-    L.total = L.ring + align_up((size_t)kRing * n_left * L.nchunks * R * sizeof(double2), 256);
+    L.raw = L.ring + align_up((size_t)kRing * n_left * L.nchunks * R * sizeof(double2), 256);
+    L.total = L.raw;
+    if (doubling_M > 0) {
+        const size_t G = grp ? grp->G : 1;
+        L.total += align_up(2 * G * (doubling_steps(doubling_M) + 1) * R * sizeof(double2), 256);
+    }
     return L;
 }
 
@@ -144,15 +155,17 @@ int device_info(DeviceInfo *out) {
 }
 
 // Runs the whole recursion for one batch; see jkpm_moments.  With grp, the
-// handle is a block-diagonal batch and mu is [n_left][G][M][R].
+// handle is a block-diagonal batch and mu is [n_left][G][M][R].  With
+// doubling (v_l = v_r, Hermitian A), S = max(1, M/2) steps produce the M
+// moments through the raw dots <T_k|T_k>, <T_k|T_{k+1}>.
 static int run_moments(const Ell &H, const void *vr, const void *vl0, const void *vl1, int R,
                        int M, double2 *mu, void *ws, size_t ws_bytes, hipStream_t s,
-                       const Groups *grp = nullptr) {
+                       const Groups *grp = nullptr, bool doubling = false) {
     DeviceInfo dev;
     int rc = device_info(&dev);
     if (rc) return rc;
-    const int n_left = vl1 ? 2 : 1;
-    const MomentsLayout L = moments_layout(H, R, n_left, dev, grp);
+    const int n_left = (vl1 || doubling) ? 2 : 1;
+    const MomentsLayout L = moments_layout(H, R, n_left, dev, grp, doubling ? M : 0);
     if (ws_bytes < L.total) {
         set_error("workspace too small: need " + std::to_string(L.total) + " bytes, got " +
                   std::to_string(ws_bytes));
@@ -190,8 +203,9 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
         a.xcur = (n == 1) ? vr : B[(n - 1) & 1];
         a.xprev = (n == 1) ? nullptr : (n == 2 ? vr : B[n & 1]);
         a.xout = B[n & 1];
-        a.vl0 = vl0;
-        a.vl1 = vl1;
+        a.vl0 = doubling ? nullptr : vl0;
+        a.vl1 = doubling ? nullptr : vl1;
+        a.doubling = doubling;
         a.ld = R;
         a.part = ring + (size_t)(n % kRing) * slot;
         a.part0 = (n == 1) ? ring : nullptr;  // step 0 lives in ring slot 0
@@ -222,19 +236,27 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
         return JKPM_OK;
     };
 
-    // Step 1 also yields mu_0; M == 1 still runs it (T_1 is discarded).
+    // Step 1 also yields the step-0 partials; M == 1 still runs it (T_1 is
+    // discarded).  Plain form: steps 1 .. M-1 reduce straight into mu.
+    // Doubling form: steps 1 .. S reduce into the raw dots [2][G][S+1][R].
+    const int G = grp ? grp->G : 1;
+    const int nsteps = doubling ? doubling_steps(M) : std::max(M, 2) - 1;
+    const int last_slot = doubling ? nsteps : M - 1;
+    double2 *target = doubling ? (double2 *)(w + L.raw) : mu;
+    const int target_M = doubling ? nsteps + 1 : M;
     int pending_lo = 0;
-    for (int n = 1; n < std::max(M, 2); ++n) {
+    for (int n = 1; n <= nsteps; ++n) {
         rc = step(n);
         if (rc) return rc;
-        const int last = std::min(n, M - 1);
-        if (n == std::max(M, 2) - 1 || n - pending_lo + 1 == kRing) {
+        const int last = std::min(n, last_slot);
+        if (n == nsteps || n - pending_lo + 1 == kRing) {
             rc = launch_reduce(ring, kRing, pending_lo, last - pending_lo + 1, n_left, L.nchunks,
-                               R, mu, M, grp ? grp->group_chunks : nullptr, grp ? grp->G : 1, s);
+                               R, target, target_M, grp ? grp->group_chunks : nullptr, G, s);
             if (rc) return rc;
             pending_lo = n + 1;
         }
     }
+    if (doubling) return launch_doubling_finish(target, G, target_M, R, M, mu, s);
     return JKPM_OK;
 }
 
@@ -457,10 +479,14 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
 
 // ---- launch shape and autotuner ----
 
+namespace {
+constexpr int kVariants = 12;  // unroll x non-temporal x lane shape (cheb_step.h)
+}
+
 int jkpm_ell_set_launch(jkpm_ell *h, int variant, int chunk_mult) {
     clear_error();
     if (!h) return arg_error("handle is NULL");
-    if (variant < 0 || variant > 3) return arg_error("variant must be 0..3");
+    if (variant < 0 || variant >= kVariants) return arg_error("variant must be 0..11");
     if (chunk_mult < 1 || chunk_mult > 64) return arg_error("chunk_mult must be 1..64");
     h->e.variant = variant;
     h->e.chunk_mult = chunk_mult;
@@ -476,7 +502,7 @@ int jkpm_ell_get_launch(const jkpm_ell *h, int *variant, int *chunk_mult) {
 }
 
 namespace {
-const int kTuneMult[] = {4, 8, 16, 32};
+const int kTuneMult[] = {2, 4, 8, 16, 32};
 }
 
 int jkpm_autotune_workspace(const jkpm_ell *h, int R, int steps, size_t *bytes) {
@@ -519,7 +545,7 @@ int jkpm_autotune(jkpm_ell *h, const void *v_r, int R, int steps, void *workspac
     const int keep_v = h->e.variant, keep_m = h->e.chunk_mult;
     double best = 1e300;
     int bv = keep_v, bm = keep_m;
-    for (int v = 0; v < 4 && !rc; ++v)
+    for (int v = 0; v < kVariants && !rc; ++v)
         for (int m : kTuneMult) {
             h->e.variant = v;
             h->e.chunk_mult = m;
@@ -651,6 +677,81 @@ int jkpm_batch_moments(const jkpm_batch *b, const void *v_r, const void *v_l0, c
     const Groups g = b->grp();
     return run_moments(b->h->e, v_r, v_l0, v_l1, R, M, (double2 *)mu, workspace, workspace_bytes,
                        (hipStream_t)stream, &g);
+}
+
+// ---- Chebyshev series application and the commutator recursion ----
+
+int jkpm_series_workspace(const jkpm_ell *h, int R, int comm, size_t *bytes) {
+    clear_error();
+    if (!h || !bytes) return arg_error("NULL argument");
+    if (R <= 0) return arg_error("R must be positive");
+    *bytes = series_workspace(h->e, R, comm != 0);
+    return JKPM_OK;
+}
+
+int jkpm_series(const jkpm_ell *h, const jkpm_ell *v_op, const void *v, int R, int M,
+                const double *c, void *out, void *tab, void *workspace, size_t workspace_bytes,
+                void *stream) {
+    clear_error();
+    if (!h || !v || !c || !workspace) return arg_error("NULL argument");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    if (!out && !tab) return arg_error("out and tab are both NULL");
+    if (tab && !v_op) return arg_error("the table form needs the commutator operator");
+    if (v_op && (v_op->e.col != h->e.col || v_op->e.vtype != h->e.vtype))
+        return arg_error("the operator must share H's slot pattern and value type "
+                         "(jkpm_ell_create_like)");
+    if (!aligned16(workspace) || ((uintptr_t)v & 7) || (out && ((uintptr_t)out & 7)) ||
+        (tab && ((uintptr_t)tab & 15)))
+        return arg_error("misaligned device pointer");
+    if (workspace_bytes < series_workspace(h->e, R, v_op != nullptr)) {
+        set_error("workspace too small for jkpm_series");
+        return JKPM_ERR_WORKSPACE;
+    }
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    return run_series(h->e, v_op ? &v_op->e : nullptr, v, R, M, c, out, tab, workspace,
+                      step_chunks(h->e, R, dev), (hipStream_t)stream);
+}
+
+// ---- single-left-set moments with flags (doubling form) ----
+
+int jkpm_moments_ex_workspace(const jkpm_ell *h, const jkpm_batch *b, int R, int M, int flags,
+                              size_t *bytes) {
+    clear_error();
+    if ((!h && !b) || !bytes) return arg_error("NULL argument");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    if (flags & ~JKPM_MOMENTS_DOUBLING) return arg_error("unknown flags");
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const bool dbl = (flags & JKPM_MOMENTS_DOUBLING) != 0;
+    if (b) {
+        const Groups g = b->grp();
+        *bytes = moments_layout(b->h->e, R, 1, dev, &g, dbl ? M : 0).total;
+    } else {
+        *bytes = moments_layout(h->e, R, 1, dev, nullptr, dbl ? M : 0).total;
+    }
+    return JKPM_OK;
+}
+
+int jkpm_moments_ex(const jkpm_ell *h, const jkpm_batch *b, const void *v_r, int R, int M,
+                    int flags, double *mu, void *workspace, size_t workspace_bytes, void *stream) {
+    clear_error();
+    if (!h && !b) return arg_error("handle and batch are both NULL");
+    if (!v_r || !mu || !workspace) return arg_error("NULL vector/mu/workspace");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    if (flags & ~JKPM_MOMENTS_DOUBLING) return arg_error("unknown flags");
+    if (!aligned16(v_r) || !aligned16(workspace) || !aligned16(mu))
+        return arg_error("device pointers must be 16-byte aligned");
+    const bool dbl = (flags & JKPM_MOMENTS_DOUBLING) != 0;
+    if (b) {
+        const Groups g = b->grp();
+        return run_moments(b->h->e, v_r, v_r, nullptr, R, M, (double2 *)mu, workspace,
+                           workspace_bytes, (hipStream_t)stream, &g, dbl);
+    }
+    return run_moments(h->e, v_r, v_r, nullptr, R, M, (double2 *)mu, workspace, workspace_bytes,
+                       (hipStream_t)stream, nullptr, dbl);
 }
 
 // ---- HIP graph capture of asynchronous entry points ----

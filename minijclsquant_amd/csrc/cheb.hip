@@ -23,7 +23,11 @@
 //                 wave order in LDS, gives one partial per (chunk, vector).
 //                 reduce_kernel sums the partials in fixed chunk order, so the
 //                 result is bitwise reproducible run to run.
-#include "common.h"
+//
+// The step kernel template lives in cheb_step.h and is instantiated per
+// precision in cheb_c64.hip / cheb_c128.hip; this file holds its launcher and
+// the other small kernels.
+#include "cheb_step.h"
 
 namespace jkpm {
 namespace {
@@ -32,218 +36,6 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kXcd = 8;
 
-template <typename Real, int N>
-using vec_t = Real __attribute__((ext_vector_type(N)));
-
-// Loads of data read once per step (T_{n-1}, v_l): non-temporal when NT, so
-// they do not displace the gather window of T_n from the caches.
-template <bool NT, typename V>
-__device__ __forceinline__ V load_stream(const V *p) {
-    if constexpr (NT)
-        return __builtin_nontemporal_load(p);
-    else
-        return *p;
-}
-
-// UNR: rows per lane per loop iteration (all their loads are issued before
-// any is used, for more bytes in flight per wave).
-template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT>
-__global__ __launch_bounds__(kBlock) void cheb_step_kernel(
-    const Real *__restrict__ val, const int *__restrict__ col, int n, int k_rt,
-    const Real *__restrict__ xcur, const Real *xprev, Real *xout,
-    const Real *__restrict__ vl0, const Real *__restrict__ vl1, int ld,
-    double2 *__restrict__ part, double2 *__restrict__ part0, int first,
-    int rows_per_chunk, int nchunks, const long *__restrict__ chunk_rows) {
-    using V = vec_t<Real, 2 * VPL>;
-    using C2 = vec_t<Real, 2>;
-    constexpr int RPW = 64 / LPR;
-    constexpr int RB = VPL * LPR;
-    const int K = KT > 0 ? KT : k_rt;
-
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int lv = lane % LPR;
-    const int lr = lane / LPR;
-
-    const int b = blockIdx.x;
-    const int chunk = (nchunks % kXcd == 0) ? (b % kXcd) * (nchunks / kXcd) + b / kXcd : b;
-    // Uniform chunks, or an explicit row table (chunks that never straddle
-    // two realizations of a block-diagonal batch).
-    const long row_begin = chunk_rows ? chunk_rows[chunk] : (long)chunk * rows_per_chunk;
-    const long row_end =
-        chunk_rows ? chunk_rows[chunk + 1] : min((long)n, row_begin + rows_per_chunk);
-
-    double ar[NL][VPL], ai[NL][VPL];
-    double a0r[NL][VPL], a0i[NL][VPL];
-#pragma unroll
-    for (int l = 0; l < NL; ++l)
-#pragma unroll
-        for (int v = 0; v < VPL; ++v) ar[l][v] = ai[l][v] = a0r[l][v] = a0i[l][v] = 0.0;
-
-    const int voff = lv * VPL * 2;
-    const Real *vls[2] = {vl0, vl1};
-
-    constexpr int STRIDE = kWaves * RPW;  // rows between a lane's consecutive rows
-    for (long base = row_begin + wave * RPW + lr; base < row_end; base += (long)STRIDE * UNR) {
-        // Rows past the chunk end are clamped to `base` (a valid row) for the
-        // loads and their results are dropped.
-        long rows[UNR];
-        bool ok[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            const long rw = base + (long)u * STRIDE;
-            ok[u] = rw < row_end;
-            rows[u] = ok[u] ? rw : base;
-        }
-        Real sr[UNR][VPL], si[UNR][VPL];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u)
-#pragma unroll
-            for (int v = 0; v < VPL; ++v) sr[u][v] = si[u][v] = Real(0);
-
-        if constexpr (KT > 0) {
-            int j[UNR][KT];
-            C2 h[UNR][KT];
-#pragma unroll
-            for (int u = 0; u < UNR; ++u)
-#pragma unroll
-                for (int s = 0; s < KT; ++s) {
-                    const long e = (long)s * n + rows[u];
-                    j[u][s] = col[e];
-                    h[u][s] = *reinterpret_cast<const C2 *>(val + 2 * e);
-                }
-            V x[UNR][KT];
-#pragma unroll
-            for (int u = 0; u < UNR; ++u)
-#pragma unroll
-                for (int s = 0; s < KT; ++s)
-                    x[u][s] = *reinterpret_cast<const V *>(xcur + (long)j[u][s] * ld * 2 + voff);
-#pragma unroll
-            for (int u = 0; u < UNR; ++u)
-#pragma unroll
-                for (int s = 0; s < KT; ++s)
-#pragma unroll
-                    for (int v = 0; v < VPL; ++v) {
-                        sr[u][v] += h[u][s].x * x[u][s][2 * v] - h[u][s].y * x[u][s][2 * v + 1];
-                        si[u][v] += h[u][s].x * x[u][s][2 * v + 1] + h[u][s].y * x[u][s][2 * v];
-                    }
-        } else {
-#pragma unroll
-            for (int u = 0; u < UNR; ++u)
-                for (int s = 0; s < K; ++s) {
-                    const long e = (long)s * n + rows[u];
-                    const int j = col[e];
-                    const C2 h = *reinterpret_cast<const C2 *>(val + 2 * e);
-                    const V x = *reinterpret_cast<const V *>(xcur + (long)j * ld * 2 + voff);
-#pragma unroll
-                    for (int v = 0; v < VPL; ++v) {
-                        sr[u][v] += h.x * x[2 * v] - h.y * x[2 * v + 1];
-                        si[u][v] += h.x * x[2 * v + 1] + h.y * x[2 * v];
-                    }
-                }
-        }
-
-        // T_{n-1} and v_l of every row first, then the arithmetic.
-        V p[UNR], q[UNR][NL], t0[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            const long vo = rows[u] * ld * 2 + voff;
-            if (first)
-                t0[u] = *reinterpret_cast<const V *>(xcur + vo);
-            else
-                p[u] = load_stream<NT>(reinterpret_cast<const V *>(xprev + vo));
-#pragma unroll
-            for (int l = 0; l < NL; ++l) q[u][l] = load_stream<NT>(reinterpret_cast<const V *>(vls[l] + vo));
-        }
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            if (!ok[u]) continue;
-            const long vo = rows[u] * ld * 2 + voff;
-            V y;
-            if (first) {
-#pragma unroll
-                for (int v = 0; v < VPL; ++v) {
-                    y[2 * v] = sr[u][v];
-                    y[2 * v + 1] = si[u][v];
-                }
-#pragma unroll
-                for (int l = 0; l < NL; ++l)
-#pragma unroll
-                    for (int v = 0; v < VPL; ++v) {
-                        const double qr = q[u][l][2 * v], qi = q[u][l][2 * v + 1];
-                        const double tr = t0[u][2 * v], ti = t0[u][2 * v + 1];
-                        a0r[l][v] += qr * tr + qi * ti;
-                        a0i[l][v] += qr * ti - qi * tr;
-                    }
-            } else {
-#pragma unroll
-                for (int v = 0; v < VPL; ++v) {
-                    y[2 * v] = Real(2) * sr[u][v] - p[u][2 * v];
-                    y[2 * v + 1] = Real(2) * si[u][v] - p[u][2 * v + 1];
-                }
-            }
-            *reinterpret_cast<V *>(xout + vo) = y;
-#pragma unroll
-            for (int l = 0; l < NL; ++l)
-#pragma unroll
-                for (int v = 0; v < VPL; ++v) {
-                    const double qr = q[u][l][2 * v], qi = q[u][l][2 * v + 1];
-                    const double yr = y[2 * v], yi = y[2 * v + 1];
-                    ar[l][v] += qr * yr + qi * yi;
-                    ai[l][v] += qr * yi - qi * yr;
-                }
-        }
-    }
-
-    // Butterfly over the lanes that hold the same vector index.
-#pragma unroll
-    for (int o = LPR; o < 64; o <<= 1) {
-#pragma unroll
-        for (int l = 0; l < NL; ++l)
-#pragma unroll
-            for (int v = 0; v < VPL; ++v) {
-                ar[l][v] += __shfl_xor(ar[l][v], o);
-                ai[l][v] += __shfl_xor(ai[l][v], o);
-                if (first) {
-                    a0r[l][v] += __shfl_xor(a0r[l][v], o);
-                    a0i[l][v] += __shfl_xor(a0i[l][v], o);
-                }
-            }
-    }
-
-    __shared__ double2 sh[2][kWaves][NL][RB];
-    if (lr == 0) {
-#pragma unroll
-        for (int l = 0; l < NL; ++l)
-#pragma unroll
-            for (int v = 0; v < VPL; ++v) {
-                sh[0][wave][l][lv * VPL + v] = make_double2(ar[l][v], ai[l][v]);
-                sh[1][wave][l][lv * VPL + v] = make_double2(a0r[l][v], a0i[l][v]);
-            }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < NL * RB; t += kBlock) {
-        const int l = t / RB, r = t % RB;
-        double sr2 = 0.0, si2 = 0.0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            sr2 += sh[0][w][l][r].x;
-            si2 += sh[0][w][l][r].y;
-        }
-        part[((long)l * nchunks + chunk) * ld + r] = make_double2(sr2, si2);
-        if (first) {
-            double s0r = 0.0, s0i = 0.0;
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                s0r += sh[1][w][l][r].x;
-                s0i += sh[1][w][l][r].y;
-            }
-            part0[((long)l * nchunks + chunk) * ld + r] = make_double2(s0r, s0i);
-        }
-    }
-}
-
-// One work-group per (vector, left set x group, step): fixed-order sum of the
 // chunk partials of that step over the group's chunks (all chunks when there
 // are no groups).
 __global__ __launch_bounds__(kBlock) void reduce_kernel(const double2 *__restrict__ ring,
@@ -320,87 +112,53 @@ __global__ void aaxby_kernel(const Real *__restrict__ val, const int *__restrict
     y[2 * t + 1] = ci;
 }
 
-__global__ void copy_kernel(const int4 *__restrict__ src, int4 *__restrict__ dst, long n16) {
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
-        dst[i] = src[i];
-}
-
-// ---- dispatch table for the step kernel ----
-struct StepEntry {
-    const void *fn;
-    int vpl, lpr, unr;
-};
-
-template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT>
-const void *step_ptr() {
-    return reinterpret_cast<const void *>(&cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT>);
-}
-
-// Variants (UNR, NT) exist for the single-left-set kernels of K = 4 and 5
-// (the lattice workloads); every other shape has the (1, plain) kernel.
-template <typename Real, int VPL, int LPR, int KT>
-const void *pick_variant(int variant) {
-    switch (variant) {
-        case 1: return step_ptr<Real, VPL, LPR, 1, KT, 2, false>();
-        case 2: return step_ptr<Real, VPL, LPR, 1, KT, 1, true>();
-        case 3: return step_ptr<Real, VPL, LPR, 1, KT, 2, true>();
-        default: return step_ptr<Real, VPL, LPR, 1, KT, 1, false>();
-    }
-}
-
-template <typename Real, int VPL, int LPR>
-const void *pick_nl(int nl, int k, int variant) {
-    if (nl == 2) {
-        if (k == 4) return step_ptr<Real, VPL, LPR, 2, 4, 1, false>();
-        if (k == 5) return step_ptr<Real, VPL, LPR, 2, 5, 1, false>();
-        return step_ptr<Real, VPL, LPR, 2, 0, 1, false>();
-    }
-    if (k == 4) return pick_variant<Real, VPL, LPR, 4>(variant);
-    if (k == 5) return pick_variant<Real, VPL, LPR, 5>(variant);
-    return step_ptr<Real, VPL, LPR, 1, 0, 1, false>();
-}
-
-int variant_unr(int nl, int k, int variant) {
-    return (nl == 1 && (k == 4 || k == 5) && (variant == 1 || variant == 3)) ? 2 : 1;
-}
-
-// Kernel for a group of rb vectors.  wide = 16-byte lanes for c64 (needs
-// 16-byte aligned rows: ld and r0 even).
-StepEntry pick_step(int vtype, int rb, bool wide, int nl, int k, int variant) {
-    const int u = variant_unr(nl, k, variant);
-    if (vtype == JKPM_C64) {
-        if (wide) {
-            switch (rb) {
-                case 2: return {pick_nl<float, 2, 1>(nl, k, variant), 2, 1, u};
-                case 4: return {pick_nl<float, 2, 2>(nl, k, variant), 2, 2, u};
-                case 8: return {pick_nl<float, 2, 4>(nl, k, variant), 2, 4, u};
-                case 16: return {pick_nl<float, 2, 8>(nl, k, variant), 2, 8, u};
-                case 32: return {pick_nl<float, 2, 16>(nl, k, variant), 2, 16, u};
-                default: break;
-            }
-        }
-        switch (rb) {
-            case 1: return {pick_nl<float, 1, 1>(nl, k, variant), 1, 1, u};
-            case 2: return {pick_nl<float, 1, 2>(nl, k, variant), 1, 2, u};
-            case 4: return {pick_nl<float, 1, 4>(nl, k, variant), 1, 4, u};
-            case 8: return {pick_nl<float, 1, 8>(nl, k, variant), 1, 8, u};
-            case 16: return {pick_nl<float, 1, 16>(nl, k, variant), 1, 16, u};
-            case 32: return {pick_nl<float, 1, 32>(nl, k, variant), 1, 32, u};
-            default: break;
-        }
+// Moments from the doubling form's dots (see launch_doubling_finish).
+__global__ void doubling_finish_kernel(const double2 *__restrict__ raw, int G, int S1, int R,
+                                       int M, double2 *__restrict__ mu) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)G * M * R) return;
+    const int r = (int)(t % R);
+    const int m = (int)((t / R) % M);
+    const int g = (int)(t / ((long)R * M));
+    auto at = [&](int l, int slot) { return raw[(((long)l * G + g) * S1 + slot) * R + r]; };
+    double2 v;
+    if (m == 0) {
+        v = at(0, 0);
+    } else if (m == 1) {
+        v = at(1, 1);
+    } else if ((m & 1) == 0) {
+        const double2 a = at(0, m / 2), a0 = at(0, 0);
+        v = make_double2(2.0 * a.x - a0.x, 2.0 * a.y - a0.y);
     } else {
-        switch (rb) {
-            case 1: return {pick_nl<double, 1, 1>(nl, k, variant), 1, 1, u};
-            case 2: return {pick_nl<double, 1, 2>(nl, k, variant), 1, 2, u};
-            case 4: return {pick_nl<double, 1, 4>(nl, k, variant), 1, 4, u};
-            case 8: return {pick_nl<double, 1, 8>(nl, k, variant), 1, 8, u};
-            case 16: return {pick_nl<double, 1, 16>(nl, k, variant), 1, 16, u};
-            case 32: return {pick_nl<double, 1, 32>(nl, k, variant), 1, 32, u};
-            default: break;
+        const double2 b = at(1, (m - 1) / 2 + 1), b0 = at(1, 1);
+        v = make_double2(2.0 * b.x - b0.x, 2.0 * b.y - b0.y);
+    }
+    mu[t] = v;
+}
+
+// Plain copy with kCopyUnroll independent 16-byte loads in flight per lane
+// before any store (a lane waiting on one load at a time cannot cover the
+// HBM latency).  Each work-group copies one contiguous tile per iteration.
+constexpr int kCopyUnroll = 4;
+
+__global__ __launch_bounds__(kBlock) void copy_kernel(const int4 *__restrict__ src,
+                                                       int4 *__restrict__ dst, long n16) {
+    const long tile = (long)kBlock * kCopyUnroll;
+    const long ntiles = (n16 + tile - 1) / tile;
+    for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const long base = t * tile + threadIdx.x;
+        int4 v[kCopyUnroll];
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; ++u) {
+            const long i = base + (long)u * kBlock;
+            if (i < n16) v[u] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; ++u) {
+            const long i = base + (long)u * kBlock;
+            if (i < n16) dst[i] = v[u];
         }
     }
-    return {nullptr, 0, 0, 1};
 }
 
 }  // namespace
@@ -420,10 +178,12 @@ int step_chunks(const Ell &H, int, const DeviceInfo &dev) {
 
 int launch_step(const StepArgs &a, hipStream_t s) {
     const Ell &H = *a.H;
-    const int nl = a.vl1 ? 2 : 1;
+    const int nl = (a.vl1 || a.doubling) ? 2 : 1;
     const size_t cb = cplx_bytes(H.vtype);
     const bool wide = (a.ld % 2 == 0) && (a.r0 % 2 == 0);
-    StepEntry e = pick_step(H.vtype, a.rb, wide, nl, H.k, H.variant);
+    const step::StepEntry e =
+        H.vtype == JKPM_C64 ? step::pick_step_c64(a.rb, wide, nl, H.k, H.variant, a.doubling)
+                            : step::pick_step_c128(a.rb, nl, H.k, H.variant, a.doubling);
     if (!e.fn) return arg_error("unsupported vector group size " + std::to_string(a.rb));
     const int rpb = kWaves * (64 / e.lpr) * e.unr;
     long rpc = ((long)H.n + a.nchunks - 1) / a.nchunks;
@@ -437,7 +197,8 @@ int launch_step(const StepArgs &a, hipStream_t s) {
     const void *p_xcur = (const char *)a.xcur + off;
     const void *p_xprev = a.xprev ? (const char *)a.xprev + off : nullptr;
     void *p_xout = (char *)a.xout + off;
-    const void *p_vl0 = (const char *)a.vl0 + off;
+    // (the doubling form reads no left vectors; its pointers stay unused)
+    const void *p_vl0 = a.vl0 ? (const char *)a.vl0 + off : nullptr;
     const void *p_vl1 = a.vl1 ? (const char *)a.vl1 + off : nullptr;
     int p_ld = a.ld;
     double2 *p_part = a.part + a.r0;
@@ -493,14 +254,24 @@ int launch_aaxby(const Ell &H, double2 a, double2 b, const void *x, void *y, int
     return JKPM_OK;
 }
 
+int launch_doubling_finish(const double2 *raw, int G, int S1, int R, int M, double2 *mu,
+                           hipStream_t s) {
+    const long tot = (long)G * M * R;
+    if (tot == 0) return JKPM_OK;
+    hipLaunchKernelGGL(doubling_finish_kernel, dim3((unsigned)((tot + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, raw, G, S1, R, M, mu);
+    JKPM_CHECK_LAUNCH("doubling_finish_kernel");
+    return JKPM_OK;
+}
+
 int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
     const long n16 = (long)(bytes / 16);
     if (n16 == 0) return JKPM_OK;
     DeviceInfo dev;
     int rc = device_info(&dev);
     if (rc) return rc;
-    long grid = (n16 + kBlock - 1) / kBlock;
-    const long cap = (long)dev.cu_count * 16;
+    long grid = (n16 + (long)kBlock * kCopyUnroll - 1) / ((long)kBlock * kCopyUnroll);
+    const long cap = (long)dev.cu_count * 8;
     if (grid > cap) grid = cap;
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)grid), dim3(kBlock), 0, s,
                        (const int4 *)src, (int4 *)dst, n16);

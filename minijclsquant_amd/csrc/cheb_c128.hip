@@ -1,0 +1,83 @@
+// complex128 instantiations of the Chebyshev step kernel and their dispatch.
+#include "cheb_step.h"
+
+namespace jkpm {
+namespace step {
+namespace {
+
+using Real = double;
+
+template <int VPL, int LPR, int NL, int KT, int UNR, bool NT>
+const void *ptr() {
+    return reinterpret_cast<const void *>(&cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT>);
+}
+
+template <int VPL, int LPR, int KT, bool NT>
+const void *dbl_ptr() {
+    return reinterpret_cast<const void *>(&cheb_step_kernel<Real, VPL, LPR, 2, KT, 1, NT, true>);
+}
+
+template <int VPL, int LPR>
+StepEntry dbl_entry(int k, bool nt) {
+    const void *f = k == 4 ? (nt ? dbl_ptr<VPL, LPR, 4, true>() : dbl_ptr<VPL, LPR, 4, false>())
+                  : k == 5 ? (nt ? dbl_ptr<VPL, LPR, 5, true>() : dbl_ptr<VPL, LPR, 5, false>())
+                           : dbl_ptr<VPL, LPR, 0, false>();
+    return {f, VPL, LPR, 1};
+}
+
+template <int VPL, int LPR, int KT>
+const void *tuned(int v) {
+    switch (v & 3) {
+        case 1: return ptr<VPL, LPR, 1, KT, 2, false>();
+        case 2: return ptr<VPL, LPR, 1, KT, 1, true>();
+        case 3: return ptr<VPL, LPR, 1, KT, 2, true>();
+        default: return ptr<VPL, LPR, 1, KT, 1, false>();
+    }
+}
+
+template <int VPL, int LPR>
+StepEntry entry(int nl, int k, int v, bool dbl) {
+    if (dbl) return dbl_entry<VPL, LPR>(k, (v & 2) != 0);
+    if (nl == 2) {
+        const void *f = k == 4 ? ptr<VPL, LPR, 2, 4, 1, false>()
+                      : k == 5 ? ptr<VPL, LPR, 2, 5, 1, false>()
+                               : ptr<VPL, LPR, 2, 0, 1, false>();
+        return {f, VPL, LPR, 1};
+    }
+    if (k == 4) return {tuned<VPL, LPR, 4>(v), VPL, LPR, (v & 1) ? 2 : 1};
+    if (k == 5) return {tuned<VPL, LPR, 5>(v), VPL, LPR, (v & 1) ? 2 : 1};
+    return {ptr<VPL, LPR, 1, 0, 1, false>(), VPL, LPR, 1};
+}
+
+}  // namespace
+
+StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl) {
+    // 16-byte lanes and up: VPL = 1 << shape (capped by the group size)
+    int shape = (variant >> 2) & 3;
+    while (shape > 0 && (1 << shape) > rb) --shape;
+    switch (rb) {
+        case 1: return entry<1, 1>(nl, k, variant, dbl);
+        case 2: return shape == 0 ? entry<1, 2>(nl, k, variant, dbl) : entry<2, 1>(nl, k, variant, dbl);
+        case 4:
+            return shape == 0   ? entry<1, 4>(nl, k, variant, dbl)
+                   : shape == 1 ? entry<2, 2>(nl, k, variant, dbl)
+                                : entry<4, 1>(nl, k, variant, dbl);
+        case 8:
+            return shape == 0   ? entry<1, 8>(nl, k, variant, dbl)
+                   : shape == 1 ? entry<2, 4>(nl, k, variant, dbl)
+                                : entry<4, 2>(nl, k, variant, dbl);
+        case 16:
+            return shape == 0   ? entry<1, 16>(nl, k, variant, dbl)
+                   : shape == 1 ? entry<2, 8>(nl, k, variant, dbl)
+                                : entry<4, 4>(nl, k, variant, dbl);
+        case 32:
+            return shape == 0   ? entry<1, 32>(nl, k, variant, dbl)
+                   : shape == 1 ? entry<2, 16>(nl, k, variant, dbl)
+                                : entry<4, 8>(nl, k, variant, dbl);
+        default: break;
+    }
+    return {nullptr, 0, 0, 1};
+}
+
+}  // namespace step
+}  // namespace jkpm

@@ -1,0 +1,259 @@
+// Chebyshev step kernel template, shared by the per-precision instantiation
+// units cheb_c64.hip and cheb_c128.hip (see cheb.hip for the design notes).
+#pragma once
+
+#include "common.h"
+
+namespace jkpm {
+namespace step {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kXcd = 8;
+
+template <typename Real, int N>
+using vec_t = Real __attribute__((ext_vector_type(N)));
+
+// Loads of data read once per step (T_{n-1}, v_l): non-temporal when NT, so
+// they do not displace the gather window of T_n from the caches.
+template <bool NT, typename V>
+__device__ __forceinline__ V load_stream(const V *p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+// UNR: rows per lane per loop iteration (all their loads are issued before
+// any is used, for more bytes in flight per wave).
+// DBL (doubling form, NL = 2, Hermitian A and v_l = v_r = T_0): instead of
+// dots with stored left vectors, the two partial sets are <T_{n+1}|T_{n+1}>
+// and <T_n|T_{n+1}> (T_n of the row itself, a cached re-read of the gather
+// source); mu_{2n} = 2<T_n|T_n> - mu_0 and mu_{2n+1} = 2<T_n|T_{n+1}> - mu_1
+// then give two moments per step with no v_l stream.  The first step's
+// step-0 partials are <T_0|T_0>.
+template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool DBL = false>
+__global__ __launch_bounds__(kBlock) void cheb_step_kernel(
+    const Real *__restrict__ val, const int *__restrict__ col, int n, int k_rt,
+    const Real *__restrict__ xcur, const Real *xprev, Real *xout,
+    const Real *__restrict__ vl0, const Real *__restrict__ vl1, int ld,
+    double2 *__restrict__ part, double2 *__restrict__ part0, int first,
+    int rows_per_chunk, int nchunks, const long *__restrict__ chunk_rows) {
+    using V = vec_t<Real, 2 * VPL>;
+    using C2 = vec_t<Real, 2>;
+    constexpr int RPW = 64 / LPR;
+    constexpr int RB = VPL * LPR;
+    const int K = KT > 0 ? KT : k_rt;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int lv = lane % LPR;
+    const int lr = lane / LPR;
+
+    const int b = blockIdx.x;
+    const int chunk = (nchunks % kXcd == 0) ? (b % kXcd) * (nchunks / kXcd) + b / kXcd : b;
+    // Uniform chunks, or an explicit row table (chunks that never straddle
+    // two realizations of a block-diagonal batch).
+    const long row_begin = chunk_rows ? chunk_rows[chunk] : (long)chunk * rows_per_chunk;
+    const long row_end =
+        chunk_rows ? chunk_rows[chunk + 1] : min((long)n, row_begin + rows_per_chunk);
+
+    double ar[NL][VPL], ai[NL][VPL];
+    double a0r[NL][VPL], a0i[NL][VPL];
+#pragma unroll
+    for (int l = 0; l < NL; ++l)
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) ar[l][v] = ai[l][v] = a0r[l][v] = a0i[l][v] = 0.0;
+
+    const int voff = lv * VPL * 2;
+    const Real *vls[2] = {vl0, vl1};
+
+    constexpr int STRIDE = kWaves * RPW;  // rows between a lane's consecutive rows
+    for (long base = row_begin + wave * RPW + lr; base < row_end; base += (long)STRIDE * UNR) {
+        // Rows past the chunk end are clamped to `base` (a valid row) for the
+        // loads and their results are dropped.
+        long rows[UNR];
+        bool ok[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const long rw = base + (long)u * STRIDE;
+            ok[u] = rw < row_end;
+            rows[u] = ok[u] ? rw : base;
+        }
+        Real sr[UNR][VPL], si[UNR][VPL];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) sr[u][v] = si[u][v] = Real(0);
+
+        if constexpr (KT > 0) {
+            int j[UNR][KT];
+            C2 h[UNR][KT];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int s = 0; s < KT; ++s) {
+                    const long e = (long)s * n + rows[u];
+                    j[u][s] = col[e];
+                    h[u][s] = *reinterpret_cast<const C2 *>(val + 2 * e);
+                }
+            V x[UNR][KT];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int s = 0; s < KT; ++s)
+                    x[u][s] = *reinterpret_cast<const V *>(xcur + (long)j[u][s] * ld * 2 + voff);
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int s = 0; s < KT; ++s)
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) {
+                        sr[u][v] += h[u][s].x * x[u][s][2 * v] - h[u][s].y * x[u][s][2 * v + 1];
+                        si[u][v] += h[u][s].x * x[u][s][2 * v + 1] + h[u][s].y * x[u][s][2 * v];
+                    }
+        } else {
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+                for (int s = 0; s < K; ++s) {
+                    const long e = (long)s * n + rows[u];
+                    const int j = col[e];
+                    const C2 h = *reinterpret_cast<const C2 *>(val + 2 * e);
+                    const V x = *reinterpret_cast<const V *>(xcur + (long)j * ld * 2 + voff);
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) {
+                        sr[u][v] += h.x * x[2 * v] - h.y * x[2 * v + 1];
+                        si[u][v] += h.x * x[2 * v + 1] + h.y * x[2 * v];
+                    }
+                }
+        }
+
+        // T_{n-1} and v_l of every row first, then the arithmetic.
+        V p[UNR], q[UNR][NL], t0[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const long vo = rows[u] * ld * 2 + voff;
+            if (first)
+                t0[u] = *reinterpret_cast<const V *>(xcur + vo);
+            else
+                p[u] = load_stream<NT>(reinterpret_cast<const V *>(xprev + vo));
+            if constexpr (DBL) {
+                if (!first) q[u][1] = *reinterpret_cast<const V *>(xcur + vo);  // T_n, cached
+            } else {
+#pragma unroll
+                for (int l = 0; l < NL; ++l)
+                    q[u][l] = load_stream<NT>(reinterpret_cast<const V *>(vls[l] + vo));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if (!ok[u]) continue;
+            const long vo = rows[u] * ld * 2 + voff;
+            V y;
+            if constexpr (DBL) {
+                if (first) {
+                    q[u][0] = t0[u];
+                    q[u][1] = t0[u];
+                }
+            }
+            if (first) {
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    y[2 * v] = sr[u][v];
+                    y[2 * v + 1] = si[u][v];
+                }
+#pragma unroll
+                for (int l = 0; l < NL; ++l)
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) {
+                        const double qr = q[u][l][2 * v], qi = q[u][l][2 * v + 1];
+                        const double tr = t0[u][2 * v], ti = t0[u][2 * v + 1];
+                        a0r[l][v] += qr * tr + qi * ti;
+                        a0i[l][v] += qr * ti - qi * tr;
+                    }
+            } else {
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    y[2 * v] = Real(2) * sr[u][v] - p[u][2 * v];
+                    y[2 * v + 1] = Real(2) * si[u][v] - p[u][2 * v + 1];
+                }
+            }
+            *reinterpret_cast<V *>(xout + vo) = y;
+            if constexpr (DBL) q[u][0] = y;
+#pragma unroll
+            for (int l = 0; l < NL; ++l)
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    const double qr = q[u][l][2 * v], qi = q[u][l][2 * v + 1];
+                    const double yr = y[2 * v], yi = y[2 * v + 1];
+                    ar[l][v] += qr * yr + qi * yi;
+                    ai[l][v] += qr * yi - qi * yr;
+                }
+        }
+    }
+
+    // Butterfly over the lanes that hold the same vector index.
+#pragma unroll
+    for (int o = LPR; o < 64; o <<= 1) {
+#pragma unroll
+        for (int l = 0; l < NL; ++l)
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                ar[l][v] += __shfl_xor(ar[l][v], o);
+                ai[l][v] += __shfl_xor(ai[l][v], o);
+                if (first) {
+                    a0r[l][v] += __shfl_xor(a0r[l][v], o);
+                    a0i[l][v] += __shfl_xor(a0i[l][v], o);
+                }
+            }
+    }
+
+    __shared__ double2 sh[2][kWaves][NL][RB];
+    if (lr == 0) {
+#pragma unroll
+        for (int l = 0; l < NL; ++l)
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                sh[0][wave][l][lv * VPL + v] = make_double2(ar[l][v], ai[l][v]);
+                sh[1][wave][l][lv * VPL + v] = make_double2(a0r[l][v], a0i[l][v]);
+            }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NL * RB; t += kBlock) {
+        const int l = t / RB, r = t % RB;
+        double sr2 = 0.0, si2 = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            sr2 += sh[0][w][l][r].x;
+            si2 += sh[0][w][l][r].y;
+        }
+        part[((long)l * nchunks + chunk) * ld + r] = make_double2(sr2, si2);
+        if (first) {
+            double s0r = 0.0, s0i = 0.0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                s0r += sh[1][w][l][r].x;
+                s0i += sh[1][w][l][r].y;
+            }
+            part0[((long)l * nchunks + chunk) * ld + r] = make_double2(s0r, s0i);
+        }
+    }
+}
+
+// One work-group per (vector, left set x group, step): fixed-order sum of the
+
+struct StepEntry {
+    const void *fn;
+    int vpl, lpr, unr;
+};
+
+// variant v = unroll | non-temporal << 1 | lane-shape << 2 (0..11); the
+// lane shape multiplies the vectors per lane (VPL) by 1, 2 or 4.  Variants
+// apply to single-left-set kernels of ELL width 4 and 5; every other shape
+// uses the base kernel.  Returns fn = nullptr for an unsupported group size.
+// dbl selects the doubling form (two partial sets, see DBL above).
+StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl = false);
+StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl = false);
+
+}  // namespace step
+}  // namespace jkpm

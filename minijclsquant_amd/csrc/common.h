@@ -80,7 +80,14 @@ struct StepArgs {
     double2 *part0;       // when first: partials of step 0 (same layout)
     int nchunks;
     const long *chunk_rows = nullptr;  // device [nchunks+1] row starts, or uniform chunks
+    bool doubling = false;  // partials <T_{n+1}|T_{n+1}>, <T_n|T_{n+1}> (no left vectors)
 };
+
+// mu[g][m][r] from the doubling form's raw dots raw[l][g][s][r] (s < S1):
+// mu_0 = a_0, mu_1 = b_0, mu_2k = 2 a_k - a_0, mu_2k+1 = 2 b_k - b_0 with
+// a_k at (l=0, s=k) and b_k at (l=1, s=k+1).
+int launch_doubling_finish(const double2 *raw, int G, int S1, int R, int M, double2 *mu,
+                           hipStream_t s);
 
 // Number of row chunks (work-groups) used for a matrix of n rows.
 int step_chunks(const Ell &H, int rb, const DeviceInfo &dev);
@@ -122,6 +129,14 @@ int launch_grad_out(const double2 *G, int n, int k, double scale, double2 *out, 
 int launch_zero(double2 *p, long cnt, hipStream_t s);
 int launch_dos_grad(const double *w, int M, int R, const double *e, int n_e, int kernel,
                     double2 *grad_mu, hipStream_t s);
+
+// ---- Chebyshev series / commutator recursion (series.hip) ----
+size_t series_workspace(const Ell &H, int R, bool comm);
+// out (if set) = sum_m c_m X_m and tab[m] (if set, commutator only) = c_m X_m,
+// X = T_m(H) v, or the commutator recursion C_m when V is set; c: host
+// complex128 [M] as (re, im) pairs.
+int run_series(const Ell &H, const Ell *V, const void *v, int R, int M, const double *c,
+               void *out, void *tab, void *ws, int nchunks, hipStream_t s);
 
 // ---- Kubo double moments (kubo.hip) ----
 int kubo_gemm_slices(long K, int M, const DeviceInfo &dev);

@@ -9,6 +9,7 @@ Keyword-only extensions (defaults reproduce the reference):
   energies  explicit energy grid instead of linspace(b0+1e-3, b1-1e-3, 2M+1)
   dtype     'c128' (default) or 'c64' recursion (moments stay FP64)
   kernel    'jackson' (default) or None
+  doubling  two moments per Chebyshev step (Hermitian H only, kpm_dos)
 A random_vector may be an [R, N] stack; the DOS is then that of the mean
 moments over the R vectors.
 """
@@ -45,10 +46,10 @@ def _rescaled(H):
     return H, s
 
 
-def kpm_moments(H, M, random_vector, *, dtype="c128", bounds=True):
+def kpm_moments(H, M, random_vector, *, dtype="c128", bounds=True, doubling=False):
     """Moments mu_m = <r|T_m(H~)|r> as a device tensor complex128 [R, M]."""
     Hk, s = _rescaled(H) if bounds else (H, 1.0)
-    return _kpm.moments(Hk, int(M), random_vector, dtype=dtype, scale=s)[0]
+    return _kpm.moments(Hk, int(M), random_vector, dtype=dtype, scale=s, doubling=doubling)[0]
 
 
 def _dos_table(H, Ef_vec, mu_rm, kernel):
@@ -60,14 +61,15 @@ def _dos_table(H, Ef_vec, mu_rm, kernel):
     return delta_mat
 
 
-def kpm_dos(H, M=None, random_vector=None, *, energies=None, dtype="c128", kernel="jackson"):
+def kpm_dos(H, M=None, random_vector=None, *, energies=None, dtype="c128", kernel="jackson",
+            doubling=False):
     """DOS [E, 2] = [energies, unnormalised DOS] (jclsquant/observables.py:26-54)."""
     if M is None:
         M = int(np.sqrt(H.shape[0]))
     Ef_vec = _energy_grid(H, M, energies)
     if random_vector is None:
         random_vector = random_vector_generator(H.shape[0])
-    mu = kpm_moments(H, M, random_vector, dtype=dtype)
+    mu = kpm_moments(H, M, random_vector, dtype=dtype, doubling=doubling)
     return _dos_table(H, Ef_vec, mu, kernel)
 
 

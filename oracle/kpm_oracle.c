@@ -331,6 +331,87 @@ int orc_grad_values(int M, const zc *data, const int *indices, int len_row, int 
     return 0;
 }
 
+/* jclsquant/recurrence_functions.py:114-143 (rec_A_vec):
+ * out = sum_{m<M} c_m T_m(A) v, with the reference's update order
+ * (A_kpm = c_0 v + c_1 T_1, then axpy2(c_m, 1, T_m, A_kpm)). */
+int orc_rec_A_vec(int M, const zc *data, const int *indices, int len_row, int n,
+                  const double *c_ri, const zc *v, zc *out, int n_threads) {
+    const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    const double two[2] = {2.0, 0.0}, mone[2] = {-1.0, 0.0};
+    zc *t0 = (zc *)malloc(sizeof(zc) * (size_t)n), *t1 = (zc *)calloc((size_t)n, sizeof(zc)), *tmp;
+    int m;
+    long i;
+    if (!t0 || !t1) { free(t0); free(t1); return -1; }
+    memcpy(t0, v, sizeof(zc) * (size_t)n);
+    orc_aAxby(one, zero, v, t1, data, indices, len_row, n, n_threads);
+    {
+        zc c0 = c_ri[0] + I * c_ri[1], c1 = M > 1 ? c_ri[2] + I * c_ri[3] : 0.0;
+        for (i = 0; i < n; ++i) out[i] = c0 * v[i] + (M > 1 ? c1 * t1[i] : 0.0);
+    }
+    for (m = 2; m < M; ++m) {
+        zc cm = c_ri[2 * m] + I * c_ri[2 * m + 1];
+        orc_aAxby(two, mone, t1, t0, data, indices, len_row, n, n_threads);
+        tmp = t0; t0 = t1; t1 = tmp;
+#pragma omp parallel for schedule(static) num_threads(nthr(n_threads))
+        for (i = 0; i < n; ++i) out[i] = cm * t1[i] + 1.0 * out[i];
+    }
+    free(t0);
+    free(t1);
+    return 0;
+}
+
+/* jclsquant/recurrence_functions.py:146-196 (rec_com_A_vec) and :202-250
+ * (rec_com_A_vec_tab): commutator recursion C_0 = 0, C_1 = V v,
+ * C_m = 2 A C_{m-1} - C_{m-2} + 2 V T_{m-1}, next to T_m(A) v.  With tab
+ * (M*n) non-NULL, tab[m] = c_m C_m; with out non-NULL, out = sum c_m C_m. */
+int orc_rec_com_A_vec(int M, const zc *a_data, const zc *v_data, const int *indices,
+                      int len_row, int n, const double *c_ri, const zc *v, zc *out, zc *tab,
+                      int n_threads) {
+    const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    const double two[2] = {2.0, 0.0}, mone[2] = {-1.0, 0.0};
+    zc *t0 = (zc *)malloc(sizeof(zc) * (size_t)n), *t1 = (zc *)calloc((size_t)n, sizeof(zc));
+    zc *k0 = (zc *)calloc((size_t)n, sizeof(zc)), *k1 = (zc *)calloc((size_t)n, sizeof(zc));
+    zc *tmp;
+    int m;
+    long i;
+    if (!t0 || !t1 || !k0 || !k1) { free(t0); free(t1); free(k0); free(k1); return -1; }
+    memcpy(t0, v, sizeof(zc) * (size_t)n);
+    orc_aAxby(one, zero, v, t1, a_data, indices, len_row, n, n_threads);
+    orc_aAxby(one, zero, v, k1, v_data, indices, len_row, n, n_threads);
+    if (out) {
+        zc c1 = M > 1 ? c_ri[2] + I * c_ri[3] : 0.0;
+        for (i = 0; i < n; ++i) out[i] = M > 1 ? c1 * k1[i] : 0.0;
+    }
+    if (tab) {
+        zc c0 = c_ri[0] + I * c_ri[1];
+        for (i = 0; i < n; ++i) tab[i] = c0 * k0[i];
+        if (M > 1) {
+            zc c1 = c_ri[2] + I * c_ri[3];
+            for (i = 0; i < n; ++i) tab[(size_t)n + i] = c1 * k1[i];
+        }
+    }
+    for (m = 2; m < M; ++m) {
+        zc cm = c_ri[2 * m] + I * c_ri[2 * m + 1];
+        orc_aAxby(two, mone, k1, k0, a_data, indices, len_row, n, n_threads);
+        {
+            const double pone[2] = {1.0, 0.0};
+            orc_aAxby(two, pone, t1, k0, v_data, indices, len_row, n, n_threads);
+        }
+        tmp = k0; k0 = k1; k1 = tmp;
+        orc_aAxby(two, mone, t1, t0, a_data, indices, len_row, n, n_threads);
+        tmp = t0; t0 = t1; t1 = tmp;
+        if (out)
+            for (i = 0; i < n; ++i) out[i] = cm * k1[i] + 1.0 * out[i];
+        if (tab)
+            for (i = 0; i < n; ++i) tab[(size_t)m * n + i] = cm * k1[i];
+    }
+    free(t0);
+    free(t1);
+    free(k0);
+    free(k1);
+    return 0;
+}
+
 /* jclsquant/cython_modules/Extra_funs.pyx:430-535 (moments_Gmin_real_c,
  * moments_Gmas_real_c, moments_delta_c, table_G_delta_c) for an explicit list
  * of rescaled energies (the reference uses linspace(-alpha, alpha, 2M)):

@@ -54,6 +54,8 @@ def lib():
             "orc_mu_nm": (i, [i, _vp, _vp, i, _vp, _vp, i, i, _vp, _vp, i]),
             "orc_grad_values": (i, [i, _vp, _vp, i, i, _vp, _vp, _vp, i]),
             "orc_table_G_delta": (None, [i, _vp, i, d, _vp, _vp]),
+            "orc_rec_A_vec": (i, [i, _vp, _vp, i, i, _vp, _vp, _vp, i]),
+            "orc_rec_com_A_vec": (i, [i, _vp, _vp, _vp, i, i, _vp, _vp, _vp, _vp, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -203,6 +205,34 @@ def mu_nm(M, h_data, h_idx, h_k, v_data, v_idx, v_k, r, n_threads=None):
                          int(v_k), r.size, _p(r), _p(out), threads(n_threads))
     assert rc == 0
     return out
+
+
+def rec_A_vec(M, data, indices, len_row, c, v, n_threads=None):
+    """sum_{m<M} c_m T_m(A) v (complex128)."""
+    data, indices, v = _z(data), _i(indices), _z(v)
+    cc = _z(np.asarray(c, dtype=np.complex128))
+    assert cc.size >= M
+    out = np.zeros(v.size, np.complex128)
+    rc = lib().orc_rec_A_vec(int(M), _p(data), _p(indices), int(len_row), v.size, _p(cc), _p(v),
+                             _p(out), threads(n_threads))
+    assert rc == 0
+    return out
+
+
+def rec_com_A_vec(M, a_data, v_data, indices, len_row, c, v, table=False, n_threads=None):
+    """sum_m c_m C_m (or the table [M, N] of c_m C_m) of the commutator
+    recursion C_m = 2 A C_{m-1} - C_{m-2} + 2 V T_{m-1}, C_0 = 0, C_1 = V v."""
+    a_data, v_data, indices, v = _z(a_data), _z(v_data), _i(indices), _z(v)
+    cc = _z(np.asarray(c, dtype=np.complex128))
+    assert cc.size >= M
+    n = v.size
+    out = None if table else np.zeros(n, np.complex128)
+    tab = np.zeros((M, n), np.complex128) if table else None
+    rc = lib().orc_rec_com_A_vec(int(M), _p(a_data), _p(v_data), _p(indices), int(len_row), n,
+                                 _p(cc), _p(v), None if out is None else _p(out),
+                                 None if tab is None else _p(tab), threads(n_threads))
+    assert rc == 0
+    return tab if table else out
 
 
 def table_G_delta(M, E, bound):

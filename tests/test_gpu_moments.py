@@ -261,19 +261,26 @@ def test_launch_variants_and_autotune(torch_gpu, dtype, tmp_path, monkeypatch):
     h = nat.DeviceELL(Ht.data, Ht.indices, A.shape[0], A.len_row, 1.0, dtype)
     vr = _kpm.as_batch(r, A.shape[0], dtype)
     ref = h.moments(vr, 200)[0].cpu().numpy()
-    for v in range(4):
-        for m in (1, 4, 8, 32):
+    for v in range(12):
+        for m in (1, 4, 32):
             h.set_launch(v, m)
             got = h.moments(vr, 200)[0].cpu().numpy()
             assert np.max(np.abs(got - ref)) / abs(ref[0, 0]) <= 1e-12, (v, m)
+    # odd batch (narrow lanes and power-of-two groups) under every variant
+    r5 = _kpm.as_batch(r[:5], A.shape[0], dtype)
+    ref5 = h.moments(r5, 90)[0].cpu().numpy()
+    for v in range(12):
+        h.set_launch(v, 8)
+        got = h.moments(r5, 90)[0].cpu().numpy()
+        assert np.max(np.abs(got - ref5)) / abs(ref5[0, 0]) <= 1e-12, v
     monkeypatch.setenv("JKPM_TUNE_TABLE", str(tmp_path / "t.json"))
     v, m = autotune.tuned(h, vr, steps=4)
-    assert 0 <= v <= 3 and m in (4, 8, 16, 32)
+    assert 0 <= v <= 11 and m in (2, 4, 8, 16, 32)
     assert (tmp_path / "t.json").exists()
     h.set_launch(0, 8)
     assert autotune.tuned(h, vr) == (v, m)  # from the table
     with pytest.raises(ValueError):
-        h.set_launch(4, 8)
+        h.set_launch(12, 8)
 
 
 def test_precision_form_full_size(torch_gpu):

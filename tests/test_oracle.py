@@ -275,6 +275,59 @@ def test_table_G_delta_vs_chebyshev_u():
     assert np.max(np.abs(s_table - np.array(s_rank))) <= 1e-11 * np.max(np.abs(s_rank))
 
 
+def test_rec_A_vec_vs_eigh_chebval():
+    """sum_m c_m T_m(H) v = U f(L) U^H v with f = numpy chebval(c)."""
+    Ht = herm_lattice(7, W=1.0, seed=5)
+    d, i, _, K = ell_of(Ht)
+    N = Ht.shape[0]
+    rng = np.random.default_rng(1)
+    v = rng.normal(size=N) + 1j * rng.normal(size=N)
+    M = 60
+    c = (rng.normal(size=M) + 1j * rng.normal(size=M)) / (1 + np.arange(M))
+    lam, U = np.linalg.eigh(Ht.toarray())
+    want = U @ (C.chebval(lam, c) * (U.conj().T @ v))
+    got = O.rec_A_vec(M, d, i, K, c, v)
+    assert np.max(np.abs(got - want)) <= 1e-11 * np.max(np.abs(want))
+
+
+def _frechet_cheb(lam, m):
+    """Divided differences of T_m at the eigenvalues (Daleckii-Krein kernel)."""
+    e_m = np.eye(m + 1)[m]  # Chebyshev coefficients of T_m
+    T = C.chebval(lam, e_m)
+    dT = C.chebval(lam, C.chebder(e_m)) if m > 0 else np.zeros_like(lam)
+    diff = lam[:, None] - lam[None, :]
+    same = np.abs(diff) < 1e-9
+    G = np.where(same, 0.0, (T[:, None] - T[None, :]) / np.where(same, 1.0, diff))
+    G[same] = np.broadcast_to(dT[:, None], G.shape)[same]
+    return G
+
+
+def test_rec_com_A_vec_vs_daleckii_krein():
+    """C_m = 2 H C_{m-1} - C_{m-2} + 2 V T_{m-1} (C_0 = 0, C_1 = V v) is the
+    derivative of T_m(H) v along V: U (Gamma_m o (U^H V U)) U^H v with Gamma_m
+    the divided differences of T_m at the eigenvalues (Daleckii-Krein)."""
+    Ht, V = _velocity_pair(3, W=1.0, seed=2)
+    hd, hi, _, hk = ell_of(Ht)
+    # V on H's slot pattern (velocity has zero diagonal slots; keep them)
+    vd = np.zeros_like(hd)
+    rows = np.arange(hd.size) // hk
+    vd[:] = np.asarray(V.toarray())[rows, hi]
+    N = Ht.shape[0]
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=N) + 1j * rng.normal(size=N)
+    M = 24
+    c = rng.normal(size=M) + 1j * rng.normal(size=M)
+    lam, U = np.linalg.eigh(Ht.toarray())
+    Vt = U.conj().T @ V.toarray() @ U
+    w = U.conj().T @ v
+    tab = O.rec_com_A_vec(M, hd, vd, hi, hk, c, v, table=True)
+    for m in range(M):
+        want = c[m] * (U @ ((_frechet_cheb(lam, m) * Vt) @ w))
+        assert np.max(np.abs(tab[m] - want)) <= 1e-9 * max(1.0, np.max(np.abs(want))), m
+    s = O.rec_com_A_vec(M, hd, vd, hi, hk, c, v)
+    assert np.max(np.abs(s - tab.sum(axis=0))) <= 1e-12 * np.max(np.abs(s))
+
+
 # ---- planted mistakes: each pin above must reject a plausible wrong oracle ----
 
 def test_pins_reject_planted_mistakes():
