@@ -19,7 +19,12 @@ N*M*R units per second.  The JSON line also carries
                 vector at a time) on a bounded sample of the same workload;
   dos_err_vs_c128  max |DOS_c64 - DOS_c128| / max |DOS_c128| on the same
                 inputs (the complex128 GPU path is pinned to the oracle by the
-                tests).
+                tests);
+  doubling      the same moments from M/2 steps of the doubling form
+                (Hermitian H), with its own roofline and DOS error;
+  plain_c128    the reference's structure on the GPU (complex128, one vector
+                per recursion, one moment per step) and the headline's and
+                doubling form's ratios over it.
 """
 from __future__ import annotations
 
@@ -113,6 +118,34 @@ def cpu_baseline(A, r, target_s=15.0):
     }
 
 
+def plain_c128(torch, nat, _kpm, A, r, B, M, value, value_dbl, reps=3):
+    """The reference's structure on this GPU: complex128 recursion, one
+    vector at a time (R = 1), one moment per step, default launch shape.
+    Timed with HIP events on the current stream over `reps` recursions."""
+    N = int(A.shape[0])
+    h = A.device("c128", 1.0 / B)
+    v1 = _kpm.as_batch(r[0], N, "c128")
+    ws = torch.empty(h.workspace_bytes(1, 1), dtype=torch.uint8, device="cuda")
+    mu = torch.empty((1, M, 1), dtype=torch.complex128, device="cuda")
+    h.moments(v1, M, workspace=ws, out=mu)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        h.moments(v1, M, workspace=ws, out=mu)
+    t1.record()
+    t1.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    v = N * M / (ms / 1e3)
+    return {
+        "value": v,
+        "unit": "N*M*R/s",
+        "ms_per_recursion": ms,
+        "sample": f"same H, R=1 vector, M={M} moments, complex128, {reps} recursions",
+        "headline_over_plain": value / v,
+        "doubling_over_plain": None if value_dbl is None else value_dbl / v,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +160,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dos-check", action="store_true")
     ap.add_argument("--no-doubling", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain complex128 one-vector GPU recursion")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shape instead of the tuned one")
     ap.add_argument("--probe", action="store_true",
@@ -182,6 +217,9 @@ def main():
 
         v, m = autotune.tuned(h, vr)
         log(f"launch shape: variant {v}, {m} work-groups per CU")
+        if not args.no_doubling:
+            v, m = autotune.tuned(h, vr, doubling=True)
+            log(f"doubling-form launch shape: variant {v}, {m} work-groups per CU")
         ws = torch.empty(h.workspace_bytes(R, 1), dtype=torch.uint8, device="cuda")
     ws_dbl = torch.empty(nat.moments_ex_workspace_bytes(h, None, R, M, True), dtype=torch.uint8,
                          device="cuda")
@@ -257,6 +295,13 @@ def main():
                                       / np.max(np.abs(d128)))),
         }
 
+    plain = None
+    if not args.no_plain:
+        plain = plain_c128(torch, nat, _kpm, A, r, B, M, units / (ms_per_step / 1e3),
+                           None if dbl is None else dbl["value"])
+        log(f"plain complex128 R=1 recursion: {plain['ms_per_recursion']:.1f} ms per "
+            f"{M}-moment recursion")
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle, complex128, bounded sample)")
@@ -286,6 +331,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "doubling": dbl,
+        "plain_c128": plain,
     }
     print(json.dumps(out), flush=True)
 

@@ -115,18 +115,23 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
  * jclsquant/cuda_cython/cuda/recursions.cu:19-20).
  * ------------------------------------------------------------------- */
 
-/* variant: 0..3 = (rows per lane, non-temporal loads of the read-once
- * streams) in {(1,no), (2,no), (1,yes), (2,yes)} (variants exist for
- * single-left-set kernels of ELL width 4 and 5; other shapes use 0);
- * chunk_mult: work-groups per CU (1..64, default 8). */
-int jkpm_ell_set_launch(jkpm_ell *h, int variant, int chunk_mult);
-int jkpm_ell_get_launch(const jkpm_ell *h, int *variant, int *chunk_mult);
+/* variant = unroll | nontemporal << 1 | shape << 2 (0..11): unroll = two
+ * rows per lane, nontemporal = non-temporal loads of the read-once streams
+ * (matrix values and indices, T_{n-1}), shape = complex64 vectors per lane
+ * 2 << shape (lanes per row shrink to match; c64 groups of 4+ vectors).
+ * Variants exist for ELL widths 4 and 5; other shapes use 0.  chunk_mult:
+ * work-groups per CU (1..64, default 8).  flags = JKPM_MOMENTS_DOUBLING
+ * addresses the doubling form's own setting; 0 the plain recursion's. */
+int jkpm_ell_set_launch(jkpm_ell *h, int variant, int chunk_mult, int flags);
+int jkpm_ell_get_launch(const jkpm_ell *h, int *variant, int *chunk_mult, int flags);
 
-/* Times `steps` fused steps of jkpm_moments on v_r (device [N][R]) for each
- * variant x chunk_mult in {4, 8, 16, 32}, keeps the fastest on the handle and
- * returns its mean step time in microseconds.  Synchronises `stream`. */
-int jkpm_autotune_workspace(const jkpm_ell *h, int R, int steps, size_t *bytes);
-int jkpm_autotune(jkpm_ell *h, const void *v_r, int R, int steps, void *workspace,
+/* Times `steps` fused steps of jkpm_moments (flags = 0) or of the doubling
+ * form of jkpm_moments_ex (flags = JKPM_MOMENTS_DOUBLING) on v_r (device
+ * [N][R]) for each variant x chunk_mult in {2, 4, 8, 16, 32}, keeps the
+ * fastest on the handle for that form and returns its mean step time in
+ * microseconds.  Synchronises `stream`. */
+int jkpm_autotune_workspace(const jkpm_ell *h, int R, int steps, int flags, size_t *bytes);
+int jkpm_autotune(jkpm_ell *h, const void *v_r, int R, int steps, int flags, void *workspace,
                   size_t workspace_bytes, void *stream, double *best_us);
 
 /* ---------------------------------------------------------------------

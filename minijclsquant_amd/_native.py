@@ -70,10 +70,10 @@ SIGNATURES = {
     "jkpm_graph_end": (_int, [_vp, _c.POINTER(_vp)]),
     "jkpm_graph_launch": (_int, [_vp, _vp]),
     "jkpm_graph_destroy": (_int, [_vp]),
-    "jkpm_ell_set_launch": (_int, [_vp, _int, _int]),
-    "jkpm_ell_get_launch": (_int, [_vp, _ip, _ip]),
-    "jkpm_autotune_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
-    "jkpm_autotune": (_int, [_vp, _vp, _int, _int, _vp, _size, _vp, _dp]),
+    "jkpm_ell_set_launch": (_int, [_vp, _int, _int, _int]),
+    "jkpm_ell_get_launch": (_int, [_vp, _ip, _ip, _int]),
+    "jkpm_autotune_workspace": (_int, [_vp, _int, _int, _int, _c.POINTER(_size)]),
+    "jkpm_autotune": (_int, [_vp, _vp, _int, _int, _int, _vp, _size, _vp, _dp]),
     "jkpm_moments_ex_workspace": (_int, [_vp, _vp, _int, _int, _int, _c.POINTER(_size)]),
     "jkpm_moments_ex": (_int, [_vp, _vp, _vp, _int, _int, _int, _vp, _vp, _size, _vp]),
     "jkpm_series_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
@@ -377,24 +377,29 @@ class DeviceELL:
                                  stream_ptr(stream)))
         return out
 
-    def set_launch(self, variant: int, chunk_mult: int) -> None:
-        check(lib().jkpm_ell_set_launch(self._h, int(variant), int(chunk_mult)))
+    def set_launch(self, variant: int, chunk_mult: int, doubling: bool = False) -> None:
+        """Step-kernel launch shape of the plain recursion (or of the
+        doubling form with doubling=True)."""
+        check(lib().jkpm_ell_set_launch(self._h, int(variant), int(chunk_mult),
+                                        JKPM_MOMENTS_DOUBLING if doubling else 0))
 
-    def get_launch(self):
+    def get_launch(self, doubling: bool = False):
         v, m = _c.c_int(), _c.c_int()
-        check(lib().jkpm_ell_get_launch(self._h, _c.byref(v), _c.byref(m)))
+        check(lib().jkpm_ell_get_launch(self._h, _c.byref(v), _c.byref(m),
+                                        JKPM_MOMENTS_DOUBLING if doubling else 0))
         return int(v.value), int(m.value)
 
-    def autotune(self, v_r, steps: int = 8, stream=None) -> float:
-        """Picks the fastest step-kernel launch shape for these vectors;
-        returns its mean step time (us)."""
+    def autotune(self, v_r, steps: int = 8, stream=None, doubling: bool = False) -> float:
+        """Picks the fastest step-kernel launch shape for these vectors (for
+        the doubling form with doubling=True); returns its mean step time (us)."""
         torch = _torch()
         R = self._check_vecs(v_r)
+        flags = JKPM_MOMENTS_DOUBLING if doubling else 0
         need = _size()
-        check(lib().jkpm_autotune_workspace(self._h, R, int(steps), _c.byref(need)))
+        check(lib().jkpm_autotune_workspace(self._h, R, int(steps), flags, _c.byref(need)))
         ws = torch.empty(int(need.value), dtype=torch.uint8, device=v_r.device)
         best = _c.c_double()
-        check(lib().jkpm_autotune(self._h, v_r.data_ptr(), R, int(steps), ws.data_ptr(),
+        check(lib().jkpm_autotune(self._h, v_r.data_ptr(), R, int(steps), flags, ws.data_ptr(),
                                   ws.numel(), stream_ptr(stream), _c.byref(best)))
         return float(best.value)
 

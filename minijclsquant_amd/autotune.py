@@ -1,10 +1,12 @@
 """Launch-shape autotuner with a persistent table.
 
-The step kernel has four variants (rows per lane x non-temporal streams) and
-a work-group count per CU; jkpm_autotune times them on the caller's matrix
-and vectors.  Results are cached in a JSON table keyed by device, matrix
-shape (N, K), value type and batch size R, so a shape is tuned once per
-machine.  ``JKPM_LAUNCH="variant,chunk_mult"`` overrides the table.
+The step kernel has twelve variants (rows per lane x non-temporal streams x
+vectors per lane) and a work-group count per CU, set separately for the
+plain recursion and for the doubling form; jkpm_autotune times them on the
+caller's matrix and vectors.  Results are cached in a JSON table keyed by
+device, matrix shape (N, K), value type, batch size R and form, so a shape
+is tuned once per machine.  ``JKPM_LAUNCH="variant,chunk_mult"`` overrides
+the table (for both forms).
 
 The reference has a single fixed launch (<<<ceil(N/1024), 1024>>>,
 jclsquant/cuda_cython/cuda/recursions.cu:19-20).
@@ -30,11 +32,12 @@ def table_path() -> Path:
     return Path(os.path.expanduser("~")) / ".cache" / "jclkpm" / "launch_table.json"
 
 
-def _key(h: "nat.DeviceELL", R: int) -> str:
+def _key(h: "nat.DeviceELL", R: int, doubling: bool) -> str:
     import torch
 
     dev = torch.cuda.get_device_name(torch.cuda.current_device())
-    return f"{dev}|N={h.n}|K={h.k}|vtype={h.vtype}|R={R}"
+    form = "|doubling" if doubling else ""
+    return f"{dev}|N={h.n}|K={h.k}|vtype={h.vtype}|R={R}{form}"
 
 
 def _load():
@@ -56,24 +59,25 @@ def _store(tab):
         pass  # a read-only home just means re-tuning next time
 
 
-def tuned(h: "nat.DeviceELL", v_r, steps: int = 8, retune: bool = False):
+def tuned(h: "nat.DeviceELL", v_r, steps: int = 8, retune: bool = False,
+          doubling: bool = False):
     """Applies the cached (or freshly tuned) launch shape to handle h for the
-    batch v_r [N, R]; returns (variant, chunk_mult)."""
+    batch v_r [N, R] and the chosen form; returns (variant, chunk_mult)."""
     env = os.environ.get("JKPM_LAUNCH")
     if env:
         v, m = (int(x) for x in env.split(","))
-        h.set_launch(v, m)
+        h.set_launch(v, m, doubling)
         return v, m
     R = int(v_r.shape[1])
-    key = _key(h, R)
+    key = _key(h, R, doubling)
     with _lock:
         tab = _load()
         if not retune and key in tab:
             v, m = tab[key]["variant"], tab[key]["chunk_mult"]
-            h.set_launch(v, m)
+            h.set_launch(v, m, doubling)
             return v, m
-        us = h.autotune(v_r, steps)
-        v, m = h.get_launch()
+        us = h.autotune(v_r, steps, doubling=doubling)
+        v, m = h.get_launch(doubling)
         tab[key] = {"variant": v, "chunk_mult": m, "step_us": us}
         _store(tab)
         return v, m

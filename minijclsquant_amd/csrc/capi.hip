@@ -107,7 +107,7 @@ MomentsLayout moments_layout(const Ell &H, int R, int n_left, const DeviceInfo &
     MomentsLayout L;
     const size_t vec = align_up((size_t)H.n * R * cplx_bytes(H.vtype), 256);
     if (doubling_M > 0) n_left = 2;
-    L.nchunks = grp ? grp->nchunks : step_chunks(H, R, dev);
+    L.nchunks = grp ? grp->nchunks : step_chunks(H, R, dev, doubling_M > 0);
     L.b0 = 0;
     L.b1 = vec;
     L.ring = 2 * vec;
@@ -483,76 +483,95 @@ namespace {
 constexpr int kVariants = 12;  // unroll x non-temporal x lane shape (cheb_step.h)
 }
 
-int jkpm_ell_set_launch(jkpm_ell *h, int variant, int chunk_mult) {
+int jkpm_ell_set_launch(jkpm_ell *h, int variant, int chunk_mult, int flags) {
     clear_error();
     if (!h) return arg_error("handle is NULL");
     if (variant < 0 || variant >= kVariants) return arg_error("variant must be 0..11");
     if (chunk_mult < 1 || chunk_mult > 64) return arg_error("chunk_mult must be 1..64");
-    h->e.variant = variant;
-    h->e.chunk_mult = chunk_mult;
+    if (flags & ~JKPM_MOMENTS_DOUBLING) return arg_error("unknown flags");
+    if (flags & JKPM_MOMENTS_DOUBLING) {
+        h->e.dvariant = variant;
+        h->e.dchunk_mult = chunk_mult;
+    } else {
+        h->e.variant = variant;
+        h->e.chunk_mult = chunk_mult;
+    }
     return JKPM_OK;
 }
 
-int jkpm_ell_get_launch(const jkpm_ell *h, int *variant, int *chunk_mult) {
+int jkpm_ell_get_launch(const jkpm_ell *h, int *variant, int *chunk_mult, int flags) {
     clear_error();
     if (!h) return arg_error("handle is NULL");
-    if (variant) *variant = h->e.variant;
-    if (chunk_mult) *chunk_mult = h->e.chunk_mult;
+    if (flags & ~JKPM_MOMENTS_DOUBLING) return arg_error("unknown flags");
+    const bool dbl = flags & JKPM_MOMENTS_DOUBLING;
+    if (variant) *variant = dbl ? h->e.dvariant : h->e.variant;
+    if (chunk_mult) *chunk_mult = dbl ? h->e.dchunk_mult : h->e.chunk_mult;
     return JKPM_OK;
 }
 
 namespace {
 const int kTuneMult[] = {2, 4, 8, 16, 32};
-}
 
-int jkpm_autotune_workspace(const jkpm_ell *h, int R, int steps, size_t *bytes) {
+// Moments one timed pass computes: `steps` fused steps in either form.
+int tune_moments(int steps, bool dbl) { return dbl ? 2 * steps : steps + 1; }
+}  // namespace
+
+int jkpm_autotune_workspace(const jkpm_ell *h, int R, int steps, int flags, size_t *bytes) {
     clear_error();
     if (!h || !bytes) return arg_error("NULL argument");
     if (R <= 0 || steps <= 0) return arg_error("R and steps must be positive");
+    if (flags & ~JKPM_MOMENTS_DOUBLING) return arg_error("unknown flags");
     DeviceInfo dev;
     int rc = device_info(&dev);
     if (rc) return rc;
+    const bool dbl = flags & JKPM_MOMENTS_DOUBLING;
+    const int M = tune_moments(steps, dbl);
     Ell probe = h->e;
     size_t need = 0;
     for (int m : kTuneMult) {
         probe.chunk_mult = m;
-        need = std::max(need, moments_layout(probe, R, 1, dev).total);
+        probe.dchunk_mult = m;
+        need = std::max(need, moments_layout(probe, R, 1, dev, nullptr, dbl ? M : 0).total);
     }
-    *bytes = need + align_up((size_t)(steps + 1) * R * sizeof(double2), 256);
+    *bytes = need + align_up((size_t)M * R * sizeof(double2), 256);
     return JKPM_OK;
 }
 
-int jkpm_autotune(jkpm_ell *h, const void *v_r, int R, int steps, void *workspace,
+int jkpm_autotune(jkpm_ell *h, const void *v_r, int R, int steps, int flags, void *workspace,
                   size_t workspace_bytes, void *stream, double *best_us) {
     clear_error();
     if (!h || !v_r || !workspace) return arg_error("NULL argument");
     if (R <= 0 || steps <= 0) return arg_error("R and steps must be positive");
     size_t need = 0;
-    int rc = jkpm_autotune_workspace(h, R, steps, &need);
+    int rc = jkpm_autotune_workspace(h, R, steps, flags, &need);
     if (rc) return rc;
     if (workspace_bytes < need) {
         set_error("workspace too small: need " + std::to_string(need) + " bytes");
         return JKPM_ERR_WORKSPACE;
     }
+    const bool dbl = flags & JKPM_MOMENTS_DOUBLING;
+    const int M = tune_moments(steps, dbl);
     hipStream_t s = (hipStream_t)stream;
-    const size_t mu_bytes = align_up((size_t)(steps + 1) * R * sizeof(double2), 256);
+    const size_t mu_bytes = align_up((size_t)M * R * sizeof(double2), 256);
     double2 *mu = (double2 *)workspace;
     char *ws = (char *)workspace + mu_bytes;
     const size_t ws_bytes = workspace_bytes - mu_bytes;
+    int &var = dbl ? h->e.dvariant : h->e.variant;
+    int &mult = dbl ? h->e.dchunk_mult : h->e.chunk_mult;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     JKPM_HIP(hipEventCreate(&e0));
     JKPM_HIP(hipEventCreate(&e1));
-    const int keep_v = h->e.variant, keep_m = h->e.chunk_mult;
+    const int keep_v = var, keep_m = mult;
     double best = 1e300;
     int bv = keep_v, bm = keep_m;
     for (int v = 0; v < kVariants && !rc; ++v)
         for (int m : kTuneMult) {
-            h->e.variant = v;
-            h->e.chunk_mult = m;
+            var = v;
+            mult = m;
             // one untimed pass, then the timed pass of `steps` fused steps
-            rc = run_moments(h->e, v_r, v_r, nullptr, R, steps + 1, mu, ws, ws_bytes, s);
+            rc = run_moments(h->e, v_r, v_r, nullptr, R, M, mu, ws, ws_bytes, s, nullptr, dbl);
             if (!rc) rc = hipEventRecord(e0, s) == hipSuccess ? 0 : JKPM_ERR_HIP;
-            if (!rc) rc = run_moments(h->e, v_r, v_r, nullptr, R, steps + 1, mu, ws, ws_bytes, s);
+            if (!rc) rc = run_moments(h->e, v_r, v_r, nullptr, R, M, mu, ws, ws_bytes, s, nullptr, dbl);
             if (!rc) rc = hipEventRecord(e1, s) == hipSuccess ? 0 : JKPM_ERR_HIP;
             if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? 0 : JKPM_ERR_HIP;
             if (rc) break;
@@ -566,8 +585,8 @@ int jkpm_autotune(jkpm_ell *h, const void *v_r, int R, int steps, void *workspac
         }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    h->e.variant = rc ? keep_v : bv;
-    h->e.chunk_mult = rc ? keep_m : bm;
+    var = rc ? keep_v : bv;
+    mult = rc ? keep_m : bm;
     if (rc) {
         if (g_err.empty()) set_error("autotune launch failed");
         return rc;

@@ -36,6 +36,7 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kXcd = 8;
 
+// One work-group per (vector, left set x group, step): fixed-order sum of the
 // chunk partials of that step over the group's chunks (all chunks when there
 // are no groups).
 __global__ __launch_bounds__(kBlock) void reduce_kernel(const double2 *__restrict__ ring,
@@ -165,10 +166,11 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(const int4 *__restrict__ s
 
 int max_group(int) { return 32; }
 
-int step_chunks(const Ell &H, int, const DeviceInfo &dev) {
+int step_chunks(const Ell &H, int, const DeviceInfo &dev, bool doubling) {
     // chunk_mult work-groups per CU (tunable, 8 by default), never fewer rows
     // per chunk than one wave-row sweep.
-    long target = (long)dev.cu_count * (H.chunk_mult > 0 ? H.chunk_mult : 8);
+    const int mult = doubling ? H.dchunk_mult : H.chunk_mult;
+    long target = (long)dev.cu_count * (mult > 0 ? mult : 8);
     long by_rows = (H.n + 63) / 64;
     long c = target < by_rows ? target : by_rows;
     if (c >= kXcd) c = c / kXcd * kXcd;
@@ -182,8 +184,10 @@ int launch_step(const StepArgs &a, hipStream_t s) {
     const size_t cb = cplx_bytes(H.vtype);
     const bool wide = (a.ld % 2 == 0) && (a.r0 % 2 == 0);
     const step::StepEntry e =
-        H.vtype == JKPM_C64 ? step::pick_step_c64(a.rb, wide, nl, H.k, H.variant, a.doubling)
-                            : step::pick_step_c128(a.rb, nl, H.k, H.variant, a.doubling);
+        H.vtype == JKPM_C64
+            ? step::pick_step_c64(a.rb, wide, nl, H.k, a.doubling ? H.dvariant : H.variant,
+                                  a.doubling)
+            : step::pick_step_c128(a.rb, nl, H.k, a.doubling ? H.dvariant : H.variant, a.doubling);
     if (!e.fn) return arg_error("unsupported vector group size " + std::to_string(a.rb));
     const int rpb = kWaves * (64 / e.lpr) * e.unr;
     long rpc = ((long)H.n + a.nchunks - 1) / a.nchunks;

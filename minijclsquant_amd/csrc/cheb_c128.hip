@@ -12,17 +12,28 @@ const void *ptr() {
     return reinterpret_cast<const void *>(&cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT>);
 }
 
-template <int VPL, int LPR, int KT, bool NT>
+template <int VPL, int LPR, int KT, int UNR, bool NT>
 const void *dbl_ptr() {
-    return reinterpret_cast<const void *>(&cheb_step_kernel<Real, VPL, LPR, 2, KT, 1, NT, true>);
+    return reinterpret_cast<const void *>(&cheb_step_kernel<Real, VPL, LPR, 2, KT, UNR, NT, true>);
 }
 
+template <int VPL, int LPR, int KT>
+const void *dbl_tuned(int v) {
+    switch (v & 3) {
+        case 1: return dbl_ptr<VPL, LPR, KT, 2, false>();
+        case 2: return dbl_ptr<VPL, LPR, KT, 1, true>();
+        case 3: return dbl_ptr<VPL, LPR, KT, 2, true>();
+        default: return dbl_ptr<VPL, LPR, KT, 1, false>();
+    }
+}
+
+// Doubling-form kernels: (unroll, non-temporal) variants for widths 4, 5.
 template <int VPL, int LPR>
-StepEntry dbl_entry(int k, bool nt) {
-    const void *f = k == 4 ? (nt ? dbl_ptr<VPL, LPR, 4, true>() : dbl_ptr<VPL, LPR, 4, false>())
-                  : k == 5 ? (nt ? dbl_ptr<VPL, LPR, 5, true>() : dbl_ptr<VPL, LPR, 5, false>())
-                           : dbl_ptr<VPL, LPR, 0, false>();
-    return {f, VPL, LPR, 1};
+StepEntry dbl_entry(int k, int v) {
+    const int u = (v & 1) ? 2 : 1;
+    if (k == 4) return {dbl_tuned<VPL, LPR, 4>(v), VPL, LPR, u};
+    if (k == 5) return {dbl_tuned<VPL, LPR, 5>(v), VPL, LPR, u};
+    return {dbl_ptr<VPL, LPR, 0, 1, false>(), VPL, LPR, 1};
 }
 
 template <int VPL, int LPR, int KT>
@@ -37,7 +48,7 @@ const void *tuned(int v) {
 
 template <int VPL, int LPR>
 StepEntry entry(int nl, int k, int v, bool dbl) {
-    if (dbl) return dbl_entry<VPL, LPR>(k, (v & 2) != 0);
+    if (dbl) return dbl_entry<VPL, LPR>(k, v);
     if (nl == 2) {
         const void *f = k == 4 ? ptr<VPL, LPR, 2, 4, 1, false>()
                       : k == 5 ? ptr<VPL, LPR, 2, 5, 1, false>()

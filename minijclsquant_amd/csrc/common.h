@@ -55,6 +55,9 @@ struct Ell {
     // {(1,no), (2,no), (1,yes), (2,yes)}; chunk_mult work-groups per CU.
     int variant = 0;
     int chunk_mult = 8;
+    // the same for the doubling form of the recursion
+    int dvariant = 0;
+    int dchunk_mult = 8;
     void *val = nullptr;   // [k][n] complex of vtype
     int *col = nullptr;    // [k][n]
     int device = 0;
@@ -90,7 +93,7 @@ int launch_doubling_finish(const double2 *raw, int G, int S1, int R, int M, doub
                            hipStream_t s);
 
 // Number of row chunks (work-groups) used for a matrix of n rows.
-int step_chunks(const Ell &H, int rb, const DeviceInfo &dev);
+int step_chunks(const Ell &H, int rb, const DeviceInfo &dev, bool doubling = false);
 
 // Largest vector group one launch handles for a value type.
 int max_group(int vtype);

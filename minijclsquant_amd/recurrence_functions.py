@@ -56,6 +56,12 @@ def _series(M, A, V, moments_kernel, v, dtype, table):
     from . import _native as nat
 
     n = int(A.shape[0])
+    # The reference always forms the m = 0 and m = 1 terms (its loop then
+    # runs m = 2 .. M-1), so M < 2 behaves as M = 2 for the sums and the
+    # table needs two rows.
+    if table and M < 2:
+        raise ValueError("the commutator table needs M >= 2")
+    M = max(int(M), 2)
     h = A.device(dtype, 1.0)
     vop = None
     if V is not None:

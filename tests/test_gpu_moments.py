@@ -273,14 +273,32 @@ def test_launch_variants_and_autotune(torch_gpu, dtype, tmp_path, monkeypatch):
         h.set_launch(v, 8)
         got = h.moments(r5, 90)[0].cpu().numpy()
         assert np.max(np.abs(got - ref5)) / abs(ref5[0, 0]) <= 1e-12, v
+    # the doubling form has its own setting; every shape of it agrees too
+    h.set_launch(0, 8)
+    refd = nat.moments_ex(h, None, vr, 200, True).cpu().numpy()
+    # (the two forms round differently: complex64 vectors leave ~1e-7)
+    tol_form = 1e-11 if dtype == "c128" else 1e-6
+    assert np.max(np.abs(refd - ref)) / abs(ref[0, 0]) <= tol_form
+    for v in range(12):
+        for m in (1, 4, 32):
+            h.set_launch(v, m, doubling=True)
+            assert h.get_launch() == (0, 8)
+            got = nat.moments_ex(h, None, vr, 200, True).cpu().numpy()
+            assert np.max(np.abs(got - refd)) / abs(ref[0, 0]) <= 1e-12, (v, m)
     monkeypatch.setenv("JKPM_TUNE_TABLE", str(tmp_path / "t.json"))
     v, m = autotune.tuned(h, vr, steps=4)
     assert 0 <= v <= 11 and m in (2, 4, 8, 16, 32)
     assert (tmp_path / "t.json").exists()
+    vd, md = autotune.tuned(h, vr, steps=4, doubling=True)
+    assert 0 <= vd <= 11 and md in (2, 4, 8, 16, 32)
+    assert h.get_launch(doubling=True) == (vd, md) and h.get_launch() == (v, m)
     h.set_launch(0, 8)
+    h.set_launch(0, 8, doubling=True)
     assert autotune.tuned(h, vr) == (v, m)  # from the table
+    assert autotune.tuned(h, vr, doubling=True) == (vd, md)
     with pytest.raises(ValueError):
         h.set_launch(12, 8)
+    assert nat.lib().jkpm_ell_set_launch(h.handle, 0, 8, 2) == nat.JKPM_ERR_ARG  # unknown flag
 
 
 def test_precision_form_full_size(torch_gpu):
