@@ -63,13 +63,13 @@ def build_workload(L, M, R, seed):
     return A, r, B
 
 
-def traffic_from_profile(N, K, R):
+def traffic_from_profile(N, K, R, key="traffic_bytes_per_launch"):
     """HBM bytes per launch from the committed counter summary, if it was
     measured on this exact workload shape."""
     if not TRAFFIC_FILE.exists():
         return None
     txt = TRAFFIC_FILE.read_text()
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    m = re.search(rf"^{key}:\s*([0-9.eE+]+)", txt, re.M)
     w = re.search(r"workload:\s*N=(\d+)\s+K=(\d+)\s+R=(\d+)", txt)
     if not m or not w or tuple(map(int, w.groups())) != (N, K, R):
         return None
@@ -118,13 +118,19 @@ def cpu_baseline(A, r, target_s=15.0):
     }
 
 
-def plain_c128(torch, nat, _kpm, A, r, B, M, value, value_dbl, reps=3):
+def plain_c128(torch, nat, _kpm, A, r, B, M, value, value_dbl, tune, reps=3):
     """The reference's structure on this GPU: complex128 recursion, one
-    vector at a time (R = 1), one moment per step, default launch shape.
-    Timed with HIP events on the current stream over `reps` recursions."""
+    vector at a time (R = 1), one moment per step, with its own tuned launch
+    shape.  Timed with HIP events on the current stream over `reps`
+    recursions."""
     N = int(A.shape[0])
     h = A.device("c128", 1.0 / B)
     v1 = _kpm.as_batch(r[0], N, "c128")
+    launch = None
+    if tune:
+        from minijclsquant_amd import autotune
+
+        launch = autotune.tuned(h, v1)
     ws = torch.empty(h.workspace_bytes(1, 1), dtype=torch.uint8, device="cuda")
     mu = torch.empty((1, M, 1), dtype=torch.complex128, device="cuda")
     h.moments(v1, M, workspace=ws, out=mu)
@@ -141,14 +147,204 @@ def plain_c128(torch, nat, _kpm, A, r, B, M, value, value_dbl, reps=3):
         "unit": "N*M*R/s",
         "ms_per_recursion": ms,
         "sample": f"same H, R=1 vector, M={M} moments, complex128, {reps} recursions",
+        "launch": launch,
         "headline_over_plain": value / v,
         "doubling_over_plain": None if value_dbl is None else value_dbl / v,
     }
 
 
+def event_ms(torch, fn, reps):
+    """Mean milliseconds of fn() over reps calls, HIP events on the current
+    stream."""
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        fn()
+    t1.record()
+    t1.synchronize()
+    return t0.elapsed_time(t1) / reps
+
+
+def base_line(metric, value, unit, args, ms, dtype, config, higher=True):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": higher,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic (seeded lattice disorder and random-phase vectors)",
+            "config": config}
+
+
+def bench_batch(args, torch, jcl, nat, _kpm):
+    """64 square-lattice realisations, L ~ U{128..1024}, W ~ U[0.5, 3],
+    t = -1, K = 5, R = 8, M = 1024, complex64, one block-diagonal batch:
+    the whole recursion of all realisations replayed as one HIP graph."""
+    from minijclsquant_amd import autotune
+    from minijclsquant_amd.batch import BatchKPM, square_lattice_ell
+
+    G, R, M, K = 64, 8, 1024, 5
+    rng = np.random.default_rng(2024)
+    Ls = rng.integers(128, 1025, size=G)
+    Ws = rng.uniform(0.5, 3.0, size=G)
+    log(f"building {G} realisations, sum N = {int((Ls ** 2).sum())}")
+    blocks, vecs = [], []
+    for g in range(G):
+        L = int(Ls[g])
+        d, ix = square_lattice_ell(L, -1.0, float(Ws[g]), seed=5000 + g)
+        blocks.append((d, ix, L * L, K, 1.0 / ((4 + Ws[g] / 2) * 1.01)))
+        vecs.append(np.stack([jcl.random_vector_generator(L * L, 100 * g + s)
+                              for s in range(R)]))
+    bk = BatchKPM(blocks, vecs, M, "c64")
+    del blocks, vecs
+    N = int(bk.offsets[-1])
+    if not args.no_tune:
+        v, m = autotune.tuned(bk.H, bk.vr)
+        vd, md = autotune.tuned(bk.H, bk.vr, doubling=True)
+        log(f"launch shapes: plain variant {v}, doubling variant {vd}")
+    bk.capture()
+    for _ in range(args.warmup):
+        bk.replay()
+    torch.cuda.synchronize()
+    ms = event_ms(torch, bk.replay, args.steps)
+    log(f"batch graph replay: {ms:.1f} ms")
+    ws_d = torch.empty(nat.moments_ex_workspace_bytes(None, bk.batch, R, M, True),
+                       dtype=torch.uint8, device="cuda")
+    mu_d = torch.empty((G, M, R), dtype=torch.complex128, device="cuda")
+
+    def dbl():
+        nat.moments_ex(None, bk.batch, bk.vr, M, True, workspace=ws_d, out=mu_d)
+
+    dbl()
+    ms_d = event_ms(torch, dbl, args.steps)
+    diff = float((mu_d - bk.mu).abs().max() / bk.mu[:, 0].abs().max())
+    units = N * M * R
+    bytes_launch = N * K * 12 + R * N * 4 * 8
+    bytes_d = N * K * 12 + R * N * 3 * 8
+    ach = bytes_launch * (M - 1) / (ms / 1e3) / 1e9
+    ach_d = bytes_d * max(1, M // 2) / (ms_d / 1e3) / 1e9
+    out = base_line("batched realisations: Chebyshev SpMV-steps/s (sum N * M * R / s)",
+                    units / (ms / 1e3), "N*M*R/s", args, ms, "c64",
+                    {"workload": f"{G} square-lattice realisations, L ~ U{{128..1024}} "
+                                 "(seed 2024), W ~ U[0.5, 3], t = -1, K = 5, R = 8, "
+                                 "M = 1024, complex64 recursion + FP64 moments, one "
+                                 "block-diagonal batch replayed as one HIP graph",
+                     "G": G, "sum_N": N, "K": K, "M": M, "R": R})
+    out["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                       "note": "whole replay: (M-1) step launches' algorithmic bytes over the "
+                               "replay time (reductions and gaps included)",
+                       "algorithmic_bytes_per_launch": bytes_launch}
+    out["doubling"] = {"value": units / (ms_d / 1e3), "ms_per_step": ms_d,
+                       "frac": round(ach_d / HBM_PEAK_GBS, 4),
+                       "max_rel_moment_diff_vs_plain": diff}
+    print(json.dumps(out), flush=True)
+
+
+def bench_grad(args, torch, jcl, nat, _kpm):
+    """Square 512 x 512, onsite U[-1, 1], t_ij = -1 + N(0, 0.05^2), M = 1024,
+    R = 8, complex128: loss = sum_E w(E) DOS(E) (256 energies, Gaussian w),
+    forward (moments + DOS) and backward (adjoint recursion) timed."""
+    from minijclsquant_amd.autograd import (chebyshev_moments, kpm_dos_torch, lattice_values,
+                                            param_slots)
+
+    L, M, R, n_e, sig = 512, 1024, 8, 256, 0.05
+    H, _ = jcl.square_lattice(L, t=-1.0, seed=2024, W=2.0, bond_noise=sig)
+    A = jcl.ell_matrix(H)
+    B = (4 * (1 + 6 * sig) + 1.0) * 1.01
+    N, K = int(A.shape[0]), int(A.len_row)
+    slot_param, pairs = param_slots(A)
+    rows = np.arange(A.data.size) // K
+    eps0 = np.real(A.data[(rows == A.indices) & (A.valid_row_values == 1)])
+    Hc = A.A.tocsr()
+    t0 = np.real(np.asarray(Hc[pairs[:, 0], pairs[:, 1]]).ravel())
+    r = np.stack([jcl.random_vector_generator(N, 300 + s) for s in range(R)])
+    rd = torch.from_numpy(r.T.copy()).cuda()
+    idx = torch.from_numpy(A.indices).cuda()
+    es = torch.as_tensor(np.linspace(-B + 0.05, B - 0.05, n_e) / B, device="cuda")
+    w = torch.as_tensor(np.random.default_rng(2025).normal(size=n_e), device="cuda")
+    eps = torch.tensor(eps0, device="cuda", requires_grad=True)
+    hop = torch.tensor(t0, device="cuda", requires_grad=True)
+
+    def forward():
+        vals = lattice_values(eps, hop, slot_param)
+        mu = chebyshev_moments(vals, idx, rd, M, scale=1.0 / B)
+        return (w * kpm_dos_torch(mu, es)).sum()
+
+    def both():
+        eps.grad = None
+        hop.grad = None
+        forward().backward()
+
+    for _ in range(args.warmup):
+        both()
+    torch.cuda.synchronize()
+    with torch.no_grad():
+        ms_f = event_ms(torch, forward, args.steps)
+    ms_fb = event_ms(torch, both, args.steps)
+    units = N * M * R
+    out = base_line("differentiation: forward + adjoint backward, N * M * R / s",
+                    units / (ms_fb / 1e3), "N*M*R/s", args, ms_fb, "c128",
+                    {"workload": "square 512x512 periodic, onsite U[-1, 1], "
+                                 "t_ij = -1 + N(0, 0.05^2), K = 5, M = 1024, R = 8, "
+                                 "complex128, loss = sum_E w(E) DOS(E) on 256 energies; "
+                                 "gradients for all onsite energies and bonds",
+                     "N": N, "K": K, "M": M, "R": R, "params": int(N + pairs.shape[0])})
+    bytes_launch = N * K * 20 + R * N * 4 * 16
+    out["forward_ms"] = ms_f
+    out["backward_ms"] = ms_fb - ms_f
+    out["backward_over_forward"] = (ms_fb - ms_f) / ms_f
+    out["forward_roofline"] = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "achieved": round(bytes_launch * (M - 1) / (ms_f / 1e3) / 1e9, 1),
+                               "note": "forward step bytes x (M-1) over the forward time"}
+    print(json.dumps(out), flush=True)
+
+
+def bench_kubo(args, torch, jcl, nat, _kpm):
+    """Honeycomb 512 x 512 cells (N = 524,288), t = -2.7, W = 1 eV, v_x on
+    H's slots, M = 512, R = 4: mu_nm (complex64 stacks, FP32 MFMA GEMM,
+    FP64 combine) and sigma on 512 energies."""
+    from minijclsquant_amd.kubo import velocity_handle
+
+    Lc, M, R, n_e = 512, 512, 4, 512
+    H, Dx, Dy = jcl.honeycomb_lattice(Lc, t=-2.7, W=1.0, seed=2024)
+    A = jcl.ell_matrix(H, Dx, Dy)
+    B = (3 * 2.7 + 0.5) * 1.01
+    A.bounds = np.array([-B, B])
+    N = int(A.shape[0])
+    Hd, Vd = velocity_handle(A, "x")
+    r = np.stack([jcl.random_vector_generator(N, 700 + s) for s in range(R)])
+    rr = _kpm.as_batch(r, N, "c64")
+    ws = torch.empty(nat.kubo_workspace_bytes(Hd, R, M), dtype=torch.uint8, device="cuda")
+    mu = torch.empty((M, M), dtype=torch.complex128, device="cuda")
+    es = torch.as_tensor(np.linspace(-0.99, 0.99, n_e), device="cuda")
+
+    def step():
+        nat.kubo_mu_nm(Hd, Vd, rr, M, workspace=ws, out=mu)
+        return nat.kubo_sigma(mu, es, 0, B)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ms = event_ms(torch, step, args.steps)
+    flops = 8.0 * N * M * M * R
+    tf = flops / (ms / 1e3) / 1e12
+    out = base_line("Kubo double moments: mu_nm GEMM TFLOP/s (8 N M^2 R per step)",
+                    tf, "TFLOP/s", args, ms, "c64 stacks, fp32 MFMA",
+                    {"workload": "honeycomb 512x512 cells periodic, t = -2.7, W = 1 eV, "
+                                 "v_x on H's slots, M = 512, R = 4, complex64 stacks, FP32 "
+                                 "MFMA accumulate, FP64 combine, sigma on 512 energies",
+                     "N": N, "M": M, "R": R})
+    out["roofline"] = {"bound": "mfma", "achieved": round(tf, 2), "peak": 157.3,
+                       "unit": "TFLOP/s", "frac": round(tf / 157.3, 4), "traffic": None,
+                       "note": "whole step (stack recursion + GEMM + sigma) against the GEMM's "
+                               "flops; the GEMM kernel alone is in the committed profile"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="dos", choices=["dos", "batch", "grad", "kubo"],
+                    help="dos: the headline precision-form DOS line; batch / grad / kubo: "
+                         "the realisation-batch, differentiation and Kubo workloads")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=1024)
@@ -166,7 +362,8 @@ def main():
                     help="keep the default launch shape instead of the tuned one")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware counters: 3 copies of 1 GiB with the "
-                         "library's copy kernel, then 16 moments of the workload")
+                         "library's copy kernel, then 16 moments of the workload (15 "
+                         "steps) and 32 moments in the doubling form (16 steps)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -182,6 +379,10 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
+    if args.workload != "dos":
+        {"batch": bench_batch, "grad": bench_grad, "kubo": bench_kubo}[args.workload](
+            args, torch, jcl, nat, _kpm)
+        return
     L, M, R = args.cells, args.moments, args.vectors
     log(f"building workload: honeycomb {L}x{L} cells, M={M}, R={R}, {args.dtype}")
     A, r, B = build_workload(L, M, R, seed=2024)
@@ -201,15 +402,24 @@ def main():
         return nat.dos(mu[0], es, nat.JKPM_KERNEL_JACKSON)
 
     if args.probe:
+        # launch shapes from the tuning table (tuned here only when absent)
+        launch = {}
+        if not args.no_tune:
+            from minijclsquant_amd import autotune
+
+            launch["plain"] = autotune.tuned(h, vr)
+            launch["doubling"] = autotune.tuned(h, vr, doubling=True)
         nbytes = 1 << 30
         src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
             nat.copy(dst, src, nbytes)
         h.moments(vr, 16, workspace=ws)
+        nat.moments_ex(h, None, vr, 32, True)
         torch.cuda.synchronize()
         print(json.dumps({"probe": "done", "copy_bytes": nbytes, "N": N, "K": K, "R": R,
-                          "step_launches": 15}), flush=True)
+                          "step_launches": 15, "doubling_step_launches": 16,
+                          "launch": launch}), flush=True)
         return
 
     if not args.no_tune:
@@ -288,6 +498,8 @@ def main():
                      "mu_2k+1 = 2<T_k|T_k+1> - mu_1 (Hermitian H); no v_l stream"),
             "roofline": {"bound": "hbm", "achieved": round(ach_d, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(ach_d / HBM_PEAK_GBS, 4),
+                         "traffic": traffic_from_profile(N, K, R,
+                                                         "doubling_traffic_bytes_per_launch"),
                          "algorithmic_bytes_per_launch": bytes_d,
                          "mean_launch_us": round(launch_d * 1e3, 2), "launches_timed": n_d},
             "dos_err_vs_c128": (None if d128 is None else
@@ -298,7 +510,7 @@ def main():
     plain = None
     if not args.no_plain:
         plain = plain_c128(torch, nat, _kpm, A, r, B, M, units / (ms_per_step / 1e3),
-                           None if dbl is None else dbl["value"])
+                           None if dbl is None else dbl["value"], not args.no_tune)
         log(f"plain complex128 R=1 recursion: {plain['ms_per_recursion']:.1f} ms per "
             f"{M}-moment recursion")
 

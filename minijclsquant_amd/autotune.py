@@ -35,7 +35,9 @@ def table_path() -> Path:
 def _key(h: "nat.DeviceELL", R: int, doubling: bool) -> str:
     import torch
 
-    dev = torch.cuda.get_device_name(torch.cuda.current_device())
+    # architecture and CU count: stable under profilers, unlike the marketing name
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    dev = f"{getattr(p, 'gcnArchName', p.name)}|cu={p.multi_processor_count}"
     form = "|doubling" if doubling else ""
     return f"{dev}|N={h.n}|K={h.k}|vtype={h.vtype}|R={R}{form}"
 
