@@ -177,7 +177,6 @@ def bench_batch(args, torch, jcl, nat, _kpm):
     """64 square-lattice realisations, L ~ U{128..1024}, W ~ U[0.5, 3],
     t = -1, K = 5, R = 8, M = 1024, complex64, one block-diagonal batch:
     the whole recursion of all realisations replayed as one HIP graph."""
-    from minijclsquant_amd import autotune
     from minijclsquant_amd.batch import BatchKPM, square_lattice_ell
 
     G, R, M, K = 64, 8, 1024, 5
@@ -192,13 +191,11 @@ def bench_batch(args, torch, jcl, nat, _kpm):
         blocks.append((d, ix, L * L, K, 1.0 / ((4 + Ws[g] / 2) * 1.01)))
         vecs.append(np.stack([jcl.random_vector_generator(L * L, 100 * g + s)
                               for s in range(R)]))
-    bk = BatchKPM(blocks, vecs, M, "c64")
+    bk = BatchKPM(blocks, vecs, M, "c64", tune=not args.no_tune)
     del blocks, vecs
     N = int(bk.offsets[-1])
-    if not args.no_tune:
-        v, m = autotune.tuned(bk.H, bk.vr)
-        vd, md = autotune.tuned(bk.H, bk.vr, doubling=True)
-        log(f"launch shapes: plain variant {v}, doubling variant {vd}")
+    log(f"launch shapes (variant, work-groups per CU): plain {bk.H.get_launch()}, "
+        f"doubling {bk.H.get_launch(doubling=True)}")
     bk.capture()
     for _ in range(args.warmup):
         bk.replay()

@@ -145,7 +145,10 @@ typedef struct jkpm_batch jkpm_batch;
  * rows [row_offsets[g], row_offsets[g+1]) (host array of G+1, strictly
  * increasing, 0 .. N); column indices of realization g stay inside its rows.
  * The rows are cut into work-group chunks proportional to each realization's
- * size that never straddle two realizations.  h must outlive the batch. */
+ * size that never straddle two realizations, about (CU count x the handle's
+ * chunk_mult) of them; the plain recursion and the doubling form each get a
+ * table from their own setting as it is at creation (tune the handle first).
+ * h must outlive the batch. */
 int jkpm_batch_create(const jkpm_ell *h, int G, const long long *row_offsets,
                       jkpm_batch **out);
 int jkpm_batch_destroy(jkpm_batch *b);

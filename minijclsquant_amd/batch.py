@@ -72,20 +72,30 @@ class BatchKPM:
 
     blocks: list of (data, indices, n_rows, len_row, scale) -- row-major ELL
     of each realization and the scale 1/B_g of its spectral rescale.
-    vectors: list of [R, N_g] arrays (same R for every realization)."""
+    vectors: list of [R, N_g] arrays (same R for every realization).
+    tune: pick the step kernel's launch shapes (plain and doubling) with the
+    autotuner before the batch's work-group chunks are laid out."""
 
-    def __init__(self, blocks, vectors, M, dtype="c64", doubling=False):
+    def __init__(self, blocks, vectors, M, dtype="c64", doubling=False, tune=False):
         torch = nat.require_device()
         data, idx, offsets, K = block_diagonal(blocks)
         self.offsets = offsets
         self.M = int(M)
         self.doubling = bool(doubling)
         self.H = nat.DeviceELL(data, idx, int(offsets[-1]), K, 1.0, dtype)
-        self.batch = nat.Batch(self.H, offsets)
+        del data, idx
         v = np.concatenate([np.asarray(vg).reshape(-1, int(b[2])) for vg, b in zip(vectors, blocks)],
                            axis=1)
         self.R = v.shape[0]
         self.vr = _kpm.as_batch(v, int(offsets[-1]), dtype)
+        del v
+        if tune:
+            # the batch's chunk tables follow the handle's launch settings
+            from . import autotune
+
+            autotune.tuned(self.H, self.vr)
+            autotune.tuned(self.H, self.vr, doubling=True)
+        self.batch = nat.Batch(self.H, offsets)
         self.ws = None
         self.mu = torch.empty((self.batch.G, self.M, self.R), dtype=torch.complex128, device="cuda")
         self.graph = None

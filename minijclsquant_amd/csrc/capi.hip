@@ -11,6 +11,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -597,20 +598,88 @@ int jkpm_autotune(jkpm_ell *h, const void *v_r, int R, int steps, int flags, voi
 
 // ---- block-diagonal batches of realizations ----
 
+// One chunk table of a batch: chunk row starts (device [nchunks+1]) and the
+// first chunk of every realization (device [G+1]).
+struct ChunkTable {
+    long *chunk_rows = nullptr;
+    int *group_chunks = nullptr;
+    int nchunks = 0;
+};
+
 struct jkpm_batch {
     const jkpm_ell *h = nullptr;  // borrowed: must outlive the batch
-    long *chunk_rows = nullptr;   // device
-    int *group_chunks = nullptr;  // device
-    int G = 0, nchunks = 0;
-    Groups grp() const {
+    int G = 0;
+    ChunkTable t[2];              // [0] plain recursion, [1] doubling form
+    Groups grp(bool doubling = false) const {
+        const ChunkTable &c = t[doubling ? 1 : 0];
         Groups g;
-        g.chunk_rows = chunk_rows;
-        g.group_chunks = group_chunks;
+        g.chunk_rows = c.chunk_rows;
+        g.group_chunks = c.group_chunks;
         g.G = G;
-        g.nchunks = nchunks;
+        g.nchunks = c.nchunks;
         return g;
     }
 };
+
+namespace {
+
+void free_table(ChunkTable &c) {
+    hipFree(c.chunk_rows);
+    hipFree(c.group_chunks);
+    c = ChunkTable{};
+}
+
+// About cu_count * mult chunks over the whole batch; no chunk straddles two
+// realizations.  Every realization starts with one chunk and the remaining
+// chunks go, one at a time, to the realization whose chunks are currently the
+// largest, so the largest chunk (the launch's critical path) stays close to
+// the mean even when realizations are few chunks long.
+int build_table(const jkpm_ell *h, int G, const long long *row_offsets, int mult,
+                const DeviceInfo &dev, ChunkTable *out) {
+    const long target = std::max<long>(G, (long)dev.cu_count * mult);
+    std::vector<long> cnt(G, 1);
+    auto rows_of = [&](int g) { return (long)(row_offsets[g + 1] - row_offsets[g]); };
+    auto size_of = [&](int g) { return (double)rows_of(g) / (double)cnt[g]; };
+    auto cmp = [&](int a, int b) { return size_of(a) < size_of(b); };
+    std::priority_queue<int, std::vector<int>, decltype(cmp)> q(cmp);
+    for (int g = 0; g < G; ++g) q.push(g);
+    long total = G;
+    while (total < target && !q.empty()) {
+        const int g = q.top();
+        q.pop();
+        if (rows_of(g) / 64 <= cnt[g]) continue;  // chunks of at least 64 rows: g is done
+        ++cnt[g];
+        ++total;
+        q.push(g);
+    }
+    std::vector<long> rows{0};
+    std::vector<int> gch{0};
+    for (int g = 0; g < G; ++g) {
+        const long n0 = row_offsets[g], ng = rows_of(g);
+        long per = (ng + cnt[g] - 1) / cnt[g];
+        per = (per + 63) / 64 * 64;
+        for (long r0 = 0; r0 < ng; r0 += per) rows.push_back(n0 + std::min(ng, r0 + per));
+        gch.push_back((int)rows.size() - 1);
+    }
+    // Pad with empty chunks to a multiple of the XCD count; they belong to no group.
+    while ((rows.size() - 1) % 8 != 0 && rows.size() - 1 >= 8) rows.push_back(h->e.n);
+    ChunkTable c;
+    c.nchunks = (int)rows.size() - 1;
+    hipError_t e;
+    if ((e = hipMalloc(&c.chunk_rows, rows.size() * sizeof(long))) != hipSuccess ||
+        (e = hipMalloc(&c.group_chunks, gch.size() * sizeof(int))) != hipSuccess ||
+        (e = hipMemcpy(c.chunk_rows, rows.data(), rows.size() * sizeof(long),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c.group_chunks, gch.data(), gch.size() * sizeof(int),
+                       hipMemcpyHostToDevice)) != hipSuccess) {
+        free_table(c);
+        return hip_fail(e, "batch tables", __FILE__, __LINE__);
+    }
+    *out = c;
+    return JKPM_OK;
+}
+
+}  // namespace
 
 int jkpm_batch_create(const jkpm_ell *h, int G, const long long *row_offsets, jkpm_batch **out) {
     clear_error();
@@ -626,36 +695,17 @@ int jkpm_batch_create(const jkpm_ell *h, int G, const long long *row_offsets, jk
     DeviceInfo dev;
     int rc = device_info(&dev);
     if (rc) return rc;
-    const double target = (double)dev.cu_count * 8;
-    std::vector<long> rows{0};
-    std::vector<int> gch{0};
-    for (int g = 0; g < G; ++g) {
-        const long n0 = row_offsets[g], ng = row_offsets[g + 1] - n0;
-        long c = (long)std::llround(target * (double)ng / (double)h->e.n);
-        c = std::max(1L, std::min(c, (ng + 63) / 64));
-        long per = (ng + c - 1) / c;
-        per = (per + 63) / 64 * 64;
-        for (long r0 = 0; r0 < ng; r0 += per) rows.push_back(n0 + std::min(ng, r0 + per));
-        gch.push_back((int)rows.size() - 1);
-    }
-    // Pad with empty chunks to a multiple of the XCD count; they belong to no group.
-    while ((rows.size() - 1) % 8 != 0 && rows.size() - 1 >= 8) rows.push_back(h->e.n);
     auto *b = new (std::nothrow) jkpm_batch;
     if (!b) return arg_error("host allocation failed");
     b->h = h;
     b->G = G;
-    b->nchunks = (int)rows.size() - 1;
-    hipError_t e;
-    if ((e = hipMalloc(&b->chunk_rows, rows.size() * sizeof(long))) != hipSuccess ||
-        (e = hipMalloc(&b->group_chunks, gch.size() * sizeof(int))) != hipSuccess ||
-        (e = hipMemcpy(b->chunk_rows, rows.data(), rows.size() * sizeof(long),
-                       hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(b->group_chunks, gch.data(), gch.size() * sizeof(int),
-                       hipMemcpyHostToDevice)) != hipSuccess) {
-        hipFree(b->chunk_rows);
-        hipFree(b->group_chunks);
+    rc = build_table(h, G, row_offsets, h->e.chunk_mult, dev, &b->t[0]);
+    if (!rc) rc = build_table(h, G, row_offsets, h->e.dchunk_mult, dev, &b->t[1]);
+    if (rc) {
+        free_table(b->t[0]);
+        free_table(b->t[1]);
         delete b;
-        return hip_fail(e, "batch tables", __FILE__, __LINE__);
+        return rc;
     }
     *out = b;
     return JKPM_OK;
@@ -664,8 +714,8 @@ int jkpm_batch_create(const jkpm_ell *h, int G, const long long *row_offsets, jk
 int jkpm_batch_destroy(jkpm_batch *b) {
     clear_error();
     if (!b) return JKPM_OK;
-    hipFree(b->chunk_rows);
-    hipFree(b->group_chunks);
+    free_table(b->t[0]);
+    free_table(b->t[1]);
     delete b;
     return JKPM_OK;
 }
@@ -746,7 +796,7 @@ int jkpm_moments_ex_workspace(const jkpm_ell *h, const jkpm_batch *b, int R, int
     if (rc) return rc;
     const bool dbl = (flags & JKPM_MOMENTS_DOUBLING) != 0;
     if (b) {
-        const Groups g = b->grp();
+        const Groups g = b->grp(dbl);
         *bytes = moments_layout(b->h->e, R, 1, dev, &g, dbl ? M : 0).total;
     } else {
         *bytes = moments_layout(h->e, R, 1, dev, nullptr, dbl ? M : 0).total;
@@ -765,7 +815,7 @@ int jkpm_moments_ex(const jkpm_ell *h, const jkpm_batch *b, const void *v_r, int
         return arg_error("device pointers must be 16-byte aligned");
     const bool dbl = (flags & JKPM_MOMENTS_DOUBLING) != 0;
     if (b) {
-        const Groups g = b->grp();
+        const Groups g = b->grp(dbl);
         return run_moments(b->h->e, v_r, v_r, nullptr, R, M, (double2 *)mu, workspace,
                            workspace_bytes, (hipStream_t)stream, &g, dbl);
     }

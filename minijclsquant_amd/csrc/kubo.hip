@@ -177,10 +177,12 @@ __global__ __launch_bounds__(kBlock) void kubo_gemm_kernel(
             for (int r = 0; r < 16; ++r) cre[i][j][r] = cim[i][j][r] = 0.f;
 
     // Loader mapping: thread t covers row t / 2 of both tiles, 16 complex k.
+    // The next stage's global loads are issued before the current stage's
+    // MFMAs, so their latency hides behind the matrix work.
     const int lrow = t >> 1, lk0 = (t & 1) * (TK / 2);
-    for (long kc = k_begin; kc < k_end; kc += TK) {
-        float2 av[TK / 2], bv[TK / 2];
-        const int gm = m0 + lrow, gn = n0 + lrow;
+    const int gm = m0 + lrow, gn = n0 + lrow;
+    float2 av[TK / 2], bv[TK / 2];
+    auto load = [&](long kc) {
 #pragma unroll
         for (int q = 0; q < TK / 2; ++q) {
             const long kk = kc + lk0 + q;
@@ -188,6 +190,9 @@ __global__ __launch_bounds__(kBlock) void kubo_gemm_kernel(
             av[q] = (kin && gm < M) ? L[(long)gm * K + kk] : make_float2(0.f, 0.f);
             bv[q] = (kin && gn < M) ? Rk[(long)gn * K + kk] : make_float2(0.f, 0.f);
         }
+    };
+    if (k_begin < k_end) load(k_begin);
+    for (long kc = k_begin; kc < k_end; kc += TK) {
         __syncthreads();  // previous stage fully consumed
 #pragma unroll
         for (int q = 0; q < TK / 2; ++q) {
@@ -197,6 +202,7 @@ __global__ __launch_bounds__(kBlock) void kubo_gemm_kernel(
             Bi[lk0 + q][lrow] = bv[q].y;
         }
         __syncthreads();
+        if (kc + TK < k_end) load(kc + TK);
 #pragma unroll 4
         for (int kk = 0; kk < TK; ++kk) {
             float a[2], bre[2], bim[2];

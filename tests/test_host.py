@@ -194,3 +194,60 @@ def test_recursion_api_argument_behaviour(capsys):
     assert jcl.rec_A_tab(10, A) is None
     assert "must introduce" in capsys.readouterr().out
     assert jcl.rec_A_tab2v(10, A, None, None) is None
+
+
+def test_coefficient_tables_closed_forms():
+    """Each table against the function it expands, summed with numpy's own
+    Chebyshev series (Clenshaw), or against an independent integral."""
+    from numpy.polynomial import chebyshev as C
+    import scipy.integrate as I
+
+    x = np.linspace(-0.95, 0.95, 41)
+    # Jacobi-Anger: sum_n c_n T_n(x) = exp(-i x t)
+    for t in (0.5, 7.3):
+        c = jcl.moments_U(t, 80)
+        assert np.max(np.abs(C.chebval(x, c) - np.exp(-1j * x * t))) < 1e-12
+    # 1/(z - x) and its Lorentzian imaginary part
+    z = 0.3 + 0.2j
+    c = jcl.moments_Gmas_2(0.3, 0.2, 600)
+    assert np.max(np.abs(C.chebval(x, c) - 1 / (z - x))) < 1e-10
+    lor = 0.2 / np.pi / ((0.3 - x) ** 2 + 0.2 ** 2)
+    assert np.max(np.abs(C.chebval(x, jcl.moments_delta_2(0.3, 0.2, 600)) - lor)) < 1e-10
+    # the i0 limits: Gmas = lim Gmas_2, Gmin = conj(Gmas), Gsum = Gmas + Gmin
+    g = jcl.moments_Gmas(0.3, 50)
+    assert np.max(np.abs(g - jcl.moments_Gmas_2(0.3, 1e-12, 50))) < 1e-8
+    assert np.allclose(jcl.moments_Gmin(0.3, 50), np.conj(g), rtol=0, atol=1e-14)
+    assert np.allclose(jcl.moments_Gsum(0.3, 50), g + np.conj(g), rtol=0, atol=1e-13)
+    # step theta(Ef - x): c_n = (2 - [n==0]) / pi * integral over theta in [acos Ef, pi]
+    Ef = -0.35
+    c = jcl.moments_FD_T0(Ef, 30)
+    th0 = np.arccos(Ef)
+    ex = [(1 if n == 0 else 2) / np.pi * I.quad(lambda th: np.cos(n * th), th0, np.pi)[0]
+          for n in range(30)]
+    assert np.max(np.abs(c - ex)) < 1e-12
+    # Fermi-Dirac tables: the two entry points sample the same occupation
+    N, T = 32, 0.05
+    E = np.linspace(-1 + 1e-3, 1 - 1e-3, 2 * N)
+    fd = jcl.moments_FD_T(0.1, T, N)
+    assert np.array_equal(fd, jcl.moments_FD(N, 1 / (1 + np.exp((E - 0.1) / T))))
+    # ... and at low temperature approach the step (Simpson on 2N points is coarse)
+    # (the grid stops 1e-3 short of the 1/sqrt(1-x^2) end points: ~1% off)
+    assert abs(jcl.moments_FD_T(0.1, 1e-3, 400)[0] - jcl.moments_FD_T0(0.1, 1)[0]) < 2e-2
+    # sum_i c_i matrix[i]
+    rng = np.random.default_rng(0)
+    mat = rng.normal(size=(7, 5)) + 1j * rng.normal(size=(7, 5))
+    cc = rng.normal(size=4)
+    assert np.allclose(jcl.moments_by_matrix(cc, mat), cc @ mat[:4], rtol=0, atol=1e-13)
+
+
+def test_coefficient_tables_reject_planted_mistakes():
+    """The closed-form checks above catch plausible slips in a table."""
+    from numpy.polynomial import chebyshev as C
+
+    x = np.linspace(-0.95, 0.95, 41)
+    c = jcl.moments_U(3.0, 60)
+    bad = c.copy()
+    bad[1:] /= 2  # forgotten factor 2 on n >= 1
+    assert np.max(np.abs(C.chebval(x, bad) - np.exp(-1j * x * 3.0))) > 1e-2
+    bad = np.conj(c)  # (+i)^n instead of (-i)^n
+    assert np.max(np.abs(C.chebval(x, bad) - np.exp(-1j * x * 3.0))) > 1e-2
