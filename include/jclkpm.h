@@ -85,6 +85,26 @@ int jkpm_ell_info(const jkpm_ell *h, int *n_rows, int *len_row, int *value_type)
 int jkpm_ell_set_values(jkpm_ell *h, const double *data, double scale,
                         int src_on_device, void *stream);
 
+/* Row-major complex128 copy of the values in the units they were uploaded in
+ * (stored values / scale), to host or device memory (dst_on_device).  The
+ * device side of ell_matrix.data (jclsquant/ell_matrix.py:119-258) after a
+ * device-side modifier.  Synchronises `stream`. */
+int jkpm_ell_get_values(const jkpm_ell *h, double *data, int dst_on_device, void *stream);
+
+/* Peierls substitution on the device:
+ *   dst(i,s) = a * src(i,s) * exp(i (ax dx(i,s) + ay dy(i,s)))
+ * with (dx, dy) = (Re, Im) of the values of `bonds` (the bond vectors, made
+ * with jkpm_ell_create_like(src, dx + i dy, ...)); a, ax, ay complex
+ * (re, im); dst shares src's slot pattern and value type (dst may be src)
+ * and takes src's scale.  Replaces the copy + rewrite of H per time step of
+ * jclsquant/cuda_cython/cuda/obv.cu:335-341 with light_modifier_circular_
+ * packed_c (modifiers.cu:25-68, alpha = i ax, beta = i ay) and the CPU
+ * light_modifier_cython (jclsquant/cython_modules/Extra_funs.pyx:327-342).
+ * Asynchronous on stream. */
+int jkpm_ell_peierls(jkpm_ell *dst, const jkpm_ell *src, const jkpm_ell *bonds, double a_re,
+                     double a_im, double ax_re, double ax_im, double ay_re, double ay_im,
+                     void *stream);
+
 /* y <- a*A*x + b*y on R interleaved device vectors of the handle's value type.
  * Replaces aAxby_device (jclsquant/cuda_cython/cuda/blas.cu:29-43) and the
  * CPU aAxby (jclsquant/cython_modules/blas_funs.pyx:64-75).  x must not alias y.

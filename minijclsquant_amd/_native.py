@@ -48,6 +48,8 @@ SIGNATURES = {
     "jkpm_ell_destroy": (_int, [_vp]),
     "jkpm_ell_info": (_int, [_vp, _ip, _ip, _ip]),
     "jkpm_ell_set_values": (_int, [_vp, _vp, _dbl, _int, _vp]),
+    "jkpm_ell_get_values": (_int, [_vp, _vp, _int, _vp]),
+    "jkpm_ell_peierls": (_int, [_vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp]),
     "jkpm_aAxby": (_int, [_vp, _dp, _dp, _vp, _vp, _int, _vp]),
     "jkpm_moments_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
     "jkpm_moments": (_int, [_vp, _vp, _vp, _vp, _int, _int, _vp, _vp, _size, _vp]),
@@ -376,6 +378,25 @@ class DeviceELL:
                                  out.data_ptr(), workspace.data_ptr(), workspace.numel(),
                                  stream_ptr(stream)))
         return out
+
+    def values(self, stream=None):
+        """Row-major complex128 values [n*k] (device tensor) in the units they
+        were uploaded in."""
+        torch = _torch()
+        out = torch.empty(self.n * self.k, dtype=torch.complex128, device="cuda")
+        check(lib().jkpm_ell_get_values(self._h, out.data_ptr(), 1, stream_ptr(stream)))
+        return out
+
+    def peierls(self, src: "DeviceELL", bonds: "DeviceELL", a=1.0, ax=0.0, ay=0.0,
+                stream=None) -> "DeviceELL":
+        """self <- a * src * exp(i (ax dx + ay dy)) slot by slot, (dx, dy) the
+        real and imaginary parts of bonds' values; self, src and bonds share
+        one slot pattern.  Asynchronous on stream."""
+        a, ax, ay = complex(a), complex(ax), complex(ay)
+        check(lib().jkpm_ell_peierls(self._h, src.handle, bonds.handle, a.real, a.imag, ax.real,
+                                     ax.imag, ay.real, ay.imag, stream_ptr(stream)))
+        self.scale = src.scale
+        return self
 
     def set_launch(self, variant: int, chunk_mult: int, doubling: bool = False) -> None:
         """Step-kernel launch shape of the plain recursion (or of the

@@ -442,6 +442,45 @@ int jkpm_ell_set_values(jkpm_ell *h, const double *data, double scale, int src_o
     return JKPM_OK;
 }
 
+int jkpm_ell_get_values(const jkpm_ell *h, double *data, int dst_on_device, void *stream) {
+    clear_error();
+    if (!h || !data) return arg_error("handle/data is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t bytes = (size_t)h->e.n * h->e.k * 2 * sizeof(double);
+    if (dst_on_device) {
+        int rc = launch_ell_unpack(h->e, data, s);
+        if (rc) return rc;
+        JKPM_HIP(hipStreamSynchronize(s));
+        return JKPM_OK;
+    }
+    double *tmp = nullptr;
+    JKPM_HIP(hipMalloc(&tmp, bytes ? bytes : 16));
+    int rc = launch_ell_unpack(h->e, tmp, s);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpyAsync(data, tmp, bytes, hipMemcpyDeviceToHost, s);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(tmp);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "copy of values to the host", __FILE__, __LINE__);
+    return JKPM_OK;
+}
+
+int jkpm_ell_peierls(jkpm_ell *dst, const jkpm_ell *src, const jkpm_ell *bonds, double a_re,
+                     double a_im, double ax_re, double ax_im, double ay_re, double ay_im,
+                     void *stream) {
+    clear_error();
+    if (!dst || !src || !bonds) return arg_error("NULL handle");
+    if (src->e.col != dst->e.col || src->e.col != bonds->e.col)
+        return arg_error("dst, src and bonds must share one slot pattern (jkpm_ell_create_like)");
+    if (dst->e.vtype != src->e.vtype) return arg_error("dst and src must have one value type");
+    int rc = launch_peierls(src->e, bonds->e, &dst->e, make_double2(a_re, a_im),
+                            make_double2(ax_re, ax_im), make_double2(ay_re, ay_im),
+                            (hipStream_t)stream);
+    if (rc) return rc;
+    dst->e.scale = src->e.scale;
+    return JKPM_OK;
+}
+
 int jkpm_aAxby(const jkpm_ell *h, const double *a, const double *b, const void *x, void *y,
                int R, void *stream) {
     clear_error();

@@ -133,6 +133,14 @@ int launch_zero(double2 *p, long cnt, hipStream_t s);
 int launch_dos_grad(const double *w, int M, int R, const double *e, int n_e, int kernel,
                     double2 *grad_mu, hipStream_t s);
 
+// ---- Peierls phases and value unpack (peierls.hip) ----
+// dst = a * src * exp(i (ax dx + ay dy)) per slot, (dx, dy) = (Re, Im) of the
+// bonds handle's values; src, bonds, dst share one slot pattern (dst may be src).
+int launch_peierls(const Ell &src, const Ell &bonds, Ell *dst, double2 a, double2 ax, double2 ay,
+                   hipStream_t s);
+// out[i*k + s] = value(i, s) / H.scale as complex128 (device, row-major).
+int launch_ell_unpack(const Ell &H, double *out, hipStream_t s);
+
 // ---- Chebyshev series / commutator recursion (series.hip) ----
 size_t series_workspace(const Ell &H, int R, bool comm);
 // out (if set) = sum_m c_m X_m and tab[m] (if set, commutator only) = c_m X_m,

@@ -272,3 +272,28 @@ def grad_values(M, data, indices, len_row, r, c, n_threads=None):
                                _p(cc), _p(out), threads(n_threads))
     assert rc == 0
     return out
+
+
+def peierls(data, dx, dy, a, ax, ay):
+    """a * data * exp(i (ax dx + ay dy)) slot by slot: the Peierls factor of
+    the reference's light modifiers (jclsquant/modifiers.py:133-181 over
+    vec_mul; GPU form jclsquant/cuda_cython/cuda/modifiers.cu:25-68 with
+    alpha = i ax, beta = i ay)."""
+    dx = np.real(np.asarray(dx))
+    dy = np.real(np.asarray(dy))
+    return a * np.asarray(data, dtype=np.complex128) * np.exp(1j * (ax * dx + ay * dy))
+
+
+def rho_neq(data, indices, len_row, dx, dy, t_vec, v, c_fd, c_u, phases, n_threads=None):
+    """(U, U_F) of the reference's time loop (jclsquant/cuda_cython/cuda/
+    obv.cu:262-361): U_F = sum_m c_fd[m] T_m(H(t_0)) v, then for every later
+    t_i: U_F <- sum_m c_u[m] T_m(H(t_i)) U_F and U <- the same of U, with
+    H(t) = peierls(data, dx, dy, 1, *phases(t)) (data already rescaled)."""
+    d0 = peierls(data, dx, dy, 1.0, *phases(t_vec[0]))
+    uf = rec_A_vec(len(c_fd), d0, indices, len_row, c_fd, v, n_threads)
+    u = np.array(v, dtype=np.complex128)
+    for t in t_vec[1:]:
+        d = peierls(data, dx, dy, 1.0, *phases(t))
+        uf = rec_A_vec(len(c_u), d, indices, len_row, c_u, uf, n_threads)
+        u = rec_A_vec(len(c_u), d, indices, len_row, c_u, u, n_threads)
+    return u, uf

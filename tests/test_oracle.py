@@ -366,3 +366,55 @@ def test_pins_reject_planted_mistakes():
     a = np.sin(np.pi * (np.arange(16) + 1) / 17)
     ac = np.array([np.dot(a[:16 - k], a[k:]) for k in range(16)]) / np.dot(a, a)
     assert np.max(np.abs(g_bad - ac)) > 0.1
+
+
+def _dense_of(data, indices, K, N):
+    A = np.zeros((N, N), dtype=np.complex128)
+    rows = np.repeat(np.arange(N), K)
+    np.add.at(A, (rows, np.asarray(indices)), np.asarray(data))
+    return A
+
+
+def test_rho_neq_vs_expm():
+    """The oracle's light-driven time loop against dense matrix functions:
+    U_F(t_0) = f(H(t_0)) v with f = sum_m c_fd[m] T_m (eigh), then every
+    time step multiplies by expm(-i H(t_i) dt) (Jacobi-Anger coefficients).
+    With real phases H(t) stays Hermitian, so the norm of U is conserved."""
+    import scipy.linalg as sla
+
+    from minijclsquant_amd.constants import hbar_fs
+
+    H, Dx, Dy = jcl.honeycomb_lattice(4, t=-2.7, W=1.0, seed=3)
+    A = jcl.ell_matrix(H, Dx, Dy)
+    N, K = A.shape[0], A.len_row
+    B = (3 * 2.7 + 0.5) * 1.01
+    data = A.data / B
+    rng = np.random.default_rng(4)
+    v = np.exp(2j * np.pi * rng.random(N))
+    t_vec = np.linspace(0.0, 3.0, 7)  # fs
+    dt_kpm = t_vec[1] * B / hbar_fs
+    c_u = jcl.moments_U(dt_kpm, 60)
+    M = 48
+    c_fd = jcl.moments_FD_T0(0.1, M) * jcl.JacksonKernel(M)
+
+    def phases(t):
+        return jcl.light_phases("circle_packed", (6.0, 0.9, "r", 1.5), t)
+
+    u, uf = O.rho_neq(data, A.indices, K, A.dx_vec, A.dy_vec, t_vec, v, c_fd, c_u, phases)
+    Hd = [_dense_of(O.peierls(data, A.dx_vec, A.dy_vec, 1.0, *phases(t)), A.indices, K, N)
+          for t in t_vec]
+    assert all(np.allclose(h, h.conj().T, atol=1e-15) for h in Hd)
+    lam, V = np.linalg.eigh(Hd[0])
+    uf_want = V @ (C.chebval(lam, c_fd) * (V.conj().T @ v))
+    u_want = v.copy()
+    for h in Hd[1:]:
+        P = sla.expm(-1j * dt_kpm * h)
+        uf_want = P @ uf_want
+        u_want = P @ u_want
+    assert np.max(np.abs(u - u_want)) <= 1e-10 * np.max(np.abs(u_want))
+    assert np.max(np.abs(uf - uf_want)) <= 1e-10 * np.max(np.abs(uf_want))
+    assert abs(np.linalg.norm(u) - np.linalg.norm(v)) <= 1e-10 * np.linalg.norm(v)
+    # the light actually acts: the field-free propagation differs
+    u0, _ = O.rho_neq(data, A.indices, K, A.dx_vec, A.dy_vec, t_vec, v, c_fd, c_u,
+                      lambda t: (0j, 0j))
+    assert np.max(np.abs(u0 - u)) > 1e-3 * np.max(np.abs(u))
