@@ -142,8 +142,17 @@ def plain_c128(torch, nat, _kpm, A, r, B, M, value, value_dbl, tune, reps=3):
     t1.synchronize()
     ms = t0.elapsed_time(t1) / reps
     v = N * M / (ms / 1e3)
+    # the reference-compatible host-pointer entry (rec_A_tab): upload of the
+    # matrix and vector over PCIe, the recursion, download of the moments
+    Ht = np.ascontiguousarray(A.data * (1.0 / B))
+    nat.rec_A_tab_host(Ht, A.indices, int(A.len_row), r[0], M)
+    t_h = time.perf_counter()
+    nat.rec_A_tab_host(Ht, A.indices, int(A.len_row), r[0], M)
+    host_s = time.perf_counter() - t_h
     return {
         "value": v,
+        "host_entry_value": N * M / host_s,
+        "host_entry_ms": host_s * 1e3,
         "unit": "N*M*R/s",
         "ms_per_recursion": ms,
         "sample": f"same H, R=1 vector, M={M} moments, complex128, {reps} recursions",
