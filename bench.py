@@ -118,6 +118,48 @@ def cpu_baseline(A, r, target_s=15.0):
     }
 
 
+def tiled_leg(torch, nat, h, vr, N, K, M, R, s_v, args, units, es):
+    """The same moments through the two-steps-per-launch path (rows
+    renumbered into LDS patches), plain and doubling forms, with the DOS; the
+    kernel's algorithmic bytes are those of two fused steps."""
+    plan = nat.TilePlan(h, R)
+    info = plan.info()
+    ws = torch.empty(plan.workspace_bytes(R, M, True), dtype=torch.uint8, device="cuda")
+    mu = torch.empty((M, R), dtype=torch.complex128, device="cuda")
+
+    def step(doubling):
+        plan.moments(vr, M, doubling=doubling, workspace=ws, out=mu)
+        return nat.dos(mu, es, nat.JKPM_KERNEL_JACKSON)
+
+    out = {"patches": info["npatch"], "halo_rows": info["halo_rows"], "max_ext": info["max_ext"]}
+    for doubling in (False, True):
+        step(doubling)
+        torch.cuda.synchronize()
+        nat.profile_read(reset=True)
+        nat.profile_enable(max(1, args.profile_every // 2))
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(doubling)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        nat.profile_enable(0)
+        ms_sum, n_prof = nat.profile_read(reset=True)
+        launch_ms = ms_sum / max(n_prof, 1)
+        nv = 4 if doubling else 5
+        bytes_launch = N * K * (s_v + 4) + R * N * nv * s_v
+        ach = bytes_launch / (launch_ms / 1e3) / 1e9
+        key = "doubling_" if doubling else ""
+        out[key + "value"] = units / (ms / 1e3)
+        out[key + "ms_per_step"] = ms
+        out[key + "roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                                 "algorithmic_bytes_per_launch": bytes_launch,
+                                 "mean_launch_us": round(launch_ms * 1e3, 2),
+                                 "launches_timed": n_prof}
+    del plan
+    return out
+
+
 def plain_c128(torch, nat, _kpm, A, r, B, M, value, value_dbl, tune, reps=3):
     """The reference's structure on this GPU: complex128 recursion, one
     vector at a time (R = 1), one moment per step, with its own tuned launch
@@ -362,6 +404,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dos-check", action="store_true")
     ap.add_argument("--no-doubling", action="store_true")
+    ap.add_argument("--no-tiles", action="store_true",
+                    help="skip the two-steps-per-launch (LDS patch) path")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain complex128 one-vector GPU recursion")
     ap.add_argument("--no-tune", action="store_true",
@@ -513,6 +557,12 @@ def main():
                                       / np.max(np.abs(d128)))),
         }
 
+    tiles = None
+    if not args.no_tiles:
+        tiles = tiled_leg(torch, nat, h, vr, N, K, M, R, s_v, args, units, es)
+        log(f"two steps per launch: {tiles['ms_per_step']:.2f} ms per step, doubling "
+            f"{tiles['doubling_ms_per_step']:.2f} ms")
+
     plain = None
     if not args.no_plain:
         plain = plain_c128(torch, nat, _kpm, A, r, B, M, units / (ms_per_step / 1e3),
@@ -550,6 +600,7 @@ def main():
         "cpu_baseline": cpu,
         "doubling": dbl,
         "plain_c128": plain,
+        "two_step_tiles": tiles,
     }
     print(json.dumps(out), flush=True)
 

@@ -221,6 +221,44 @@ int jkpm_moments_ex(const jkpm_ell *h, const jkpm_batch *b, const void *v_r, int
                     void *stream);
 
 /* ---------------------------------------------------------------------
+ * Two Chebyshev steps per launch over LDS patches (no reference
+ * counterpart: the reference makes one SpMV launch per step and gathers
+ * every T_n from device memory, jclsquant/cuda_cython/cuda/recursions.cu:
+ * 52-91).  The rows are renumbered into patches grown breadth-first on the
+ * matrix graph; one work-group keeps a patch's T_{n+1}, with its halo
+ * recomputed, in LDS and makes T_{n+2} from it, so a launch reads the
+ * matrix, T_{n-1}, T_n and v_l once for two steps.  Moments are unchanged by
+ * the renumbering.
+ * ------------------------------------------------------------------- */
+
+typedef struct jkpm_tiles jkpm_tiles;
+
+/* Builds the plan of handle h for up to R vectors per call: patches of at
+ * most max_rows rows of patch + halo (<= 0: as many as the LDS tile of
+ * min(R, 16) vectors holds).  Synchronous; h must outlive the plan.  The plan
+ * copies h's values: after jkpm_ell_set_values call jkpm_tiles_refresh. */
+int jkpm_tiles_create(const jkpm_ell *h, int R, int max_rows, jkpm_tiles **out);
+int jkpm_tiles_destroy(jkpm_tiles *t);
+int jkpm_tiles_info(const jkpm_tiles *t, int *npatch, long long *halo_rows, int *max_ext,
+                    int *cap);
+int jkpm_tiles_refresh(jkpm_tiles *t, void *stream);
+
+/* Host only (no device needed): builds the plan of a slot-major [K][N]
+ * column array for tiles of `cap` rows (patch + halo) and checks it fully
+ * (bijective renumbering, halos outside their patch, every patch-local
+ * column naming the right row); returns the patch count, total halo rows
+ * and the largest patch + halo. */
+int jkpm_tiles_plan_check(int n_rows, int len_row, const int *col_slot_major, int cap,
+                          int *npatch, long long *halo_rows, int *max_ext);
+
+/* mu[m][r] = <r|T_m(A)|r> (complex128 [M][R]) for v_r device [N][R] in the
+ * handle's value type, flags 0 or JKPM_MOMENTS_DOUBLING (Hermitian A).
+ * Asynchronous on stream; allocates nothing. */
+int jkpm_tiles_workspace(const jkpm_tiles *t, int R, int M, int flags, size_t *bytes);
+int jkpm_tiles_moments(const jkpm_tiles *t, const void *v_r, int R, int M, int flags,
+                       double *mu, void *workspace, size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------
  * HIP graphs: capture the launches of asynchronous entry points (moments,
  * batch moments, DOS, ...) issued on `stream` between begin and end, then
  * replay them with one launch.  Pointers and sizes are frozen at capture.

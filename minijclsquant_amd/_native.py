@@ -49,6 +49,13 @@ SIGNATURES = {
     "jkpm_ell_info": (_int, [_vp, _ip, _ip, _ip]),
     "jkpm_ell_set_values": (_int, [_vp, _vp, _dbl, _int, _vp]),
     "jkpm_ell_get_values": (_int, [_vp, _vp, _int, _vp]),
+    "jkpm_tiles_create": (_int, [_vp, _int, _int, _c.POINTER(_vp)]),
+    "jkpm_tiles_destroy": (_int, [_vp]),
+    "jkpm_tiles_plan_check": (_int, [_int, _int, _vp, _int, _ip, _c.POINTER(_c.c_longlong), _ip]),
+    "jkpm_tiles_info": (_int, [_vp, _ip, _c.POINTER(_c.c_longlong), _ip, _ip]),
+    "jkpm_tiles_refresh": (_int, [_vp, _vp]),
+    "jkpm_tiles_workspace": (_int, [_vp, _int, _int, _int, _c.POINTER(_size)]),
+    "jkpm_tiles_moments": (_int, [_vp, _vp, _int, _int, _int, _vp, _vp, _size, _vp]),
     "jkpm_ell_peierls": (_int, [_vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp]),
     "jkpm_aAxby": (_int, [_vp, _dp, _dp, _vp, _vp, _int, _vp]),
     "jkpm_moments_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
@@ -471,6 +478,77 @@ def dos(mu, energies_scaled, kernel: int = JKPM_KERNEL_JACKSON, stream=None):
     check(lib().jkpm_dos(mu.data_ptr(), M, R, e.data_ptr(), e.numel(), int(kernel),
                          out.data_ptr(), ws.data_ptr(), ws.numel() * 8, stream_ptr(stream)))
     return out
+
+
+def tiles_plan_check(indices, n_rows: int, len_row: int, cap: int):
+    """Host-only build + full check of a tile plan for a row-major ELL column
+    array; returns {npatch, halo_rows, max_ext}.  Raises ValueError when the
+    plan is inconsistent."""
+    idx = np.ascontiguousarray(np.asarray(indices, dtype=np.int32).reshape(n_rows, len_row).T)
+    npatch, mext = _c.c_int(), _c.c_int()
+    halo = _c.c_longlong()
+    check(lib().jkpm_tiles_plan_check(int(n_rows), int(len_row), idx.ctypes.data, int(cap),
+                                      _c.byref(npatch), _c.byref(halo), _c.byref(mext)))
+    return {"npatch": npatch.value, "halo_rows": halo.value, "max_ext": mext.value}
+
+
+class TilePlan:
+    """Rows of a DeviceELL renumbered into LDS patches for the two-steps-per-
+    launch kernel (jkpm_tiles_*).  Built for up to R vectors per call; the
+    handle must outlive the plan.  After the handle's values change, call
+    refresh()."""
+
+    def __init__(self, H: DeviceELL, R: int, max_rows: int = 0):
+        require_device()
+        self.H = H
+        self.R = int(R)
+        self._t = _vp()
+        check(lib().jkpm_tiles_create(H.handle, self.R, int(max_rows), _c.byref(self._t)))
+
+    def info(self):
+        npatch, mext, cap = _c.c_int(), _c.c_int(), _c.c_int()
+        halo = _c.c_longlong()
+        check(lib().jkpm_tiles_info(self._t, _c.byref(npatch), _c.byref(halo), _c.byref(mext),
+                                    _c.byref(cap)))
+        return {"npatch": npatch.value, "halo_rows": halo.value, "max_ext": mext.value,
+                "cap": cap.value}
+
+    def refresh(self, stream=None):
+        check(lib().jkpm_tiles_refresh(self._t, stream_ptr(stream)))
+
+    def workspace_bytes(self, R: int, M: int, doubling: bool = False) -> int:
+        out = _size()
+        check(lib().jkpm_tiles_workspace(self._t, int(R), int(M),
+                                         JKPM_MOMENTS_DOUBLING if doubling else 0,
+                                         _c.byref(out)))
+        return int(out.value)
+
+    def moments(self, v_r, M: int, doubling: bool = False, workspace=None, out=None,
+                stream=None):
+        """mu [M, R] complex128 of <r|T_m|r> (device)."""
+        torch = _torch()
+        R = self.H._check_vecs(v_r)
+        M = int(M)
+        if M <= 0:
+            raise ValueError("M must be positive")
+        need = self.workspace_bytes(R, M, doubling)
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty(need, dtype=torch.uint8, device=v_r.device)
+        if out is None:
+            out = torch.empty((M, R), dtype=torch.complex128, device=v_r.device)
+        elif tuple(out.shape) != (M, R) or out.dtype != torch.complex128 or not out.is_contiguous():
+            raise ValueError("out must be contiguous complex128 [M, R]")
+        check(lib().jkpm_tiles_moments(self._t, v_r.data_ptr(), R, M,
+                                       JKPM_MOMENTS_DOUBLING if doubling else 0, out.data_ptr(),
+                                       workspace.data_ptr(), workspace.numel(),
+                                       stream_ptr(stream)))
+        return out
+
+    def __del__(self):
+        t = getattr(self, "_t", None)
+        if t is not None and t.value and _lib is not None:
+            _lib.jkpm_tiles_destroy(t)
+            self._t = _vp()
 
 
 class Batch:

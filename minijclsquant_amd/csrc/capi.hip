@@ -17,13 +17,6 @@
 
 #include "common.h"
 
-struct jkpm_ell {
-    jkpm::Ell e;
-    // Column array ownership, shared by handles made with jkpm_ell_create_like
-    // (freed when the last handle that uses it is destroyed).
-    std::shared_ptr<int> col_owner;
-};
-
 namespace jkpm {
 
 namespace {
@@ -258,6 +251,123 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
         }
     }
     if (doubling) return launch_doubling_finish(target, G, target_M, R, M, mu, s);
+    return JKPM_OK;
+}
+
+// ---- two steps per launch over LDS patches ----
+
+struct TilesLayout {
+    size_t vp = 0, buf = 0, ring = 0, raw = 0, total = 0, vec = 0;
+};
+
+TilesLayout tiles_layout(const jkpm_tiles &T, int R, int M, bool doubling) {
+    TilesLayout L;
+    L.vec = align_up((size_t)T.n * R * cplx_bytes(T.vtype), 256);
+    L.vp = 0;
+    L.buf = L.vec;
+    L.ring = L.buf + 4 * L.vec;
+    const int n_left = doubling ? 2 : 1;
+    L.raw = L.ring + align_up((size_t)kRing * n_left * T.npatch * R * sizeof(double2), 256);
+    L.total = L.raw;
+    if (doubling) L.total += align_up(2 * (size_t)(doubling_steps(M) + 1) * R * sizeof(double2), 256);
+    return L;
+}
+
+// Moments of v_r (v_l = v_r) through the tile plan: the vectors are permuted
+// into plan order once, then every launch advances the recursion by two
+// steps.  Plain form: launch k makes T_{2k+1}, T_{2k+2} and mu of both (and
+// mu_0 on the first launch) into ring slots; doubling form: the raw dots
+// a_k, b_k of the same steps, mapped to mu_0 .. mu_{M-1} at the end.
+int run_tiles(const jkpm_tiles &T, const void *v_r, int R, int M, bool doubling, double2 *mu,
+              void *ws, size_t ws_bytes, hipStream_t s) {
+    const TilesLayout L = tiles_layout(T, R, M, doubling);
+    if (ws_bytes < L.total) {
+        set_error("workspace too small: need " + std::to_string(L.total) + " bytes, got " +
+                  std::to_string(ws_bytes));
+        return JKPM_ERR_WORKSPACE;
+    }
+    char *w = (char *)ws;
+    void *vp = w + L.vp;
+    void *B[4] = {w + L.buf, w + L.buf + L.vec, w + L.buf + 2 * L.vec, w + L.buf + 3 * L.vec};
+    double2 *ring = (double2 *)(w + L.ring);
+    const int n_left = doubling ? 2 : 1;
+    const size_t slot = (size_t)n_left * T.npatch * R;
+    int rc = launch_vec_permute(T.vtype, v_r, T.perm, T.n, R, vp, s);
+    if (rc) return rc;
+
+    std::vector<std::pair<int, int>> groups;  // (r0, rb), powers of two, largest first
+    for (int r0 = 0; r0 < R;) {
+        int rb = tile_max_group();
+        while (rb > R - r0) rb >>= 1;
+        groups.push_back({r0, rb});
+        r0 += rb;
+    }
+
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    JKPM_HIP(hipStreamIsCapturing(s, &cap));
+    Prof &P = prof();
+    std::unique_lock<std::mutex> plk(P.mu);
+    const int every = P.every;
+    const bool profiling = every > 0 && cap == hipStreamCaptureStatusNone;
+    if (!profiling) plk.unlock();
+
+    // Slots needed: plain mu_0 .. mu_{M-1}; doubling raw slots 0 .. S.
+    const int S = doubling_steps(M);
+    const int last_slot = doubling ? S : M - 1;
+    const int nlaunch = std::max(1, (last_slot + 1) / 2);  // launch k covers slots 2k+1, 2k+2
+    double2 *target = doubling ? (double2 *)(w + L.raw) : mu;
+    const int target_M = doubling ? S + 1 : M;
+    int pending_lo = 0;
+    for (int kq = 0; kq < nlaunch; ++kq) {
+        const int s1 = 2 * kq + 1, s2 = 2 * kq + 2;
+        TileArgs a;
+        a.T = &T;
+        a.first = kq == 0;
+        a.doubling = doubling;
+        a.xprev = kq == 0 ? nullptr : B[2 * ((kq - 1) % 2)];
+        a.xcur = kq == 0 ? vp : B[2 * ((kq - 1) % 2) + 1];
+        a.out1 = B[2 * (kq % 2)];
+        a.out2 = B[2 * (kq % 2) + 1];
+        a.vl = doubling ? nullptr : vp;
+        a.ld = R;
+        a.a1 = kq == 0 ? 1.0 : 2.0;
+        a.b1 = kq == 0 ? 0.0 : -1.0;
+        a.part0 = kq == 0 ? ring : nullptr;
+        a.part1 = ring + (size_t)(s1 % kRing) * slot;
+        a.part2 = ring + (size_t)(s2 % kRing) * slot;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool rec = profiling && (P.counter++ % every == 0);
+        if (rec) {
+            e0 = take_event(P);
+            e1 = take_event(P);
+            if (!e0 || !e1) return hip_fail(hipErrorOutOfMemory, "hipEventCreate", __FILE__, __LINE__);
+            JKPM_HIP(hipEventRecord(e0, s));
+        }
+        for (auto &g : groups) {
+            a.r0 = g.first;
+            a.rb = g.second;
+            rc = launch_tile2(a, s);
+            if (rc) return rc;
+        }
+        if (rec) {
+            JKPM_HIP(hipEventRecord(e1, s));
+            P.pending.push_back({e0, e1});
+            if (P.pending.size() >= 8192) {
+                rc = drain_profile(P);
+                if (rc) return rc;
+            }
+        }
+        // Fold the ring before the next launch could overwrite a pending slot.
+        const int last = std::min(s2, last_slot);
+        const bool final_launch = kq == nlaunch - 1;
+        if (final_launch || s2 + 2 - pending_lo + 1 > kRing) {
+            rc = launch_reduce(ring, kRing, pending_lo, last - pending_lo + 1, n_left, T.npatch,
+                               R, target, target_M, nullptr, 1, s);
+            if (rc) return rc;
+            pending_lo = s2 + 1;
+        }
+    }
+    if (doubling) return launch_doubling_finish(target, 1, target_M, R, M, mu, s);
     return JKPM_OK;
 }
 
@@ -1254,6 +1364,163 @@ int rec_A_tab_2(const double *data, const int *indices, int len_row, int N, doub
                 const double *v_l, const double *v_r, int M) {
     clear_error();
     return rec_host(data, indices, len_row, N, A_vec, v_l, v_r, M);
+}
+
+// ---- tile plans (two steps per launch) ----
+
+int jkpm_tiles_create(const jkpm_ell *h, int R, int max_rows, jkpm_tiles **out) {
+    clear_error();
+    if (!out) return arg_error("out is NULL");
+    *out = nullptr;
+    if (!h) return arg_error("handle is NULL");
+    if (R <= 0) return arg_error("R must be positive");
+    const Ell &H = h->e;
+    int rb = tile_max_group();
+    while (rb > R) rb >>= 1;
+    int cap = tile_cap_rows(H.vtype, rb);
+    if (max_rows > 0) cap = std::min(cap, max_rows);
+    std::vector<int> colh((size_t)H.n * H.k);
+    JKPM_HIP(hipMemcpy(colh.data(), H.col, colh.size() * sizeof(int), hipMemcpyDeviceToHost));
+    TilePlanHost plan;
+    int rc = build_tile_plan(H.n, H.k, colh, cap, &plan);
+    if (!rc) rc = validate_tile_plan(H.n, H.k, colh, cap, plan);
+    if (rc) return rc;
+    auto *t = new (std::nothrow) jkpm_tiles;
+    if (!t) return arg_error("host allocation failed");
+    t->h = h;
+    t->n = H.n;
+    t->k = H.k;
+    t->vtype = H.vtype;
+    t->rb = rb;
+    t->cap = cap;
+    t->npatch = plan.npatch;
+    t->max_ext = plan.max_ext;
+    t->halo_rows = (long long)plan.hrows.size();
+    auto fail = [&](hipError_t e, const char *what) {
+        jkpm_tiles_destroy(t);
+        return hip_fail(e, what, __FILE__, __LINE__);
+    };
+    const size_t nk = (size_t)H.n * H.k;
+    hipError_t e;
+    if ((e = hipMalloc(&t->perm, (size_t)H.n * sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&t->inv, (size_t)H.n * sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&t->val, nk * cplx_bytes(H.vtype))) != hipSuccess ||
+        (e = hipMalloc(&t->col, nk * sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&t->lcol, nk * sizeof(short))) != hipSuccess ||
+        (e = hipMalloc(&t->pstart, plan.pstart.size() * sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&t->hoff, plan.hoff.size() * sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&t->hrows, std::max<size_t>(1, plan.hrows.size()) * sizeof(int))) !=
+            hipSuccess ||
+        (e = hipMalloc(&t->hval, std::max<size_t>(1, plan.hrows.size() * H.k) *
+                                     cplx_bytes(H.vtype))) != hipSuccess ||
+        (e = hipMalloc(&t->hcol, std::max<size_t>(1, plan.hrows.size() * H.k) * sizeof(int))) !=
+            hipSuccess)
+        return fail(e, "hipMalloc(tile plan)");
+    if ((e = hipMemcpy(t->perm, plan.perm.data(), (size_t)H.n * sizeof(int),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(t->inv, plan.inv.data(), (size_t)H.n * sizeof(int),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(t->lcol, plan.lcol.data(), nk * sizeof(short), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(t->pstart, plan.pstart.data(), plan.pstart.size() * sizeof(int),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(t->hoff, plan.hoff.data(), plan.hoff.size() * sizeof(int),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (!plan.hrows.empty() &&
+         (e = hipMemcpy(t->hrows, plan.hrows.data(), plan.hrows.size() * sizeof(int),
+                        hipMemcpyHostToDevice)) != hipSuccess))
+        return fail(e, "hipMemcpy(tile plan)");
+    rc = launch_tile_permute(H, t->perm, t->inv, t->val, t->col, true, nullptr);
+    if (!rc) rc = launch_halo_gather(*t, true, nullptr);
+    if (!rc && (e = hipStreamSynchronize(nullptr)) != hipSuccess) return fail(e, "tile permute");
+    if (rc) {
+        jkpm_tiles_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return JKPM_OK;
+}
+
+int jkpm_tiles_plan_check(int n_rows, int len_row, const int *col_slot_major, int cap,
+                          int *npatch, long long *halo_rows, int *max_ext) {
+    clear_error();
+    if (!col_slot_major) return arg_error("columns are NULL");
+    if (n_rows <= 0 || len_row <= 0) return arg_error("n_rows and len_row must be positive");
+    if (cap > 32767) return arg_error("cap must fit the 16-bit patch-local columns");
+    std::vector<int> colh(col_slot_major, col_slot_major + (size_t)n_rows * len_row);
+    for (int j : colh)
+        if (j < 0 || j >= n_rows) return arg_error("column index out of range");
+    TilePlanHost plan;
+    int rc = build_tile_plan(n_rows, len_row, colh, cap, &plan);
+    if (!rc) rc = validate_tile_plan(n_rows, len_row, colh, cap, plan);
+    if (rc) return rc;
+    if (npatch) *npatch = plan.npatch;
+    if (halo_rows) *halo_rows = (long long)plan.hrows.size();
+    if (max_ext) *max_ext = plan.max_ext;
+    return JKPM_OK;
+}
+
+int jkpm_tiles_destroy(jkpm_tiles *t) {
+    clear_error();
+    if (!t) return JKPM_OK;
+    hipFree(t->perm);
+    hipFree(t->inv);
+    hipFree(t->val);
+    hipFree(t->col);
+    hipFree(t->lcol);
+    hipFree(t->pstart);
+    hipFree(t->hoff);
+    hipFree(t->hrows);
+    hipFree(t->hval);
+    hipFree(t->hcol);
+    delete t;
+    return JKPM_OK;
+}
+
+int jkpm_tiles_info(const jkpm_tiles *t, int *npatch, long long *halo_rows, int *max_ext,
+                    int *cap) {
+    clear_error();
+    if (!t) return arg_error("plan is NULL");
+    if (npatch) *npatch = t->npatch;
+    if (halo_rows) *halo_rows = t->halo_rows;
+    if (max_ext) *max_ext = t->max_ext;
+    if (cap) *cap = t->cap;
+    return JKPM_OK;
+}
+
+int jkpm_tiles_refresh(jkpm_tiles *t, void *stream) {
+    clear_error();
+    if (!t) return arg_error("plan is NULL");
+    int rc = launch_tile_permute(t->h->e, t->perm, t->inv, t->val, t->col, false,
+                                 (hipStream_t)stream);
+    if (rc) return rc;
+    return launch_halo_gather(*t, false, (hipStream_t)stream);
+}
+
+int jkpm_tiles_workspace(const jkpm_tiles *t, int R, int M, int flags, size_t *bytes) {
+    clear_error();
+    if (!t || !bytes) return arg_error("NULL argument");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    if (flags & ~JKPM_MOMENTS_DOUBLING) return arg_error("unknown flags");
+    *bytes = tiles_layout(*t, R, M, (flags & JKPM_MOMENTS_DOUBLING) != 0).total;
+    return JKPM_OK;
+}
+
+int jkpm_tiles_moments(const jkpm_tiles *t, const void *v_r, int R, int M, int flags,
+                       double *mu, void *workspace, size_t workspace_bytes, void *stream) {
+    clear_error();
+    if (!t) return arg_error("plan is NULL");
+    if (!v_r || !mu || !workspace) return arg_error("NULL vector/mu/workspace");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    if (flags & ~JKPM_MOMENTS_DOUBLING) return arg_error("unknown flags");
+    if (!aligned16(v_r) || !aligned16(workspace) || !aligned16(mu))
+        return arg_error("device pointers must be 16-byte aligned");
+    int rb = tile_max_group();
+    while (rb > R) rb >>= 1;
+    if (rb > t->rb && t->max_ext > tile_cap_rows(t->vtype, rb))
+        return arg_error("the plan was built for fewer vectors per group; rebuild it for this R");
+    return run_tiles(*t, v_r, R, M, (flags & JKPM_MOMENTS_DOUBLING) != 0, (double2 *)mu,
+                     workspace, workspace_bytes, (hipStream_t)stream);
 }
 
 }  // extern "C"

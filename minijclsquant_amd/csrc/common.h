@@ -5,9 +5,14 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <string>
 
+#include <vector>
+
 #include "../../include/jclkpm.h"
+
+struct jkpm_tiles;
 
 namespace jkpm {
 
@@ -141,6 +146,43 @@ int launch_peierls(const Ell &src, const Ell &bonds, Ell *dst, double2 a, double
 // out[i*k + s] = value(i, s) / H.scale as complex128 (device, row-major).
 int launch_ell_unpack(const Ell &H, double *out, hipStream_t s);
 
+// ---- two steps per launch over LDS patches (tiles.hip) ----
+struct TilePlanHost {
+    int npatch = 0, max_ext = 0;
+    std::vector<int> perm, inv, pstart, hoff, hrows;  // new -> old, old -> new, tables
+    std::vector<short> lcol;                          // [k][n] patch-local columns
+};
+// Rows of T_{n+1} the LDS tile holds for a vector group of rb (value type vtype).
+int tile_cap_rows(int vtype, int rb);
+int tile_max_group();
+// Patches of at most `cap` patch + halo rows, grown breadth-first; colh is the
+// slot-major [k][n] column array.
+int build_tile_plan(int n, int k, const std::vector<int> &colh, int cap, TilePlanHost *out);
+int validate_tile_plan(int n, int k, const std::vector<int> &colh, int cap,
+                       const TilePlanHost &plan);
+// Values (and columns, renumbered) of H in the plan's row order.
+int launch_tile_permute(const Ell &H, const int *perm, const int *inv, void *val_t, int *col_t,
+                        bool with_cols, hipStream_t s);
+// dst[i][:] = src[perm[i]][:] for R interleaved complex vectors of vtype.
+int launch_vec_permute(int vtype, const void *src, const int *perm, int n, int R, void *dst,
+                       hipStream_t s);
+struct TileArgs {
+    const jkpm_tiles *T = nullptr;
+    const void *xprev = nullptr;  // T_{n-1} (unused when first)
+    const void *xcur = nullptr;   // T_n
+    void *out1 = nullptr;         // T_{n+1}
+    void *out2 = nullptr;         // T_{n+2}
+    const void *vl = nullptr;     // left vectors (plain form)
+    int ld = 1, r0 = 0, rb = 1;
+    bool first = false, doubling = false;
+    double a1 = 2.0, b1 = -1.0;   // T_{n+1} = a1 A T_n + b1 T_{n-1}
+    double2 *part0 = nullptr, *part1 = nullptr, *part2 = nullptr;  // ring slots
+};
+int launch_tile2(const TileArgs &a, hipStream_t s);
+// Per-patch contiguous copies of the halo rows' matrix slots (after the
+// plan's values, and with_cols its columns, are in place).
+int launch_halo_gather(const jkpm_tiles &T, bool with_cols, hipStream_t s);
+
 // ---- Chebyshev series / commutator recursion (series.hip) ----
 size_t series_workspace(const Ell &H, int R, bool comm);
 // out (if set) = sum_m c_m X_m and tab[m] (if set, commutator only) = c_m X_m,
@@ -163,3 +205,26 @@ int launch_kubo_sigma(const double2 *mu, int M, const double *e, int n_e, int ki
                       double2 *sigma, hipStream_t s);
 
 }  // namespace jkpm
+
+// The opaque handle of the C-ABI: the device matrix plus the ownership of its
+// column array, shared by handles made with jkpm_ell_create_like (freed when
+// the last handle that uses it is destroyed).
+struct jkpm_ell {
+    jkpm::Ell e;
+    std::shared_ptr<int> col_owner;
+};
+
+// A tile plan of a handle: rows renumbered into patches, the matrix in that
+// order, and the patch tables (all device arrays owned by the plan).
+struct jkpm_tiles {
+    const jkpm_ell *h = nullptr;  // borrowed: must outlive the plan
+    int n = 0, k = 0, vtype = 0, rb = 1, cap = 0, npatch = 0, max_ext = 0;
+    long long halo_rows = 0;
+    int *perm = nullptr, *inv = nullptr;
+    void *val = nullptr;
+    int *col = nullptr;
+    short *lcol = nullptr;
+    int *pstart = nullptr, *hoff = nullptr, *hrows = nullptr;
+    void *hval = nullptr;  // [k][halo_rows] halo rows' values, patch by patch
+    int *hcol = nullptr;   // [k][halo_rows] their columns
+};

@@ -251,3 +251,38 @@ def test_coefficient_tables_reject_planted_mistakes():
     assert np.max(np.abs(C.chebval(x, bad) - np.exp(-1j * x * 3.0))) > 1e-2
     bad = np.conj(c)  # (+i)^n instead of (-i)^n
     assert np.max(np.abs(C.chebval(x, bad) - np.exp(-1j * x * 3.0))) > 1e-2
+
+
+@pytest.mark.parametrize("cap", [12, 64, 512])
+def test_tile_plan_host_check(cap):
+    """The patch renumbering of the two-steps-per-launch path, built and
+    fully checked on the host (bijection, halos outside their patch, every
+    patch-local column naming the right row) for lattices and a random
+    graph; halos stay a modest share of the rows on lattices."""
+    from minijclsquant_amd import _native as nat
+    from minijclsquant_amd.batch import square_lattice_ell
+
+    H, _, _ = jcl.honeycomb_lattice(24, t=-2.7, W=1.0, seed=1)
+    A = jcl.ell_matrix(H)
+    r = nat.tiles_plan_check(A.indices, A.shape[0], A.len_row, cap)
+    assert r["max_ext"] <= cap
+    if cap == 512:
+        assert r["halo_rows"] < 0.25 * A.shape[0]
+    d, ix = square_lattice_ell(20, -1.0, 1.0, 0)
+    r = nat.tiles_plan_check(ix, 400, 5, cap)
+    assert r["max_ext"] <= cap
+    # a random graph: large halos, still a consistent plan
+    rng = np.random.default_rng(cap)
+    n, k = 300, 4
+    idx = rng.integers(0, n, size=(n, k)).astype(np.int32)
+    r = nat.tiles_plan_check(idx.ravel(), n, k, cap)
+    assert r["max_ext"] <= cap and r["npatch"] >= 1
+
+
+def test_tile_plan_rejects_bad_input():
+    from minijclsquant_amd import _native as nat
+
+    with pytest.raises(ValueError):
+        nat.tiles_plan_check(np.array([0, 5], np.int32), 2, 1, 64)  # column out of range
+    with pytest.raises(ValueError):
+        nat.tiles_plan_check(np.zeros(8, np.int32), 2, 4, 6)  # cap below 2K+2
