@@ -1,7 +1,7 @@
 """Launch-shape autotuner with a persistent table.
 
-The step kernel has twelve variants (rows per lane x non-temporal streams x
-vectors per lane) and a work-group count per CU, set separately for the
+The step kernel has 24 variants (rows per lane x non-temporal loads x
+vectors per lane x non-temporal stores) and a work-group count per CU, set separately for the
 plain recursion and for the doubling form; jkpm_autotune times them on the
 caller's matrix and vectors.  Results are cached in a JSON table keyed by
 device, matrix shape (N, K), value type, batch size R and form, so a shape

@@ -142,22 +142,25 @@ __global__ void doubling_finish_kernel(const double2 *__restrict__ raw, int G, i
 // HBM latency).  Each work-group copies one contiguous tile per iteration.
 constexpr int kCopyUnroll = 4;
 
-__global__ __launch_bounds__(kBlock) void copy_kernel(const int4 *__restrict__ src,
-                                                       int4 *__restrict__ dst, long n16) {
+// Non-temporal loads and stores: the fastest streaming form measured on
+// MI355X (a copy that goes through the caches normally runs ~5 % slower).
+using i32x4 = int __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kBlock) void copy_kernel(const i32x4 *__restrict__ src,
+                                                       i32x4 *__restrict__ dst, long n16) {
     const long tile = (long)kBlock * kCopyUnroll;
     const long ntiles = (n16 + tile - 1) / tile;
     for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const long base = t * tile + threadIdx.x;
-        int4 v[kCopyUnroll];
+        i32x4 v[kCopyUnroll];
 #pragma unroll
         for (int u = 0; u < kCopyUnroll; ++u) {
             const long i = base + (long)u * kBlock;
-            if (i < n16) v[u] = src[i];
+            if (i < n16) v[u] = __builtin_nontemporal_load(src + i);
         }
 #pragma unroll
         for (int u = 0; u < kCopyUnroll; ++u) {
             const long i = base + (long)u * kBlock;
-            if (i < n16) dst[i] = v[u];
+            if (i < n16) __builtin_nontemporal_store(v[u], dst + i);
         }
     }
 }
@@ -275,10 +278,10 @@ int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
     int rc = device_info(&dev);
     if (rc) return rc;
     long grid = (n16 + (long)kBlock * kCopyUnroll - 1) / ((long)kBlock * kCopyUnroll);
-    const long cap = (long)dev.cu_count * 8;
+    const long cap = (long)dev.cu_count * 32;
     if (grid > cap) grid = cap;
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)grid), dim3(kBlock), 0, s,
-                       (const int4 *)src, (int4 *)dst, n16);
+                       (const i32x4 *)src, (i32x4 *)dst, n16);
     JKPM_CHECK_LAUNCH("copy_kernel");
     return JKPM_OK;
 }

@@ -7,24 +7,31 @@ namespace {
 
 using Real = float;
 
-template <int VPL, int LPR, int NL, int KT, int UNR, bool NT>
+template <int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool NS = false>
 const void *ptr() {
-    return reinterpret_cast<const void *>(&cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT>);
+    return reinterpret_cast<const void *>(
+        &cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT, false, NS>);
 }
 
-template <int VPL, int LPR, int KT, int UNR, bool NT>
+template <int VPL, int LPR, int KT, int UNR, bool NT, bool NS = false>
 const void *dbl_ptr() {
-    return reinterpret_cast<const void *>(&cheb_step_kernel<Real, VPL, LPR, 2, KT, UNR, NT, true>);
+    return reinterpret_cast<const void *>(
+        &cheb_step_kernel<Real, VPL, LPR, 2, KT, UNR, NT, true, NS>);
+}
+
+template <int VPL, int LPR, int KT, bool NS>
+const void *dbl_tuned_ns(int v) {
+    switch (v & 3) {
+        case 1: return dbl_ptr<VPL, LPR, KT, 2, false, NS>();
+        case 2: return dbl_ptr<VPL, LPR, KT, 1, true, NS>();
+        case 3: return dbl_ptr<VPL, LPR, KT, 2, true, NS>();
+        default: return dbl_ptr<VPL, LPR, KT, 1, false, NS>();
+    }
 }
 
 template <int VPL, int LPR, int KT>
 const void *dbl_tuned(int v) {
-    switch (v & 3) {
-        case 1: return dbl_ptr<VPL, LPR, KT, 2, false>();
-        case 2: return dbl_ptr<VPL, LPR, KT, 1, true>();
-        case 3: return dbl_ptr<VPL, LPR, KT, 2, true>();
-        default: return dbl_ptr<VPL, LPR, KT, 1, false>();
-    }
+    return v >= 12 ? dbl_tuned_ns<VPL, LPR, KT, true>(v) : dbl_tuned_ns<VPL, LPR, KT, false>(v);
 }
 
 // Doubling-form kernels: (unroll, non-temporal) variants for widths 4, 5.
@@ -36,14 +43,19 @@ StepEntry dbl_entry(int k, int v) {
     return {dbl_ptr<VPL, LPR, 0, 1, false>(), VPL, LPR, 1};
 }
 
+template <int VPL, int LPR, int KT, bool NS>
+const void *tuned_ns(int v) {
+    switch (v & 3) {
+        case 1: return ptr<VPL, LPR, 1, KT, 2, false, NS>();
+        case 2: return ptr<VPL, LPR, 1, KT, 1, true, NS>();
+        case 3: return ptr<VPL, LPR, 1, KT, 2, true, NS>();
+        default: return ptr<VPL, LPR, 1, KT, 1, false, NS>();
+    }
+}
+
 template <int VPL, int LPR, int KT>
 const void *tuned(int v) {
-    switch (v & 3) {
-        case 1: return ptr<VPL, LPR, 1, KT, 2, false>();
-        case 2: return ptr<VPL, LPR, 1, KT, 1, true>();
-        case 3: return ptr<VPL, LPR, 1, KT, 2, true>();
-        default: return ptr<VPL, LPR, 1, KT, 1, false>();
-    }
+    return v >= 12 ? tuned_ns<VPL, LPR, KT, true>(v) : tuned_ns<VPL, LPR, KT, false>(v);
 }
 
 // Base kernels of every (left sets, width) and the tuned ones for nl = 1,
@@ -72,7 +84,7 @@ StepEntry entry(int nl, int k, int v, bool dbl) {
 StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl) {
     if (wide) {
         // 16-byte lanes and up: VPL = 2 << shape (capped by the group size)
-        int shape = (variant >> 2) & 3;
+        int shape = ((variant % 12) >> 2) & 3;
         while (shape > 0 && (2 << shape) > rb) --shape;
         switch (rb) {
             case 2: return entry<2, 1, true>(nl, k, variant, dbl);

@@ -32,7 +32,9 @@ __device__ __forceinline__ V load_stream(const V *p) {
 // source); mu_{2n} = 2<T_n|T_n> - mu_0 and mu_{2n+1} = 2<T_n|T_{n+1}> - mu_1
 // then give two moments per step with no v_l stream.  The first step's
 // step-0 partials are <T_0|T_0>.
-template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool DBL = false>
+// NS: the T_{n+1} stream is written with non-temporal stores.
+template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool DBL = false,
+          bool NS = false>
 __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const Real *__restrict__ val, const int *__restrict__ col, int n, int k_rt,
     const Real *__restrict__ xcur, const Real *xprev, Real *xout,
@@ -178,7 +180,10 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
                     y[2 * v + 1] = Real(2) * si[u][v] - p[u][2 * v + 1];
                 }
             }
-            *reinterpret_cast<V *>(xout + vo) = y;
+            if constexpr (NS)
+                __builtin_nontemporal_store(y, reinterpret_cast<V *>(xout + vo));
+            else
+                *reinterpret_cast<V *>(xout + vo) = y;
             if constexpr (DBL) q[u][0] = y;
 #pragma unroll
             for (int l = 0; l < NL; ++l)
@@ -247,8 +252,9 @@ struct StepEntry {
     int vpl, lpr, unr;
 };
 
-// variant v = unroll | non-temporal << 1 | lane-shape << 2 (0..11); the
-// lane shape multiplies the vectors per lane (VPL) by 1, 2 or 4.  Variants
+// variant v = unroll | non-temporal << 1 | lane-shape << 2 (0..11), plus 12
+// for non-temporal stores of T_{n+1} (12..23); the lane shape multiplies the
+// vectors per lane (VPL) by 1, 2 or 4.  Variants
 // apply to single-left-set kernels of ELL width 4 and 5; every other shape
 // uses the base kernel.  Returns fn = nullptr for an unsupported group size.
 // dbl selects the doubling form (two partial sets, see DBL above).

@@ -15,6 +15,8 @@
 //            launch) in FP64.
 // Per two steps the matrix, T_n, T_{n-1} and v_l are read once (plus the
 // halo's share) instead of twice, and T_{n+1} is never gathered from HBM.
+// The doubling form loads T_{n-1} and stores both outputs non-temporally
+// (measured faster there; the plain form is faster with cached accesses).
 // Algorithmic bytes per launch (two steps): N K (s_v + 4) + R N 5 s_x
 // (T_{n-1}, T_n, v_l read; T_{n+1}, T_{n+2} written).
 //
@@ -69,6 +71,22 @@ __global__ void vec_permute_kernel(const Real *__restrict__ src, const int *__re
         dst[2 * e] = src[2 * o];
         dst[2 * e + 1] = src[2 * o + 1];
     }
+}
+
+template <bool NT, typename V, typename Real>
+__device__ __forceinline__ V load_in(const Real *p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+    else
+        return *reinterpret_cast<const V *>(p);
+}
+
+template <bool NT, typename V, typename Real>
+__device__ __forceinline__ void store_out(const V &y, Real *p) {
+    if constexpr (NT)
+        __builtin_nontemporal_store(y, reinterpret_cast<V *>(p));
+    else
+        *reinterpret_cast<V *>(p) = y;
 }
 
 // Accumulator slots: 0 = step 0 (first launch only), 1 = step n+1, 2 = step
@@ -156,7 +174,7 @@ __global__ __launch_bounds__(kTB) void tile2_kernel(
             for (int s = 0; s < KT; ++s)
                 x[s] = *reinterpret_cast<const V *>(xcur + (long)j[s] * ld * 2 + voff);
             V pv;
-            if (!first) pv = *reinterpret_cast<const V *>(xprev + g * ld * 2 + voff);
+            if (!first) pv = load_in<DBL, V>(xprev + g * ld * 2 + voff);
             if (own) tn[ip] = *reinterpret_cast<const V *>(xcur + g * ld * 2 + voff);
             C2 hc[KT];
 #pragma unroll
@@ -191,7 +209,7 @@ __global__ __launch_bounds__(kTB) void tile2_kernel(
                 y[2 * v + 1] = first ? a1 * si[v] : a1 * si[v] + b1 * pv[2 * v + 1];
             }
             *reinterpret_cast<V *>(tile + (long)e * ROWR + voff) = y;
-            if (own) *reinterpret_cast<V *>(out1 + g * ld * 2 + voff) = y;
+            if (own) store_out<DBL>(y, out1 + g * ld * 2 + voff);
         }
     } else {
 #pragma unroll
@@ -226,7 +244,7 @@ __global__ __launch_bounds__(kTB) void tile2_kernel(
                     y[2 * v + 1] = a1 * si[v];
                 }
             } else {
-                const V pv = *reinterpret_cast<const V *>(xprev + g * ld * 2 + voff);
+                const V pv = load_in<DBL, V>(xprev + g * ld * 2 + voff);
 #pragma unroll
                 for (int v = 0; v < VPL; ++v) {
                     y[2 * v] = a1 * sr[v] + b1 * pv[2 * v];
@@ -234,7 +252,7 @@ __global__ __launch_bounds__(kTB) void tile2_kernel(
                 }
             }
             *reinterpret_cast<V *>(tile + (long)e * ROWR + voff) = y;
-            if (own) *reinterpret_cast<V *>(out1 + g * ld * 2 + voff) = y;
+            if (own) store_out<DBL>(y, out1 + g * ld * 2 + voff);
         }
     }
     __syncthreads();
@@ -301,7 +319,7 @@ __global__ __launch_bounds__(kTB) void tile2_kernel(
             y[2 * v] = Real(2) * sr[v] - tn[ip][2 * v];
             y[2 * v + 1] = Real(2) * si[v] - tn[ip][2 * v + 1];
         }
-        *reinterpret_cast<V *>(out2 + vo) = y;
+        store_out<DBL>(y, out2 + vo);
         if constexpr (DBL) {
 #pragma unroll
             for (int v = 0; v < VPL; ++v) {
