@@ -15,14 +15,13 @@
 //   kubo_step_kernel   T_{n+1} = a H T_n + b T_{n-1} and, fused on the same
 //                      gathered T_n, W_n = v T_n (v shares H's slot pattern).
 //   kubo_gemm_kernel   split-K complex GEMM on v_mfma_f32_32x32x2f32: the
-//                      MFMA's K = 2 is one complex element (re, im); the real
-//                      part accumulates Lr*Rr + Li*Ri and the imaginary part
-//                      Lr*Ri - Li*Rr (B operand rotated by -i in registers).
+//                      MFMA's K = 2 spans two k; four MFMAs per k pair give
+//                      re += Lr*Rr + Li*Ri and im += Lr*Ri - Li*Rr.
 //                      Tiles of 128 x 128 complex outputs per work-group, 4
-//                      waves of 64 x 64; operands staged k-major in LDS as
-//                      separate re/im planes (conflict-free half-wave reads).
-//                      Work-groups of one K slice sit on one XCD so the slice
-//                      is fetched from HBM once per XCD L2.
+//                      waves of 64 x 64; each wave streams its operand rows
+//                      from global memory into MFMA registers (no LDS, no
+//                      barriers).  Work-groups of one K slice sit on one XCD
+//                      so the slice is fetched from HBM once per XCD L2.
 //   kubo_reduce_kernel FP64 sum of the FP32 slice partials in fixed order.
 //   kubo_sigma_kernel  sigma_i = sum_mn (a_m b_n + c_m d_n) mu_mn per energy.
 #include "common.h"
@@ -143,12 +142,25 @@ KStep pick(int rb, bool wide, bool dual) {
 
 constexpr int TM = 128;  // complex rows of L per work-group tile
 constexpr int TN = 128;  // complex rows of Rk per work-group tile
-constexpr int TK = 32;   // complex k per LDS stage
+constexpr int TK = 32;   // granularity of the split-K slices
 
-__global__ __launch_bounds__(kBlock) void kubo_gemm_kernel(
+constexpr int DK = 8;    // complex k per register batch (4 MFMA k-pairs)
+
+// No LDS and no barriers: every wave streams its own operand rows from
+// global memory (L2 serves the rows the wave pairs of a work-group share)
+// straight into MFMA operand registers, with the next batch in flight while
+// the current one is multiplied.  The MFMA's K = 2 spans two k: lane
+// (l32, half) supplies element k0 + half * DK/2 + q of its row for the q-th
+// k-pair, so each lane reads DK/2 consecutive complex numbers (two 16-byte
+// loads) per row and batch.  Four MFMAs per k-pair and 32 x 32 block give
+//   re += Lr Rr + Li Ri,   im += Lr Ri - Li Rr   (conj(L) Rk).
+// ALIGNED (K even): the row starts are 16-byte aligned and loads go two
+// complex numbers at a time.
+template <bool ALIGNED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void kubo_gemm_kernel(
     const float2 *__restrict__ L, const float2 *__restrict__ Rk, long K, int M, int ntm,
     int ntn, int nslices, long k_per_slice, float2 *__restrict__ part) {
-    __shared__ float Ar[TK][TM], Ai[TK][TM], Br[TK][TN], Bi[TK][TN];
+    constexpr int QP = DK / 2;  // k-pairs per batch
     const int ntiles = ntm * ntn;
     const int b = blockIdx.x;
     int slice, tile;
@@ -164,9 +176,21 @@ __global__ __launch_bounds__(kBlock) void kubo_gemm_kernel(
     const long k_begin = (long)slice * k_per_slice;
     const long k_end = min(K, k_begin + k_per_slice);
 
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;  // 2 x 2 waves of 64 x 64
     const int half = lane >> 5, l32 = lane & 31;
+
+    const float2 *pa[2], *pb[2];
+    bool va[2], vb[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int m = m0 + wm * 64 + i * 32 + l32;
+        va[i] = m < M;
+        pa[i] = L + (long)(va[i] ? m : 0) * K;
+        const int nn = n0 + wn * 64 + i * 32 + l32;
+        vb[i] = nn < M;
+        pb[i] = Rk + (long)(vb[i] ? nn : 0) * K;
+    }
 
     floatx16 cre[2][2], cim[2][2];
 #pragma unroll
@@ -176,58 +200,71 @@ __global__ __launch_bounds__(kBlock) void kubo_gemm_kernel(
 #pragma unroll
             for (int r = 0; r < 16; ++r) cre[i][j][r] = cim[i][j][r] = 0.f;
 
-    // Loader mapping: thread t covers row t / 2 of both tiles, 16 complex k.
-    // The next stage's global loads are issued before the current stage's
-    // MFMAs, so their latency hides behind the matrix work.
-    const int lrow = t >> 1, lk0 = (t & 1) * (TK / 2);
-    const int gm = m0 + lrow, gn = n0 + lrow;
-    float2 av[TK / 2], bv[TK / 2];
-    auto load = [&](long kc) {
+    // Three register batches in rotation: while batch t is multiplied,
+    // batches t+1 and t+2 are in flight.
+    float2 A0[2][QP], B0[2][QP], A1[2][QP], B1[2][QP], A2[2][QP], B2[2][QP];
+    auto load = [&](long kc, float2 (&a)[2][QP], float2 (&bb)[2][QP]) {
+        const long k0 = kc + half * QP;
 #pragma unroll
-        for (int q = 0; q < TK / 2; ++q) {
-            const long kk = kc + lk0 + q;
-            const bool kin = kk < k_end;
-            av[q] = (kin && gm < M) ? L[(long)gm * K + kk] : make_float2(0.f, 0.f);
-            bv[q] = (kin && gn < M) ? Rk[(long)gn * K + kk] : make_float2(0.f, 0.f);
+        for (int i = 0; i < 2; ++i) {
+            if constexpr (ALIGNED) {
+#pragma unroll
+                for (int q = 0; q < QP; q += 2) {
+                    const long kk = k0 + q;  // even, and k_end is even
+                    float4 x = make_float4(0.f, 0.f, 0.f, 0.f), y = x;
+                    if (kk < k_end) {
+                        if (va[i]) x = *reinterpret_cast<const float4 *>(pa[i] + kk);
+                        if (vb[i]) y = *reinterpret_cast<const float4 *>(pb[i] + kk);
+                    }
+                    a[i][q] = make_float2(x.x, x.y);
+                    a[i][q + 1] = make_float2(x.z, x.w);
+                    bb[i][q] = make_float2(y.x, y.y);
+                    bb[i][q + 1] = make_float2(y.z, y.w);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < QP; ++q) {
+                    const long kk = k0 + q;
+                    const bool in = kk < k_end;
+                    a[i][q] = (in && va[i]) ? pa[i][kk] : make_float2(0.f, 0.f);
+                    bb[i][q] = (in && vb[i]) ? pb[i][kk] : make_float2(0.f, 0.f);
+                }
+            }
         }
     };
-    if (k_begin < k_end) load(k_begin);
-    for (long kc = k_begin; kc < k_end; kc += TK) {
-        __syncthreads();  // previous stage fully consumed
+    // Eight independent accumulators between two updates of the same one.
+    auto mul = [&](const float2 (&a)[2][QP], const float2 (&bb)[2][QP]) {
 #pragma unroll
-        for (int q = 0; q < TK / 2; ++q) {
-            Ar[lk0 + q][lrow] = av[q].x;
-            Ai[lk0 + q][lrow] = av[q].y;
-            Br[lk0 + q][lrow] = bv[q].x;
-            Bi[lk0 + q][lrow] = bv[q].y;
-        }
-        __syncthreads();
-        if (kc + TK < k_end) load(kc + TK);
-#pragma unroll 4
-        for (int kk = 0; kk < TK; ++kk) {
-            float a[2], bre[2], bim[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int m = wm * 64 + i * 32 + l32;
-                a[i] = half == 0 ? Ar[kk][m] : Ai[kk][m];
-            }
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int nn = wn * 64 + j * 32 + l32;
-                const float br = Br[kk][nn], bi = Bi[kk][nn];
-                // lane half 0 holds the real component of the K = 2 pair, half 1
-                // the imaginary one: B = (Rr, Ri); rotated B' = (Ri, -Rr).
-                bre[j] = half == 0 ? br : bi;
-                bim[j] = half == 0 ? bi : -br;
-            }
+        for (int q = 0; q < QP; ++q) {
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    cre[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], bre[j], cre[i][j], 0, 0, 0);
-                    cim[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], bim[j], cim[i][j], 0, 0, 0);
+                    const float2 x = a[i][q], c = bb[j][q];
+                    cre[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.x, c.x, cre[i][j], 0, 0, 0);
+                    cim[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.x, c.y, cim[i][j], 0, 0, 0);
+                }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const float2 x = a[i][q], c = bb[j][q];
+                    cre[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.y, c.y, cre[i][j], 0, 0, 0);
+                    cim[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.y, -c.x, cim[i][j], 0, 0, 0);
                 }
         }
+    };
+    if (k_begin < k_end) load(k_begin, A0, B0);
+    if (k_begin + DK < k_end) load(k_begin + DK, A1, B1);
+    for (long kc = k_begin; kc < k_end; kc += 3 * DK) {
+        if (kc + 2 * DK < k_end) load(kc + 2 * DK, A2, B2);
+        mul(A0, B0);
+        if (kc + DK >= k_end) break;
+        if (kc + 3 * DK < k_end) load(kc + 3 * DK, A0, B0);
+        mul(A1, B1);
+        if (kc + 2 * DK >= k_end) break;
+        if (kc + 4 * DK < k_end) load(kc + 4 * DK, A1, B1);
+        mul(A2, B2);
     }
 
     // D[i][j] of a 32 x 32 block: lane l, register r -> j = l % 32,
@@ -370,8 +407,12 @@ int launch_kubo_gemm(const float2 *L, const float2 *Rk, long K, int M, int nslic
                      hipStream_t s) {
     const int ntm = (M + TM - 1) / TM, ntn = (M + TN - 1) / TN;
     const long kps = ((K + nslices - 1) / nslices + TK - 1) / TK * TK;
-    hipLaunchKernelGGL(kubo_gemm_kernel, dim3(ntm * ntn * nslices), dim3(kBlock), 0, s, L, Rk, K,
-                       M, ntm, ntn, nslices, kps, part);
+    if (K % 2 == 0)
+        hipLaunchKernelGGL(kubo_gemm_kernel<true>, dim3(ntm * ntn * nslices), dim3(kBlock), 0, s, L,
+                           Rk, K, M, ntm, ntn, nslices, kps, part);
+    else
+        hipLaunchKernelGGL(kubo_gemm_kernel<false>, dim3(ntm * ntn * nslices), dim3(kBlock), 0, s, L,
+                           Rk, K, M, ntm, ntn, nslices, kps, part);
     JKPM_CHECK_LAUNCH("kubo_gemm_kernel");
     return JKPM_OK;
 }

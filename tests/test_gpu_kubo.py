@@ -119,3 +119,21 @@ def test_kubo_argument_errors(torch_gpu):
                        np.array([1.5]), 0, B)
     out = nat.kubo_mu_nm(H64, V64, r + 1, 8)
     assert out.shape == (8, 8)
+
+
+@pytest.mark.parametrize("L,R,M", [(5, 1, 40), (7, 3, 33)])
+def test_mu_nm_odd_k(torch_gpu, L, R, M):
+    """K = N*R odd (unaligned operand rows) and M not a multiple of the tile:
+    square lattices with the v = I identity against the single moments."""
+    H, _ = jcl.square_lattice(L, W=1.0, seed=L)
+    A = jcl.ell_matrix(H)
+    B = 4.5 * 1.01
+    A.bounds = np.array([-B, B])
+    N = A.shape[0]
+    assert (N * R) % 2 == 1
+    r = vectors(N, R, 5)
+    mu2 = jcl.kpm_kubo_moments(A, M, r, v_data=identity_velocity(A)).cpu().numpy()
+    single = jcl.kpm_moments(A, 2 * M, r, dtype="c128").cpu().numpy().mean(axis=0)
+    m = np.arange(M)
+    want = 0.5 * (single[m[:, None] + m[None, :]] + single[np.abs(m[:, None] - m[None, :])])
+    assert np.max(np.abs(mu2 - want)) / N <= TOL_MU
