@@ -98,7 +98,10 @@ __global__ __launch_bounds__(kBlock) void plain_step_kernel(
     }
 }
 
-template <typename Real, int VPL, int LPR>
+// KT > 0 (ELL width KT): per row, each lambda_k row is read once and the KT
+// gathered T rows of the same k are folded into KT register accumulators;
+// KT = 0: any width, one slot at a time.
+template <typename Real, int VPL, int LPR, int KT>
 __global__ __launch_bounds__(kBlock) void grad_fold_kernel(
     const int *__restrict__ col, int n, int k, const Real *__restrict__ lam,
     const Real *__restrict__ tprev, long vstride, int count, int k_lo, double2 *__restrict__ G,
@@ -115,42 +118,88 @@ __global__ __launch_bounds__(kBlock) void grad_fold_kernel(
     // butterfly below always has all lanes present.
     const long span = row_end > row_begin ? row_end - row_begin : 0;
     const long iters = (span + kWaves * RPW - 1) / (kWaves * RPW);
+    // lambda * conj(T), summed over this lane's vectors
+    auto fold = [](const V &a, const V &t, double &pr, double &pi) {
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            const double a_r = a[2 * v], a_i = a[2 * v + 1];
+            const double t_r = t[2 * v], t_i = t[2 * v + 1];
+            pr += a_r * t_r + a_i * t_i;
+            pi += a_i * t_r - a_r * t_i;
+        }
+    };
     for (long it = 0; it < iters; ++it) {
         const long row = row_begin + it * kWaves * RPW + wave * RPW + lr;
         const bool live = row < row_end;
-        for (int s = 0; s < k; ++s) {
-            double ar = 0.0, ai = 0.0;
+        if constexpr (KT > 0) {
+            double ar[KT], ai[KT];
+            int j[KT];
+#pragma unroll
+            for (int s = 0; s < KT; ++s) {
+                ar[s] = ai[s] = 0.0;
+                j[s] = live ? col[(long)s * n + row] : 0;
+            }
             if (live) {
-                const int j = col[(long)s * n + row];
                 for (int q = 0; q < count; ++q) {
                     const double f = (k_lo + q == 1) ? 1.0 : 2.0;
                     const V lv_ = *reinterpret_cast<const V *>(lam + q * vstride * 2 +
                                                                row * ld * 2 + voff);
-                    const V tv = *reinterpret_cast<const V *>(tprev + q * vstride * 2 +
-                                                              (long)j * ld * 2 + voff);
-                    double pr = 0.0, pi = 0.0;
+                    V tv[KT];
 #pragma unroll
-                    for (int v = 0; v < VPL; ++v) {
-                        const double a_r = lv_[2 * v], a_i = lv_[2 * v + 1];
-                        const double t_r = tv[2 * v], t_i = tv[2 * v + 1];
-                        // lambda * conj(T)
-                        pr += a_r * t_r + a_i * t_i;
-                        pi += a_i * t_r - a_r * t_i;
+                    for (int s = 0; s < KT; ++s)
+                        tv[s] = *reinterpret_cast<const V *>(tprev + q * vstride * 2 +
+                                                             (long)j[s] * ld * 2 + voff);
+#pragma unroll
+                    for (int s = 0; s < KT; ++s) {
+                        double pr = 0.0, pi = 0.0;
+                        fold(lv_, tv[s], pr, pi);
+                        ar[s] += f * pr;
+                        ai[s] += f * pi;
                     }
-                    ar += f * pr;
-                    ai += f * pi;
                 }
             }
 #pragma unroll
-            for (int o = 1; o < LPR; o <<= 1) {
-                ar += __shfl_xor(ar, o);
-                ai += __shfl_xor(ai, o);
+            for (int s = 0; s < KT; ++s) {
+#pragma unroll
+                for (int o = 1; o < LPR; o <<= 1) {
+                    ar[s] += __shfl_xor(ar[s], o);
+                    ai[s] += __shfl_xor(ai[s], o);
+                }
+                if (live && lv == 0) {
+                    double2 g = G[(long)s * n + row];
+                    g.x += ar[s];
+                    g.y += ai[s];
+                    G[(long)s * n + row] = g;
+                }
             }
-            if (live && lv == 0) {
-                double2 g = G[(long)s * n + row];
-                g.x += ar;
-                g.y += ai;
-                G[(long)s * n + row] = g;
+        } else {
+            for (int s = 0; s < k; ++s) {
+                double ar = 0.0, ai = 0.0;
+                if (live) {
+                    const int j = col[(long)s * n + row];
+                    for (int q = 0; q < count; ++q) {
+                        const double f = (k_lo + q == 1) ? 1.0 : 2.0;
+                        const V lv_ = *reinterpret_cast<const V *>(lam + q * vstride * 2 +
+                                                                   row * ld * 2 + voff);
+                        const V tv = *reinterpret_cast<const V *>(tprev + q * vstride * 2 +
+                                                                  (long)j * ld * 2 + voff);
+                        double pr = 0.0, pi = 0.0;
+                        fold(lv_, tv, pr, pi);
+                        ar += f * pr;
+                        ai += f * pi;
+                    }
+                }
+#pragma unroll
+                for (int o = 1; o < LPR; o <<= 1) {
+                    ar += __shfl_xor(ar, o);
+                    ai += __shfl_xor(ai, o);
+                }
+                if (live && lv == 0) {
+                    double2 g = G[(long)s * n + row];
+                    g.x += ar;
+                    g.y += ai;
+                    G[(long)s * n + row] = g;
+                }
             }
         }
     }
@@ -173,7 +222,13 @@ __global__ void zero_kernel(double2 *__restrict__ p, long cnt) {
 template <typename Real, int VPL, int LPR>
 struct Fns {
     static const void *step() { return reinterpret_cast<const void *>(&plain_step_kernel<Real, VPL, LPR>); }
-    static const void *fold() { return reinterpret_cast<const void *>(&grad_fold_kernel<Real, VPL, LPR>); }
+    static const void *fold(int k) {
+        switch (k) {
+            case 4: return reinterpret_cast<const void *>(&grad_fold_kernel<Real, VPL, LPR, 4>);
+            case 5: return reinterpret_cast<const void *>(&grad_fold_kernel<Real, VPL, LPR, 5>);
+            default: return reinterpret_cast<const void *>(&grad_fold_kernel<Real, VPL, LPR, 0>);
+        }
+    }
 };
 
 struct Pick {
@@ -182,20 +237,20 @@ struct Pick {
 };
 
 template <typename Real>
-Pick pick_real(int rb) {
+Pick pick_real(int rb, int k) {
     switch (rb) {
-        case 1: return {Fns<Real, 1, 1>::step(), Fns<Real, 1, 1>::fold(), 1};
-        case 2: return {Fns<Real, 1, 2>::step(), Fns<Real, 1, 2>::fold(), 2};
-        case 4: return {Fns<Real, 1, 4>::step(), Fns<Real, 1, 4>::fold(), 4};
-        case 8: return {Fns<Real, 1, 8>::step(), Fns<Real, 1, 8>::fold(), 8};
-        case 16: return {Fns<Real, 1, 16>::step(), Fns<Real, 1, 16>::fold(), 16};
-        case 32: return {Fns<Real, 1, 32>::step(), Fns<Real, 1, 32>::fold(), 32};
+        case 1: return {Fns<Real, 1, 1>::step(), Fns<Real, 1, 1>::fold(k), 1};
+        case 2: return {Fns<Real, 1, 2>::step(), Fns<Real, 1, 2>::fold(k), 2};
+        case 4: return {Fns<Real, 1, 4>::step(), Fns<Real, 1, 4>::fold(k), 4};
+        case 8: return {Fns<Real, 1, 8>::step(), Fns<Real, 1, 8>::fold(k), 8};
+        case 16: return {Fns<Real, 1, 16>::step(), Fns<Real, 1, 16>::fold(k), 16};
+        case 32: return {Fns<Real, 1, 32>::step(), Fns<Real, 1, 32>::fold(k), 32};
         default: return {nullptr, nullptr, 0};
     }
 }
 
-Pick pick(int vtype, int rb) {
-    return vtype == JKPM_C64 ? pick_real<float>(rb) : pick_real<double>(rb);
+Pick pick(int vtype, int rb, int k) {
+    return vtype == JKPM_C64 ? pick_real<float>(rb, k) : pick_real<double>(rb, k);
 }
 
 long rows_per_chunk(int n, int nchunks, int lpr) {
@@ -213,7 +268,7 @@ int launch_plain_step(const Ell &H, const void *x, const void *prev, void *out,
     for (int r0 = 0; r0 < R;) {
         int rb = 32;
         while (rb > R - r0) rb >>= 1;
-        Pick p = pick(H.vtype, rb);
+        Pick p = pick(H.vtype, rb, H.k);
         if (!p.step) return arg_error("unsupported vector group");
         const size_t off = (size_t)r0 * cb;
         const void *a_val = H.val;
@@ -246,7 +301,7 @@ int launch_grad_fold(const Ell &H, const void *lam, const void *tprev, int count
     for (int r0 = 0; r0 < R;) {
         int rb = 32;
         while (rb > R - r0) rb >>= 1;
-        Pick p = pick(H.vtype, rb);
+        Pick p = pick(H.vtype, rb, H.k);
         if (!p.fold) return arg_error("unsupported vector group");
         const size_t off = (size_t)r0 * cb;
         const int *a_col = H.col;
