@@ -224,6 +224,51 @@ def base_line(metric, value, unit, args, ms, dtype, config, higher=True):
             "config": config}
 
 
+def ingest_leg(blocks, torch, nat):
+    """One-time cost of getting the realisations onto the device, two ways:
+    host CSR -> row-major ELL per realisation (what ell_matrix does,
+    jclsquant/ell_matrix.py:187-200), block stacking and upload; or stacking
+    the CSR arrays and converting on the device (jkpm_ell_create_csr).  Both
+    handles must hold identical columns and values."""
+    import scipy.sparse as sp
+
+    from minijclsquant_amd.batch import block_diagonal, stack_csr
+    from minijclsquant_amd.ell_matrix import data_indices_cython_0
+
+    mats = [sp.csr_matrix((d, ix, np.arange(int(n) + 1, dtype=np.int64) * int(k)),
+                          shape=(int(n), int(n))) for d, ix, n, k, _ in blocks]
+    scales = [b[4] for b in blocks]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ell_blocks = []
+    for m, s in zip(mats, scales):
+        d, ix, _ = data_indices_cython_0(5, m.shape[0], m.indptr, m.data, m.indices)
+        ell_blocks.append((d, ix, m.shape[0], 5, s))
+    data, idx, offsets, K = block_diagonal(ell_blocks)
+    h_host = nat.DeviceELL(data, idx, int(offsets[-1]), K, 1.0, "c64")
+    torch.cuda.synchronize()
+    t_host = time.perf_counter() - t0
+    del ell_blocks, data, idx
+    t0 = time.perf_counter()
+    indptr, indices, cdata, offs = stack_csr(mats, scales)
+    h_dev = nat.DeviceELL.from_csr(indptr, indices, cdata, int(offs[-1]), 1.0, "c64")
+    torch.cuda.synchronize()
+    t_dev = time.perf_counter() - t0
+    del indptr, indices, cdata
+    same = bool(torch.equal(h_host.indices(), h_dev.indices()) and
+                torch.equal(h_host.values(), h_dev.values()))
+    if not same:
+        raise RuntimeError("device CSR ingest differs from the host ELL build")
+    del h_host, h_dev
+    torch.cuda.synchronize()
+    log(f"ingest: host CSR->ELL + upload {t_host:.2f} s, device CSR ingest {t_dev:.2f} s")
+    return {"host_csr_to_ell_upload_s": round(t_host, 3),
+            "device_csr_ingest_s": round(t_dev, 3), "identical": same,
+            "note": "one-time, outside the timed region: row-major ELL per realisation on "
+                    "the host + stacking + upload, against stacked CSR converted on the "
+                    "device"}
+
+
 def bench_batch(args, torch, jcl, nat, _kpm):
     """64 square-lattice realisations, L ~ U{128..1024}, W ~ U[0.5, 3],
     t = -1, K = 5, R = 8, M = 1024, complex64, one block-diagonal batch:
@@ -242,6 +287,7 @@ def bench_batch(args, torch, jcl, nat, _kpm):
         blocks.append((d, ix, L * L, K, 1.0 / ((4 + Ws[g] / 2) * 1.01)))
         vecs.append(np.stack([jcl.random_vector_generator(L * L, 100 * g + s)
                               for s in range(R)]))
+    ingest = ingest_leg(blocks, torch, nat)
     bk = BatchKPM(blocks, vecs, M, "c64", tune=not args.no_tune)
     del blocks, vecs
     N = int(bk.offsets[-1])
@@ -283,6 +329,7 @@ def bench_batch(args, torch, jcl, nat, _kpm):
     out["doubling"] = {"value": units / (ms_d / 1e3), "ms_per_step": ms_d,
                        "frac": round(ach_d / HBM_PEAK_GBS, 4),
                        "max_rel_moment_diff_vs_plain": diff}
+    out["ingest"] = ingest
     print(json.dumps(out), flush=True)
 
 

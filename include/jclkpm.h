@@ -77,6 +77,28 @@ int jkpm_ell_create(const double *data, const int *indices, int n_rows, int len_
                     double scale, int value_type, int src_on_device, void *stream,
                     jkpm_ell **out);
 
+/* Device-side ingest of a CSR matrix (indptr int32 [n_rows+1], indices
+ * int32 [nnz], data complex128 [nnz], nnz = indptr[n_rows]; host or device
+ * pointers, chosen by src_on_device).  Row i's entries go, in CSR order, to
+ * slots 0..len_i-1; slots len_i..K-1 repeat the row's last column with value
+ * 0, as jclsquant/ell_matrix.py:187-200 over data_indices_c_0
+ * (jclsquant/cython_modules/Extra_funs.pyx:182-219) does on the host (an
+ * empty row pads with its own index).  K = len_row, or the longest row when
+ * len_row is 0 (1 when every row is empty); a row longer than len_row, an
+ * indptr that decreases or leaves [0, nnz], or a column outside [0, n_rows)
+ * is an error.  Values are multiplied by `scale` in complex128 and cast to
+ * value_type, as in jkpm_ell_create.  The host-side row-major ELL is never
+ * built.  Device arrays must hold n_rows+1 and nnz entries (only indptr's
+ * last entry is read before the scan).  Synchronises `stream`. */
+int jkpm_ell_create_csr(const int *indptr, const int *indices, const double *data, int n_rows,
+                        int len_row, double scale, int value_type, int src_on_device,
+                        void *stream, jkpm_ell **out);
+
+/* Row-major int32 copy of the column indices (ell_matrix.indices,
+ * jclsquant/ell_matrix.py:119-258) to host or device memory
+ * (dst_on_device).  Synchronises `stream`. */
+int jkpm_ell_get_indices(const jkpm_ell *h, int *indices, int dst_on_device, void *stream);
+
 int jkpm_ell_destroy(jkpm_ell *h);
 
 int jkpm_ell_info(const jkpm_ell *h, int *n_rows, int *len_row, int *value_type);

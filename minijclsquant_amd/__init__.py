@@ -7,6 +7,7 @@ GPU-named) and lattice builders.  The compute path is the HIP library
 lib/libjclkpm_hip.so (C-ABI in include/jclkpm.h), called through ctypes.
 """
 from .ell_matrix import *  # noqa: F401,F403
+from .hams import *  # noqa: F401,F403
 from .kernel_and_moments import *  # noqa: F401,F403
 from .kubo import *  # noqa: F401,F403
 from .lattice import *  # noqa: F401,F403

@@ -49,6 +49,9 @@ SIGNATURES = {
     "jkpm_ell_info": (_int, [_vp, _ip, _ip, _ip]),
     "jkpm_ell_set_values": (_int, [_vp, _vp, _dbl, _int, _vp]),
     "jkpm_ell_get_values": (_int, [_vp, _vp, _int, _vp]),
+    "jkpm_ell_create_csr": (_int, [_vp, _vp, _vp, _int, _int, _dbl, _int, _int, _vp,
+                                   _c.POINTER(_vp)]),
+    "jkpm_ell_get_indices": (_int, [_vp, _vp, _int, _vp]),
     "jkpm_tiles_create": (_int, [_vp, _int, _int, _c.POINTER(_vp)]),
     "jkpm_tiles_destroy": (_int, [_vp]),
     "jkpm_tiles_plan_check": (_int, [_int, _int, _vp, _int, _ip, _c.POINTER(_c.c_longlong), _ip]),
@@ -300,6 +303,52 @@ class DeviceELL:
                                          stream_ptr(stream), _c.byref(self._h)))
         return self
 
+    @classmethod
+    def from_csr(cls, indptr, indices, data, n_rows: int, scale: float = 1.0, dtype="c128",
+                 len_row: int = 0, stream=None):
+        """Device-side CSR -> ELL ingest (jkpm_ell_create_csr): the row-major
+        host ELL of ell_matrix.csr_to_ell (jclsquant/ell_matrix.py:187-200) is
+        never built.  indptr int32 [n_rows+1], indices int32 [nnz], data
+        complex128 [nnz]: numpy arrays (uploaded) or contiguous torch device
+        tensors.  len_row 0 takes the longest row."""
+        torch = require_device()
+        self = cls.__new__(cls)
+        self.n = int(n_rows)
+        self.vtype = vtype_of(dtype)
+        self.scale = float(scale)
+        self._h = _vp()
+        on_dev = isinstance(indptr, torch.Tensor)
+        if on_dev:
+            for t, dt in ((indptr, torch.int32), (indices, torch.int32), (data, torch.complex128)):
+                if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != dt:
+                    raise ValueError("device CSR input must be int32 indptr/indices and "
+                                     "complex128 data, all device tensors")
+            keep = (indptr.contiguous(), indices.contiguous(), data.contiguous())
+            if keep[0].numel() != self.n + 1:
+                raise ValueError("indptr must hold n_rows+1 entries")
+            nnz = int(keep[0][-1].item())
+            if keep[1].numel() < nnz or keep[2].numel() < nnz:
+                raise ValueError("indices/data must hold indptr[n_rows] entries")
+            ptrs = [t.data_ptr() for t in keep]
+        else:
+            keep = (np.ascontiguousarray(indptr, dtype=np.int32),
+                    np.ascontiguousarray(indices, dtype=np.int32),
+                    np.ascontiguousarray(data, dtype=np.complex128))
+            if keep[0].size != self.n + 1:
+                raise ValueError("indptr must hold n_rows+1 entries")
+            nnz = int(keep[0][-1])
+            if keep[1].size < nnz or keep[2].size < nnz:
+                raise ValueError("indices/data must hold indptr[n_rows] entries")
+            ptrs = [t.ctypes.data for t in keep]
+        check(lib().jkpm_ell_create_csr(ptrs[0], ptrs[1], ptrs[2], self.n, int(len_row),
+                                        self.scale, self.vtype, 1 if on_dev else 0,
+                                        stream_ptr(stream), _c.byref(self._h)))
+        del keep
+        n, k, vt = _c.c_int(), _c.c_int(), _c.c_int()
+        check(lib().jkpm_ell_info(self._h, _c.byref(n), _c.byref(k), _c.byref(vt)))
+        self.k = int(k.value)
+        return self
+
     @property
     def handle(self):
         return self._h
@@ -392,6 +441,13 @@ class DeviceELL:
         torch = _torch()
         out = torch.empty(self.n * self.k, dtype=torch.complex128, device="cuda")
         check(lib().jkpm_ell_get_values(self._h, out.data_ptr(), 1, stream_ptr(stream)))
+        return out
+
+    def indices(self, stream=None):
+        """Row-major int32 column indices [n*k] (device tensor)."""
+        torch = _torch()
+        out = torch.empty(self.n * self.k, dtype=torch.int32, device="cuda")
+        check(lib().jkpm_ell_get_indices(self._h, out.data_ptr(), 1, stream_ptr(stream)))
         return out
 
     def peierls(self, src: "DeviceELL", bonds: "DeviceELL", a=1.0, ax=0.0, ay=0.0,

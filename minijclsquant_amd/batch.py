@@ -19,7 +19,7 @@ import numpy as np
 from . import _kpm
 from . import _native as nat
 
-__all__ = ["square_lattice_ell", "block_diagonal", "BatchKPM", "kpm_dos_batch"]
+__all__ = ["square_lattice_ell", "block_diagonal", "stack_csr", "BatchKPM", "kpm_dos_batch"]
 
 
 def square_lattice_ell(L, t=-1.0, W=0.0, seed=None):
@@ -67,6 +67,28 @@ def block_diagonal(blocks):
     return data, idx, offsets, K
 
 
+def stack_csr(mats, scales):
+    """Block-diagonal CSR of square CSR blocks, block g's values times
+    scales[g]: (indptr int32, indices int32, data complex128, row offsets)."""
+    sizes = [int(m.shape[0]) for m in mats]
+    offsets = np.zeros(len(mats) + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum(sizes)
+    nnz = np.zeros(len(mats) + 1, dtype=np.int64)
+    nnz[1:] = np.cumsum([int(m.indptr[-1]) for m in mats])
+    if offsets[-1] >= 2**31 or nnz[-1] >= 2**31:
+        raise ValueError("the stacked CSR exceeds 2^31-1 rows or entries")
+    indptr = np.empty(int(offsets[-1]) + 1, dtype=np.int32)
+    indices = np.empty(int(nnz[-1]), dtype=np.int32)
+    data = np.empty(int(nnz[-1]), dtype=np.complex128)
+    indptr[0] = 0
+    for g, (m, s) in enumerate(zip(mats, scales)):
+        r0, r1, e0, e1 = int(offsets[g]), int(offsets[g + 1]), int(nnz[g]), int(nnz[g + 1])
+        indptr[r0 + 1:r1 + 1] = np.asarray(m.indptr[1:], dtype=np.int64) + e0
+        indices[e0:e1] = np.asarray(m.indices[:e1 - e0], dtype=np.int64) + r0
+        np.multiply(np.asarray(m.data[:e1 - e0], dtype=np.complex128), s, out=data[e0:e1])
+    return indptr, indices, data, offsets
+
+
 class BatchKPM:
     """Device state for the moments of G realizations x R vectors.
 
@@ -77,14 +99,36 @@ class BatchKPM:
     autotuner before the batch's work-group chunks are laid out."""
 
     def __init__(self, blocks, vectors, M, dtype="c64", doubling=False, tune=False):
-        torch = nat.require_device()
+        nat.require_device()
         data, idx, offsets, K = block_diagonal(blocks)
+        H = nat.DeviceELL(data, idx, int(offsets[-1]), K, 1.0, dtype)
+        del data, idx
+        self._setup(H, offsets, vectors, [int(b[2]) for b in blocks], M, dtype, doubling, tune)
+
+    @classmethod
+    def from_csr(cls, mats, scales, vectors, M, dtype="c64", doubling=False, tune=False):
+        """The batch from scipy CSR realizations (square, each with its scale
+        1/B_g): their CSR arrays are stacked block-diagonally on the host and
+        converted to ELL on the device (jkpm_ell_create_csr), so no host ELL
+        is built.  The ELL width is the longest row of any realization, and
+        the slots follow CSR order with the reference padding, as
+        ``BatchKPM(blocks, ...)`` over ``ell_matrix`` blocks."""
+        nat.require_device()
+        indptr, indices, data, offsets = stack_csr(mats, scales)
+        H = nat.DeviceELL.from_csr(indptr, indices, data, int(offsets[-1]), 1.0, dtype)
+        del indptr, indices, data
+        self = cls.__new__(cls)
+        self._setup(H, offsets, vectors, [int(m.shape[0]) for m in mats], M, dtype, doubling,
+                    tune)
+        return self
+
+    def _setup(self, H, offsets, vectors, sizes, M, dtype, doubling, tune):
+        torch = nat.require_device()
         self.offsets = offsets
         self.M = int(M)
         self.doubling = bool(doubling)
-        self.H = nat.DeviceELL(data, idx, int(offsets[-1]), K, 1.0, dtype)
-        del data, idx
-        v = np.concatenate([np.asarray(vg).reshape(-1, int(b[2])) for vg, b in zip(vectors, blocks)],
+        self.H = H
+        v = np.concatenate([np.asarray(vg).reshape(-1, n) for vg, n in zip(vectors, sizes)],
                            axis=1)
         self.R = v.shape[0]
         self.vr = _kpm.as_batch(v, int(offsets[-1]), dtype)

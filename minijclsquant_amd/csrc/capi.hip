@@ -487,6 +487,147 @@ int jkpm_ell_create(const double *data, const int *indices, int n_rows, int len_
     return JKPM_OK;
 }
 
+int jkpm_ell_create_csr(const int *indptr, const int *indices, const double *data, int n_rows,
+                        int len_row, double scale, int value_type, int src_on_device,
+                        void *stream, jkpm_ell **out) {
+    clear_error();
+    if (!out) return arg_error("out is NULL");
+    *out = nullptr;
+    if (!indptr) return arg_error("indptr is NULL");
+    if (n_rows <= 0) return arg_error("n_rows must be positive");
+    if (len_row < 0) return arg_error("len_row must be >= 0 (0: the longest row)");
+    int rc = validate_vtype(value_type);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    // nnz = indptr[n_rows]
+    int nnz_i = 0;
+    if (src_on_device) {
+        if ((e = hipMemcpyAsync(&nnz_i, indptr + n_rows, sizeof(int), hipMemcpyDeviceToHost, s)) !=
+                hipSuccess ||
+            (e = hipStreamSynchronize(s)) != hipSuccess)
+            return hip_fail(e, "read of indptr[n_rows]", __FILE__, __LINE__);
+    } else {
+        nnz_i = indptr[n_rows];
+    }
+    if (nnz_i < 0) return arg_error("indptr[n_rows] (nnz) is negative");
+    const long nnz = nnz_i;
+    if (nnz > 0 && (!indices || !data)) return arg_error("indices/data is NULL");
+
+    // Device copies of the CSR arrays (staged when they live on the host).
+    int *d_ptr = nullptr, *d_idx = nullptr;
+    double *d_val = nullptr;
+    unsigned *stats = nullptr;
+    jkpm_ell *h = nullptr;
+    auto cleanup = [&](int code) {
+        if (!src_on_device) {
+            if (d_ptr) hipFree(d_ptr);
+            if (d_idx) hipFree(d_idx);
+            if (d_val) hipFree(d_val);
+        }
+        if (stats) hipFree(stats);
+        if (h) {
+            if (h->e.val) hipFree(h->e.val);
+            if (h->e.col) hipFree(h->e.col);
+            delete h;
+        }
+        return code;
+    };
+    if (src_on_device) {
+        d_ptr = const_cast<int *>(indptr);
+        d_idx = const_cast<int *>(indices);
+        d_val = const_cast<double *>(data);
+    } else {
+        const size_t pb = (size_t)(n_rows + 1) * sizeof(int);
+        const size_t ib = (size_t)(nnz ? nnz : 1) * sizeof(int);
+        const size_t vb = (size_t)(nnz ? nnz : 1) * 16;
+        if ((e = hipMalloc(&d_ptr, pb)) != hipSuccess || (e = hipMalloc(&d_idx, ib)) != hipSuccess ||
+            (e = hipMalloc(&d_val, vb)) != hipSuccess)
+            return cleanup(hip_fail(e, "hipMalloc(CSR staging)", __FILE__, __LINE__));
+        if ((e = hipMemcpyAsync(d_ptr, indptr, pb, hipMemcpyHostToDevice, s)) != hipSuccess ||
+            (nnz && (e = hipMemcpyAsync(d_idx, indices, (size_t)nnz * sizeof(int),
+                                        hipMemcpyHostToDevice, s)) != hipSuccess) ||
+            (nnz && (e = hipMemcpyAsync(d_val, data, (size_t)nnz * 16, hipMemcpyHostToDevice, s)) !=
+                        hipSuccess))
+            return cleanup(hip_fail(e, "hipMemcpyAsync(CSR)", __FILE__, __LINE__));
+    }
+    if ((e = hipMalloc(&stats, 3 * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMemsetAsync(stats, 0, 3 * sizeof(unsigned), s)) != hipSuccess)
+        return cleanup(hip_fail(e, "scan statistics", __FILE__, __LINE__));
+    rc = launch_csr_scan(d_ptr, d_idx, n_rows, nnz, stats, s);
+    if (rc) return cleanup(rc);
+    unsigned st[3] = {0, 0, 0};
+    if ((e = hipMemcpyAsync(st, stats, sizeof(st), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+        return cleanup(hip_fail(e, "CSR scan", __FILE__, __LINE__));
+    if (st[1]) {
+        set_error(std::to_string(st[1]) +
+                  " rows have an indptr range that decreases or leaves [0, nnz]");
+        return cleanup(JKPM_ERR_ARG);
+    }
+    if (st[2]) {
+        set_error(std::to_string(st[2]) + " column indices are outside [0, n_rows)");
+        return cleanup(JKPM_ERR_ARG);
+    }
+    const int longest = (int)st[0];
+    int K = len_row;
+    if (K == 0) K = longest > 0 ? longest : 1;
+    if (longest > K) {
+        set_error("a row has " + std::to_string(longest) + " entries, more than len_row = " +
+                  std::to_string(K));
+        return cleanup(JKPM_ERR_ARG);
+    }
+    if ((long long)n_rows * K > (1LL << 31) - 1)
+        return cleanup(arg_error("n_rows*len_row exceeds 2^31-1 slots"));
+
+    h = new (std::nothrow) jkpm_ell;
+    if (!h) return cleanup(arg_error("host allocation failed"));
+    Ell &H = h->e;
+    H.n = n_rows;
+    H.k = K;
+    H.vtype = value_type;
+    H.scale = scale;
+    hipGetDevice(&H.device);
+    const size_t tot = (size_t)n_rows * K;
+    if ((e = hipMalloc(&H.val, tot * cplx_bytes(value_type))) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMalloc(values)", __FILE__, __LINE__));
+    if ((e = hipMalloc(&H.col, tot * sizeof(int))) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMalloc(cols)", __FILE__, __LINE__));
+    rc = launch_csr_to_ell(d_ptr, d_idx, d_val, scale, &H, s);
+    if (rc) return cleanup(rc);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__));
+    jkpm_ell *done = h;
+    h = nullptr;
+    cleanup(JKPM_OK);
+    done->col_owner = std::shared_ptr<int>(done->e.col, [](int *p) { hipFree(p); });
+    *out = done;
+    return JKPM_OK;
+}
+
+int jkpm_ell_get_indices(const jkpm_ell *h, int *indices, int dst_on_device, void *stream) {
+    clear_error();
+    if (!h || !indices) return arg_error("handle/indices is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t bytes = (size_t)h->e.n * h->e.k * sizeof(int);
+    if (dst_on_device) {
+        int rc = launch_ell_cols_unpack(h->e, indices, s);
+        if (rc) return rc;
+        JKPM_HIP(hipStreamSynchronize(s));
+        return JKPM_OK;
+    }
+    int *tmp = nullptr;
+    JKPM_HIP(hipMalloc(&tmp, bytes ? bytes : 4));
+    int rc = launch_ell_cols_unpack(h->e, tmp, s);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpyAsync(indices, tmp, bytes, hipMemcpyDeviceToHost, s);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(tmp);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "copy of indices to the host", __FILE__, __LINE__);
+    return JKPM_OK;
+}
+
 int jkpm_ell_create_like(const jkpm_ell *pattern, const double *data, double scale,
                          int value_type, int src_on_device, void *stream, jkpm_ell **out) {
     clear_error();

@@ -56,8 +56,9 @@ struct Ell {
     int vtype = JKPM_C128;
     double scale = 1.0;    // the stored values are scale * (uploaded values)
     // Launch shape of the step kernel (jkpm_ell_set_launch / the autotuner):
-    // variant 0..3 = (rows per lane, non-temporal streams) in
-    // {(1,no), (2,no), (1,yes), (2,yes)}; chunk_mult work-groups per CU.
+    // variant 0..23 = unroll | non-temporal loads << 1 | lane shape << 2,
+    // plus 12 for non-temporal stores of T_{n+1}; chunk_mult work-groups
+    // per CU.
     int variant = 0;
     int chunk_mult = 8;
     // the same for the doubling form of the recursion
@@ -145,6 +146,14 @@ int launch_peierls(const Ell &src, const Ell &bonds, Ell *dst, double2 a, double
                    hipStream_t s);
 // out[i*k + s] = value(i, s) / H.scale as complex128 (device, row-major).
 int launch_ell_unpack(const Ell &H, double *out, hipStream_t s);
+
+// stats (device unsigned[3], zeroed by the caller): longest row, bad indptr
+// entries, bad column indices.
+int launch_csr_scan(const int *indptr, const int *indices, int n, long nnz, unsigned *stats,
+                    hipStream_t s);
+int launch_csr_to_ell(const int *indptr, const int *indices, const double *data, double scale,
+                      Ell *H, hipStream_t s);
+int launch_ell_cols_unpack(const Ell &H, int *out, hipStream_t s);
 
 // ---- two steps per launch over LDS patches (tiles.hip) ----
 struct TilePlanHost {

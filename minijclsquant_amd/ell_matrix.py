@@ -17,7 +17,6 @@ from __future__ import annotations
 
 import copy
 import os
-import pickle
 
 import numpy as np
 import scipy.sparse as sp
@@ -210,22 +209,42 @@ class ell_matrix:
                 self.DY.data, cj, self.n_threads)
 
     def save(self, path=None):
-        """Pickles the object (the reference uses joblib.dump,
-        jclsquant/ell_matrix.py:202-207)."""
+        """joblib.dump of the object, as jclsquant/ell_matrix.py:202-207 does
+        (device copies are not saved); ``load_ell`` reads it back, and reads
+        files the reference saved."""
+        import joblib
+
         if path is None:
             print('None path selected save as ./H.json')
             path = './H.json'
-        state = self.__getstate__()
-        with open(path, 'wb') as f:
-            pickle.dump(state, f)
+        joblib.dump(self, path)
+
+    def __reduce__(self):
+        # Rebuilt through a module-level function: the package namespace
+        # binds the name ell_matrix to this class (shadowing this module, as
+        # the reference's star import does), which pickle extensions that
+        # resolve classes through package attributes cannot look through.
+        from .hams import _restore_ell
+
+        return (_restore_ell, (self.__getstate__(),))
 
     def __getstate__(self):
         st = dict(self.__dict__)
-        st['_dev'] = {}
+        st.pop('_dev', None)
+        # the reference's attribute name, so both layouts read the same way
+        st['data'] = st.pop('_data', None)
         return st
 
     def __setstate__(self, st):
+        st = dict(st)
+        data = st.pop('data', st.pop('_data', None))
+        # attributes this class has and older or reference states may lack
+        for name in ('type', 'len_row', 'len_col', 'ell_data', 'ell_indices', 'shape',
+                     'n_threads', 'DX', 'DY', 'CJ', 'dx_vec', 'dy_vec', 'dz_vec', 'Omega',
+                     'conjugates', 'valid_row_values', 'A', 'bounds', 'space', 'velocities'):
+            self.__dict__.setdefault(name, None)
         self.__dict__.update(st)
+        self._data = data
         self._dev = {}
 
     def deep_copy(self):
