@@ -14,16 +14,18 @@
 // Kernels
 //   kubo_step_kernel   T_{n+1} = a H T_n + b T_{n-1} and, fused on the same
 //                      gathered T_n, W_n = v T_n (v shares H's slot pattern).
-//   kubo_gemm_kernel   split-K complex GEMM on v_mfma_f32_32x32x2f32: the
-//                      MFMA's K = 2 spans two k; four MFMAs per k pair give
-//                      re += Lr*Rr + Li*Ri and im += Lr*Ri - Li*Rr.
-//                      Tiles of 128 x 128 complex outputs per work-group, 4
-//                      waves of 64 x 64; each wave streams its operand rows
-//                      from global memory into MFMA registers (no LDS, no
-//                      barriers).  Work-groups of one K slice sit on one XCD
-//                      so the slice is fetched from HBM once per XCD L2.
+//   kubo_gemm_kernel   split-K complex GEMM on v_mfma_f32_16x16x4f32: four
+//                      MFMAs per k-set give re += Lr*Rr + Li*Ri and
+//                      im += Lr*Ri - Li*Rr.  Tiles of 128 x 128 complex
+//                      outputs per work-group, 4 waves of 64 x 64; each wave
+//                      streams its operand rows from global memory into
+//                      MFMA registers (no LDS, no barriers).  Work-groups of
+//                      one K slice sit on one XCD so the slice is fetched
+//                      from HBM once per XCD L2.
 //   kubo_reduce_kernel FP64 sum of the FP32 slice partials in fixed order.
 //   kubo_sigma_kernel  sigma_i = sum_mn (a_m b_n + c_m d_n) mu_mn per energy.
+#include <cstdlib>
+
 #include "common.h"
 
 namespace jkpm {
@@ -35,7 +37,7 @@ constexpr int kXcd = 8;
 
 template <typename Real, int N>
 using vec_t = Real __attribute__((ext_vector_type(N)));
-using floatx16 = vec_t<float, 16>;
+using floatx4 = vec_t<float, 4>;
 
 // ------------------------------------------------------------ stack steps
 
@@ -144,23 +146,29 @@ constexpr int TM = 128;  // complex rows of L per work-group tile
 constexpr int TN = 128;  // complex rows of Rk per work-group tile
 constexpr int TK = 32;   // granularity of the split-K slices
 
-constexpr int DK = 8;    // complex k per register batch (4 MFMA k-pairs)
+constexpr int DK = 8;    // complex k per register batch (two MFMA k-sets of 4)
 
 // No LDS and no barriers: every wave streams its own operand rows from
 // global memory (L2 serves the rows the wave pairs of a work-group share)
-// straight into MFMA operand registers, with the next batch in flight while
-// the current one is multiplied.  The MFMA's K = 2 spans two k: lane
-// (l32, half) supplies element k0 + half * DK/2 + q of its row for the q-th
-// k-pair, so each lane reads DK/2 consecutive complex numbers (two 16-byte
-// loads) per row and batch.  Four MFMAs per k-pair and 32 x 32 block give
+// straight into MFMA operand registers (v_mfma_f32_16x16x4f32), with the
+// next two batches in flight while the current one is multiplied.
+// Lane (l16, quarter) of a 16 x 16 block supplies row l16 and the quarter's
+// two consecutive complex k of each DK = 8 batch, so one 16-byte load per
+// row tile covers 16 rows x 64 contiguous bytes: every 64-byte sector a
+// load touches is fully used by that load.  (With v_mfma_f32_32x32x2f32 a
+// load touches 32 rows with 32 bytes each, every sector is requested twice,
+// and the GEMM ran at half the rate.)  The two complex numbers of a lane
+// are the K = 4 of two MFMAs (k-sets {0,2,4,6} and {1,3,5,7} of the batch);
+// four MFMAs per k-set and block give
 //   re += Lr Rr + Li Ri,   im += Lr Ri - Li Rr   (conj(L) Rk).
+// Wave tile 64 x 64 complex = 4 x 4 blocks; D of a block: lane l, register
+// r -> row 4 (l / 16) + r, column l % 16.
 // ALIGNED (K even): the row starts are 16-byte aligned and loads go two
 // complex numbers at a time.
 template <bool ALIGNED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void kubo_gemm_kernel(
     const float2 *__restrict__ L, const float2 *__restrict__ Rk, long K, int M, int ntm,
     int ntn, int nslices, long k_per_slice, float2 *__restrict__ part) {
-    constexpr int QP = DK / 2;  // k-pairs per batch
     const int ntiles = ntm * ntn;
     const int b = blockIdx.x;
     int slice, tile;
@@ -178,106 +186,117 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;  // 2 x 2 waves of 64 x 64
-    const int half = lane >> 5, l32 = lane & 31;
+    const int quarter = lane >> 4, l16 = lane & 15;
 
-    const float2 *pa[2], *pb[2];
-    bool va[2], vb[2];
+    // Rows past M read row 0: their products only reach output rows /
+    // columns that are never stored.
+    const float2 *pa[4], *pb[4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const int m = m0 + wm * 64 + i * 32 + l32;
-        va[i] = m < M;
-        pa[i] = L + (long)(va[i] ? m : 0) * K;
-        const int nn = n0 + wn * 64 + i * 32 + l32;
-        vb[i] = nn < M;
-        pb[i] = Rk + (long)(vb[i] ? nn : 0) * K;
+    for (int i = 0; i < 4; ++i) {
+        const int m = m0 + wm * 64 + i * 16 + l16;
+        pa[i] = L + (long)(m < M ? m : 0) * K;
+        const int nn = n0 + wn * 64 + i * 16 + l16;
+        pb[i] = Rk + (long)(nn < M ? nn : 0) * K;
     }
 
-    floatx16 cre[2][2], cim[2][2];
+    floatx4 cre[4][4], cim[4][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) cre[i][j][r] = cim[i][j][r] = 0.f;
+            for (int r = 0; r < 4; ++r) cre[i][j][r] = cim[i][j][r] = 0.f;
 
-    // Three register batches in rotation: while batch t is multiplied,
-    // batches t+1 and t+2 are in flight.
-    float2 A0[2][QP], B0[2][QP], A1[2][QP], B1[2][QP], A2[2][QP], B2[2][QP];
-    auto load = [&](long kc, float2 (&a)[2][QP], float2 (&bb)[2][QP]) {
-        const long k0 = kc + half * QP;
+    // Three register batches in rotation, as in kubo_gemm_kernel: plain
+    // loads in the main loop (a prefetch past k_end reads the slice's last
+    // valid k and is never multiplied), zero-masked loads for the at most
+    // three batches that reach past k_end.
+    float4 A0[4], B0[4], A1[4], B1[4], A2[4], B2[4];
+    const long k_last = k_end - 2;  // last two-complex load start (ALIGNED)
+    auto load = [&](long kc, float4 (&a)[4], float4 (&bb)[4], bool masked) {
+        const long kk = kc + quarter * 2;
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < 4; ++i) {
             if constexpr (ALIGNED) {
-#pragma unroll
-                for (int q = 0; q < QP; q += 2) {
-                    const long kk = k0 + q;  // even, and k_end is even
-                    float4 x = make_float4(0.f, 0.f, 0.f, 0.f), y = x;
-                    if (kk < k_end) {
-                        if (va[i]) x = *reinterpret_cast<const float4 *>(pa[i] + kk);
-                        if (vb[i]) y = *reinterpret_cast<const float4 *>(pb[i] + kk);
-                    }
-                    a[i][q] = make_float2(x.x, x.y);
-                    a[i][q + 1] = make_float2(x.z, x.w);
-                    bb[i][q] = make_float2(y.x, y.y);
-                    bb[i][q + 1] = make_float2(y.z, y.w);
-                }
+                const long kc_ = kk < k_end ? kk : k_last;
+                float4 x = *reinterpret_cast<const float4 *>(pa[i] + kc_);
+                float4 y = *reinterpret_cast<const float4 *>(pb[i] + kc_);
+                if (masked && kk >= k_end) x = y = make_float4(0.f, 0.f, 0.f, 0.f);
+                a[i] = x;
+                bb[i] = y;
             } else {
-#pragma unroll
-                for (int q = 0; q < QP; ++q) {
-                    const long kk = k0 + q;
-                    const bool in = kk < k_end;
-                    a[i][q] = (in && va[i]) ? pa[i][kk] : make_float2(0.f, 0.f);
-                    bb[i][q] = (in && vb[i]) ? pb[i][kk] : make_float2(0.f, 0.f);
-                }
+                const long k0 = kk < k_end ? kk : k_end - 1;
+                const long k1 = kk + 1 < k_end ? kk + 1 : k_end - 1;
+                float2 x0 = pa[i][k0], x1 = pa[i][k1], y0 = pb[i][k0], y1 = pb[i][k1];
+                if (masked && kk >= k_end) x0 = y0 = make_float2(0.f, 0.f);
+                if (masked && kk + 1 >= k_end) x1 = y1 = make_float2(0.f, 0.f);
+                a[i] = make_float4(x0.x, x0.y, x1.x, x1.y);
+                bb[i] = make_float4(y0.x, y0.y, y1.x, y1.y);
             }
         }
     };
-    // Eight independent accumulators between two updates of the same one.
-    auto mul = [&](const float2 (&a)[2][QP], const float2 (&bb)[2][QP]) {
+    // Sixteen independent accumulator pairs between two updates of one.
+    auto mul = [&](const float4 (&a)[4], const float4 (&bb)[4]) {
 #pragma unroll
-        for (int q = 0; q < QP; ++q) {
+        for (int g = 0; g < 2; ++g) {
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const float2 x = a[i][q], c = bb[j][q];
-                    cre[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.x, c.x, cre[i][j], 0, 0, 0);
-                    cim[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.x, c.y, cim[i][j], 0, 0, 0);
+                for (int j = 0; j < 4; ++j) {
+                    const float ar = g ? a[i].z : a[i].x, br = g ? bb[j].z : bb[j].x;
+                    const float bi = g ? bb[j].w : bb[j].y;
+                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, cre[i][j], 0, 0, 0);
+                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, cim[i][j], 0, 0, 0);
                 }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const float2 x = a[i][q], c = bb[j][q];
-                    cre[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.y, c.y, cre[i][j], 0, 0, 0);
-                    cim[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.y, -c.x, cim[i][j], 0, 0, 0);
+                for (int j = 0; j < 4; ++j) {
+                    const float ai = g ? a[i].w : a[i].y, br = g ? bb[j].z : bb[j].x;
+                    const float bi = g ? bb[j].w : bb[j].y;
+                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, cre[i][j], 0, 0, 0);
+                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, -br, cim[i][j], 0, 0, 0);
                 }
         }
     };
-    if (k_begin < k_end) load(k_begin, A0, B0);
-    if (k_begin + DK < k_end) load(k_begin + DK, A1, B1);
-    for (long kc = k_begin; kc < k_end; kc += 3 * DK) {
-        if (kc + 2 * DK < k_end) load(kc + 2 * DK, A2, B2);
-        mul(A0, B0);
-        if (kc + DK >= k_end) break;
-        if (kc + 3 * DK < k_end) load(kc + 3 * DK, A0, B0);
-        mul(A1, B1);
-        if (kc + 2 * DK >= k_end) break;
-        if (kc + 4 * DK < k_end) load(kc + 4 * DK, A1, B1);
-        mul(A2, B2);
+    if (k_begin < k_end) {
+        const long nb = (k_end - k_begin + DK - 1) / DK;
+        const long nfull = (k_end - k_begin) / DK;
+        const long n3 = nfull / 3 * 3;
+        load(k_begin, A0, B0, false);
+        __builtin_amdgcn_sched_barrier(0);
+        load(k_begin + DK, A1, B1, false);
+        __builtin_amdgcn_sched_barrier(0);
+        for (long t = 0; t < n3; t += 3) {
+            const long kc = k_begin + t * DK;
+            load(kc + 2 * DK, A2, B2, false);
+            __builtin_amdgcn_sched_barrier(0);
+            mul(A0, B0);
+            __builtin_amdgcn_sched_barrier(0);
+            load(kc + 3 * DK, A0, B0, false);
+            __builtin_amdgcn_sched_barrier(0);
+            mul(A1, B1);
+            __builtin_amdgcn_sched_barrier(0);
+            load(kc + 4 * DK, A1, B1, false);
+            __builtin_amdgcn_sched_barrier(0);
+            mul(A2, B2);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        for (long t = n3; t < nb; ++t) {
+            load(k_begin + t * DK, A2, B2, true);
+            mul(A2, B2);
+        }
     }
 
-    // D[i][j] of a 32 x 32 block: lane l, register r -> j = l % 32,
-    // i = 8 (r / 4) + 4 (l / 32) + r % 4.
     float2 *P = part + (long)slice * M * M;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = m0 + wm * 64 + i * 32 + 8 * (r / 4) + 4 * half + (r % 4);
-                const int cc = n0 + wn * 64 + j * 32 + l32;
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm * 64 + i * 16 + 4 * quarter + r;
+                const int cc = n0 + wn * 64 + j * 16 + l16;
                 if (row < M && cc < M) P[(long)row * M + cc] = make_float2(cre[i][j][r], cim[i][j][r]);
             }
 }
@@ -363,7 +382,15 @@ __global__ __launch_bounds__(kBlock) void kubo_sigma_kernel(const double2 *__res
 
 int kubo_gemm_slices(long K, int M, const DeviceInfo &dev) {
     const int ntiles = ((M + TM - 1) / TM) * ((M + TN - 1) / TN);
-    long want = ((long)dev.cu_count * 4 + ntiles - 1) / ntiles;  // ~4 work-groups per CU
+    // ~16 work-groups per CU over the launch (8 resident): short slices keep
+    // the 16 tiles of a slice, which share its rows and run side by side on
+    // one XCD, close together in k, so the rows they share stay in L2.
+    long want = ((long)dev.cu_count * 16 + ntiles - 1) / ntiles;
+    // JKPM_KUBO_SLICES overrides the split-K slice count (tuning)
+    if (const char *env = getenv("JKPM_KUBO_SLICES")) {
+        const long v = atol(env);
+        if (v > 0) want = v;
+    }
     want = (want + kXcd - 1) / kXcd * kXcd;
     const long max_by_k = (K + TK - 1) / TK;
     if (want > max_by_k) want = max_by_k;
