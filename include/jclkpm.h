@@ -194,6 +194,16 @@ typedef struct jkpm_batch jkpm_batch;
  * h must outlive the batch. */
 int jkpm_batch_create(const jkpm_ell *h, int G, const long long *row_offsets,
                       jkpm_batch **out);
+
+/* As jkpm_batch_create, but realization g with chunk_rows[g] > 0 is cut into
+ * chunks of exactly chunk_rows[g] rows (the last one shorter); 0 keeps the
+ * proportional cut.  Giving a lattice's row stride (the index distance of
+ * its vertical neighbours) makes the work-groups of consecutive chunks, which
+ * run side by side on one XCD, read each other's neighbour rows at the same
+ * time, so each T_n row is fetched from HBM once instead of once per use.
+ * No reference counterpart (the reference runs one realization per call). */
+int jkpm_batch_create_strided(const jkpm_ell *h, int G, const long long *row_offsets,
+                              const int *chunk_rows, jkpm_batch **out);
 int jkpm_batch_destroy(jkpm_batch *b);
 int jkpm_batch_workspace(const jkpm_batch *b, int R, int n_left, size_t *bytes);
 

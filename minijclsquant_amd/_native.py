@@ -75,6 +75,7 @@ SIGNATURES = {
     "jkpm_moments_grad": (_int, [_vp, _vp, _int, _int, _vp, _int, _vp, _vp, _size, _vp]),
     "jkpm_dos_grad": (_int, [_vp, _int, _int, _vp, _int, _int, _vp, _vp]),
     "jkpm_batch_create": (_int, [_vp, _int, _vp, _c.POINTER(_vp)]),
+    "jkpm_batch_create_strided": (_int, [_vp, _int, _vp, _vp, _c.POINTER(_vp)]),
     "jkpm_batch_destroy": (_int, [_vp]),
     "jkpm_batch_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
     "jkpm_batch_moments": (_int, [_vp, _vp, _vp, _vp, _int, _int, _vp, _vp, _size, _vp]),
@@ -611,14 +612,23 @@ class Batch:
     """A DeviceELL seen as a block-diagonal batch of G realizations with rows
     [offsets[g], offsets[g+1]) (jkpm_batch_*)."""
 
-    def __init__(self, H: DeviceELL, offsets):
+    def __init__(self, H: DeviceELL, offsets, chunk_rows=None):
+        """chunk_rows (optional, per realization): chunks of exactly that many
+        rows (0: the proportional cut), jkpm_batch_create_strided."""
         require_device()
         off = np.ascontiguousarray(offsets, dtype=np.int64)
         self.H = H
         self.G = off.size - 1
         self.offsets = off
         self._b = _vp()
-        check(lib().jkpm_batch_create(H.handle, self.G, off.ctypes.data, _c.byref(self._b)))
+        if chunk_rows is None:
+            check(lib().jkpm_batch_create(H.handle, self.G, off.ctypes.data, _c.byref(self._b)))
+        else:
+            cr = np.ascontiguousarray(chunk_rows, dtype=np.int32)
+            if cr.size != self.G:
+                raise ValueError("chunk_rows needs one entry per realization")
+            check(lib().jkpm_batch_create_strided(H.handle, self.G, off.ctypes.data,
+                                                  cr.ctypes.data, _c.byref(self._b)))
 
     def workspace_bytes(self, R: int, n_left: int = 1) -> int:
         out = _size()

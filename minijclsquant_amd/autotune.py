@@ -20,7 +20,7 @@ from pathlib import Path
 
 from . import _native as nat
 
-__all__ = ["tuned", "table_path"]
+__all__ = ["tuned", "tuned_batch", "table_path"]
 
 _lock = threading.Lock()
 
@@ -83,3 +83,52 @@ def tuned(h: "nat.DeviceELL", v_r, steps: int = 8, retune: bool = False,
         tab[key] = {"variant": v, "chunk_mult": m, "step_us": us}
         _store(tab)
         return v, m
+
+
+def tuned_batch(bk, steps: int = 16, retune: bool = False, variants=range(24)):
+    """Step-kernel variant of both forms for a BatchKPM whose chunk tables are
+    fixed (chunks of one lattice stride): times every variant on the batch
+    itself, where the tables, not chunk_mult, set the work-group count.
+    Cached in the same table under the batch's own key (ΣN, K, R, the
+    strides' sum).  Returns {"plain": v, "doubling": v}."""
+    import torch
+
+    env = os.environ.get("JKPM_LAUNCH")
+    h, R = bk.H, bk.R
+    out = {}
+    for doubling in (False, True):
+        cur_v, cur_m = h.get_launch(doubling)
+        if env:
+            out["doubling" if doubling else "plain"] = cur_v
+            continue
+        key = _key(h, R, doubling) + f"|batch-strided|G={bk.batch.G}|S={sum(bk.strides or [])}"
+        with _lock:
+            tab = _load()
+            if not retune and key in tab:
+                v = tab[key]["variant"]
+                h.set_launch(v, cur_m, doubling)
+                out["doubling" if doubling else "plain"] = v
+                continue
+            M = steps + 1 if not doubling else 2 * steps
+            need = nat.moments_ex_workspace_bytes(None, bk.batch, R, M, doubling)
+            ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+            mu = torch.empty((bk.batch.G, M, R), dtype=torch.complex128, device="cuda")
+            best = (float("inf"), cur_v)
+            for v in variants:
+                h.set_launch(v, cur_m, doubling)
+                nat.moments_ex(None, bk.batch, bk.vr, M, doubling, workspace=ws, out=mu)
+                t = float("inf")
+                for _ in range(2):
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    nat.moments_ex(None, bk.batch, bk.vr, M, doubling, workspace=ws, out=mu)
+                    e1.record()
+                    e1.synchronize()
+                    t = min(t, e0.elapsed_time(e1))
+                if t < best[0]:
+                    best = (t, v)
+            h.set_launch(best[1], cur_m, doubling)
+            tab[key] = {"variant": best[1], "chunk_mult": cur_m, "step_us": best[0] * 1e3 / steps}
+            _store(tab)
+            out["doubling" if doubling else "plain"] = best[1]
+    return out

@@ -19,7 +19,8 @@ import numpy as np
 from . import _kpm
 from . import _native as nat
 
-__all__ = ["square_lattice_ell", "block_diagonal", "stack_csr", "BatchKPM", "kpm_dos_batch"]
+__all__ = ["square_lattice_ell", "block_diagonal", "stack_csr", "lattice_stride", "BatchKPM",
+           "kpm_dos_batch"]
 
 
 def square_lattice_ell(L, t=-1.0, W=0.0, seed=None):
@@ -67,6 +68,31 @@ def block_diagonal(blocks):
     return data, idx, offsets, K
 
 
+def lattice_stride(rows, cols, n_rows, lo=64):
+    """The row stride of a lattice-like matrix: the most common column
+    distance |j - i| in (lo, n_rows/2] over its entries (rows[e], cols[e]),
+    e.g. L for a row-major L x L square lattice; 0 when no such distance
+    holds a quarter of the rows (no dominant long-range neighbour)."""
+    d = np.abs(np.asarray(cols, dtype=np.int64) - np.asarray(rows, dtype=np.int64))
+    d = d[(d > lo) & (d <= n_rows // 2)]
+    if d.size < max(1, n_rows // 4):
+        return 0
+    counts = np.bincount(d)
+    s = int(np.argmax(counts))
+    return s if counts[s] >= max(1, n_rows // 4) else 0
+
+
+def _ell_stride(indices, n, k):
+    n, k = int(n), int(k)
+    return lattice_stride(np.repeat(np.arange(n, dtype=np.int64), k), indices, n)
+
+
+def _csr_stride(m):
+    n = int(m.shape[0])
+    return lattice_stride(np.repeat(np.arange(n, dtype=np.int64), np.diff(m.indptr)),
+                          m.indices[:int(m.indptr[-1])], n)
+
+
 def stack_csr(mats, scales):
     """Block-diagonal CSR of square CSR blocks, block g's values times
     scales[g]: (indptr int32, indices int32, data complex128, row offsets)."""
@@ -96,17 +122,24 @@ class BatchKPM:
     of each realization and the scale 1/B_g of its spectral rescale.
     vectors: list of [R, N_g] arrays (same R for every realization).
     tune: pick the step kernel's launch shapes (plain and doubling) with the
-    autotuner before the batch's work-group chunks are laid out."""
+    autotuner before the batch's work-group chunks are laid out.
+    stride_chunks: cut each realization whose matrix has a dominant row
+    stride (``lattice_stride``, e.g. L for an L x L square lattice) into
+    chunks of that many rows; False keeps the proportional cut."""
 
-    def __init__(self, blocks, vectors, M, dtype="c64", doubling=False, tune=False):
+    def __init__(self, blocks, vectors, M, dtype="c64", doubling=False, tune=False,
+                 stride_chunks=True):
         nat.require_device()
         data, idx, offsets, K = block_diagonal(blocks)
         H = nat.DeviceELL(data, idx, int(offsets[-1]), K, 1.0, dtype)
         del data, idx
-        self._setup(H, offsets, vectors, [int(b[2]) for b in blocks], M, dtype, doubling, tune)
+        strides = [_ell_stride(b[1], b[2], b[3]) for b in blocks] if stride_chunks else None
+        self._setup(H, offsets, vectors, [int(b[2]) for b in blocks], M, dtype, doubling, tune,
+                    strides)
 
     @classmethod
-    def from_csr(cls, mats, scales, vectors, M, dtype="c64", doubling=False, tune=False):
+    def from_csr(cls, mats, scales, vectors, M, dtype="c64", doubling=False, tune=False,
+                 stride_chunks=True):
         """The batch from scipy CSR realizations (square, each with its scale
         1/B_g): their CSR arrays are stacked block-diagonally on the host and
         converted to ELL on the device (jkpm_ell_create_csr), so no host ELL
@@ -118,11 +151,12 @@ class BatchKPM:
         H = nat.DeviceELL.from_csr(indptr, indices, data, int(offsets[-1]), 1.0, dtype)
         del indptr, indices, data
         self = cls.__new__(cls)
+        strides = [_csr_stride(m) for m in mats] if stride_chunks else None
         self._setup(H, offsets, vectors, [int(m.shape[0]) for m in mats], M, dtype, doubling,
-                    tune)
+                    tune, strides)
         return self
 
-    def _setup(self, H, offsets, vectors, sizes, M, dtype, doubling, tune):
+    def _setup(self, H, offsets, vectors, sizes, M, dtype, doubling, tune, strides=None):
         torch = nat.require_device()
         self.offsets = offsets
         self.M = int(M)
@@ -139,7 +173,18 @@ class BatchKPM:
 
             autotune.tuned(self.H, self.vr)
             autotune.tuned(self.H, self.vr, doubling=True)
-        self.batch = nat.Batch(self.H, offsets)
+        # Chunks of one lattice stride: the work-groups of consecutive chunks
+        # run side by side on one XCD and read each other's vertical
+        # neighbour rows at the same time, so T_n is fetched from HBM once.
+        self.strides = strides
+        self.batch = nat.Batch(self.H, offsets,
+                               None if not strides or not any(strides) else strides)
+        if tune and strides and any(strides):
+            # the work-group count now comes from the strides: pick the
+            # variants on the batch itself
+            from . import autotune
+
+            autotune.tuned_batch(self)
         self.ws = None
         self.mu = torch.empty((self.batch.G, self.M, self.R), dtype=torch.complex128, device="cuda")
         self.graph = None

@@ -924,8 +924,13 @@ void free_table(ChunkTable &c) {
 // chunks go, one at a time, to the realization whose chunks are currently the
 // largest, so the largest chunk (the launch's critical path) stays close to
 // the mean even when realizations are few chunks long.
-int build_table(const jkpm_ell *h, int G, const long long *row_offsets, int mult,
-                const DeviceInfo &dev, ChunkTable *out) {
+// chunk_rows (optional, per realization): a positive entry cuts that
+// realization into chunks of exactly that many rows instead (the last one
+// shorter), so that the work-groups of consecutive chunks, which run side by
+// side on one XCD, stand one lattice stride apart and read each other's
+// neighbour rows at the same time (one HBM fetch shared through L2).
+int build_table(const jkpm_ell *h, int G, const long long *row_offsets, const int *chunk_rows,
+                int mult, const DeviceInfo &dev, ChunkTable *out) {
     const long target = std::max<long>(G, (long)dev.cu_count * mult);
     std::vector<long> cnt(G, 1);
     auto rows_of = [&](int g) { return (long)(row_offsets[g + 1] - row_offsets[g]); };
@@ -948,6 +953,7 @@ int build_table(const jkpm_ell *h, int G, const long long *row_offsets, int mult
         const long n0 = row_offsets[g], ng = rows_of(g);
         long per = (ng + cnt[g] - 1) / cnt[g];
         per = (per + 63) / 64 * 64;
+        if (chunk_rows && chunk_rows[g] > 0) per = chunk_rows[g];
         for (long r0 = 0; r0 < ng; r0 += per) rows.push_back(n0 + std::min(ng, r0 + per));
         gch.push_back((int)rows.size() - 1);
     }
@@ -972,10 +978,18 @@ int build_table(const jkpm_ell *h, int G, const long long *row_offsets, int mult
 }  // namespace
 
 int jkpm_batch_create(const jkpm_ell *h, int G, const long long *row_offsets, jkpm_batch **out) {
+    return jkpm_batch_create_strided(h, G, row_offsets, nullptr, out);
+}
+
+int jkpm_batch_create_strided(const jkpm_ell *h, int G, const long long *row_offsets,
+                              const int *chunk_rows, jkpm_batch **out) {
     clear_error();
     if (!out) return arg_error("out is NULL");
     *out = nullptr;
     if (!h || !row_offsets) return arg_error("NULL argument");
+    if (chunk_rows)
+        for (int g = 0; g < G; ++g)
+            if (chunk_rows[g] < 0) return arg_error("chunk_rows must be >= 0");
     if (G <= 0) return arg_error("G must be positive");
     if (row_offsets[0] != 0 || row_offsets[G] != h->e.n)
         return arg_error("row_offsets must start at 0 and end at the handle's row count");
@@ -989,8 +1003,8 @@ int jkpm_batch_create(const jkpm_ell *h, int G, const long long *row_offsets, jk
     if (!b) return arg_error("host allocation failed");
     b->h = h;
     b->G = G;
-    rc = build_table(h, G, row_offsets, h->e.chunk_mult, dev, &b->t[0]);
-    if (!rc) rc = build_table(h, G, row_offsets, h->e.dchunk_mult, dev, &b->t[1]);
+    rc = build_table(h, G, row_offsets, chunk_rows, h->e.chunk_mult, dev, &b->t[0]);
+    if (!rc) rc = build_table(h, G, row_offsets, chunk_rows, h->e.dchunk_mult, dev, &b->t[1]);
     if (rc) {
         free_table(b->t[0]);
         free_table(b->t[1]);

@@ -101,3 +101,30 @@ def test_batching_workload_full_size(torch_gpu):
         ref = unbatched(blocks[g], vecs[g], M, "c64")
         assert np.max(np.abs(mu[g] - ref)) / np.max(np.abs(ref)) <= 1e-12, g
     del torch
+
+
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_stride_chunks_equal_proportional_chunks(torch_gpu, dtype):
+    """Chunks of one lattice stride (L rows) give the moments of the
+    proportional cut up to the FP64 order of the dot partials, and the
+    oracle's in complex128."""
+    Ls, Ws = [70, 129, 100, 40], [0.5, 2.0, 1.3, 3.0]
+    R, M = 3, 120
+    blocks, vecs = realizations(Ls, Ws, R, 7)
+    bs = jcl.BatchKPM(blocks, vecs, M, dtype)
+    assert bs.strides == [70, 129, 100, 0]
+    bp = jcl.BatchKPM(blocks, vecs, M, dtype, stride_chunks=False)
+    ms, mp = bs.run().cpu().numpy(), bp.run().cpu().numpy()
+    for g in range(len(Ls)):
+        assert np.max(np.abs(ms[g] - mp[g])) / np.max(np.abs(mp[g])) <= 1e-12, g
+    if dtype == "c128":
+        for g in (0, 1):
+            for r in range(R):
+                o = O.rec_A_tab(M, blocks[g][0] * blocks[g][4], blocks[g][1], 5, vecs[g][r])
+                assert np.max(np.abs(ms[g][:, r] - o)) / abs(o[0]) <= 1e-10
+    # the doubling form on the stride tables as well
+    bd = jcl.BatchKPM(blocks, vecs, M, dtype, doubling=True)
+    md = bd.run().cpu().numpy()
+    tol = 1e-10 if dtype == "c128" else 1e-5
+    for g in range(len(Ls)):
+        assert np.max(np.abs(md[g] - mp[g])) / np.max(np.abs(mp[g])) <= tol, g

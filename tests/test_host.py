@@ -286,3 +286,22 @@ def test_tile_plan_rejects_bad_input():
         nat.tiles_plan_check(np.array([0, 5], np.int32), 2, 1, 64)  # column out of range
     with pytest.raises(ValueError):
         nat.tiles_plan_check(np.zeros(8, np.int32), 2, 4, 6)  # cap below 2K+2
+
+
+def test_lattice_stride():
+    """The dominant long-range column distance of lattice matrices."""
+    from minijclsquant_amd.batch import lattice_stride, square_lattice_ell, _ell_stride, _csr_stride
+
+    for L in (65, 100, 257):
+        d, i = square_lattice_ell(L, -1.0, 1.0, seed=0)
+        assert _ell_stride(i, L * L, 5) == L
+    H, _ = jcl.square_lattice(90, seed=1)
+    assert _csr_stride(H) == 90
+    # short-range only, or no dominant distance: 0
+    d, i = square_lattice_ell(40, -1.0, 0.0, seed=0)
+    assert _ell_stride(i, 1600, 5) == 0
+    rng = np.random.default_rng(0)
+    n = 5000
+    rows = np.repeat(np.arange(n), 4)
+    cols = rng.integers(0, n, size=4 * n)
+    assert lattice_stride(rows, cols, n) == 0
