@@ -14,8 +14,9 @@ constexpr int kXcd = 8;
 template <typename Real, int N>
 using vec_t = Real __attribute__((ext_vector_type(N)));
 
-// Loads of data read once per step (T_{n-1}, v_l): non-temporal when NT, so
-// they do not displace the gather window of T_n from the caches.
+// Loads of data read once per step (the matrix values and columns, T_{n-1},
+// v_l): non-temporal when NT, so they do not displace the gather window of
+// T_n from the caches.
 template <bool NT, typename V>
 __device__ __forceinline__ V load_stream(const V *p) {
     if constexpr (NT)
@@ -96,8 +97,8 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
 #pragma unroll
                 for (int s = 0; s < KT; ++s) {
                     const long e = (long)s * n + rows[u];
-                    j[u][s] = col[e];
-                    h[u][s] = *reinterpret_cast<const C2 *>(val + 2 * e);
+                    j[u][s] = load_stream<NT>(col + e);
+                    h[u][s] = load_stream<NT>(reinterpret_cast<const C2 *>(val + 2 * e));
                 }
             V x[UNR][KT];
 #pragma unroll
@@ -119,8 +120,8 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
             for (int u = 0; u < UNR; ++u)
                 for (int s = 0; s < K; ++s) {
                     const long e = (long)s * n + rows[u];
-                    const int j = col[e];
-                    const C2 h = *reinterpret_cast<const C2 *>(val + 2 * e);
+                    const int j = load_stream<NT>(col + e);
+                    const C2 h = load_stream<NT>(reinterpret_cast<const C2 *>(val + 2 * e));
                     const V x = *reinterpret_cast<const V *>(xcur + (long)j * ld * 2 + voff);
 #pragma unroll
                     for (int v = 0; v < VPL; ++v) {
