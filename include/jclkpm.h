@@ -158,10 +158,11 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
  * ------------------------------------------------------------------- */
 
 /* variant = unroll | nontemporal << 1 | shape << 2 (0..11), plus 12 for
- * non-temporal stores of T_{n+1} (12..23): unroll = two rows per lane,
- * nontemporal = non-temporal loads of the read-once streams (T_{n-1}, v_l),
- * shape = complex64 vectors per lane 2 << shape (lanes per row shrink to
- * match; c64 groups of 4+ vectors).
+ * non-temporal stores of T_{n+1} (12..23), plus 24 for non-temporal loads of
+ * the matrix values and columns (24..47): unroll = two rows per lane,
+ * nontemporal = non-temporal loads of the read-once vector streams
+ * (T_{n-1}, v_l), shape = complex64 vectors per lane 2 << shape (lanes per
+ * row shrink to match; c64 groups of 4+ vectors).
  * Variants exist for ELL widths 4 and 5; other shapes use 0.  chunk_mult:
  * work-groups per CU (1..64, default 8).  flags = JKPM_MOMENTS_DOUBLING
  * addresses the doubling form's own setting; 0 the plain recursion's. */

@@ -1,7 +1,8 @@
 """Launch-shape autotuner with a persistent table.
 
-The step kernel has 24 variants (rows per lane x non-temporal loads x
-vectors per lane x non-temporal stores) and a work-group count per CU, set separately for the
+The step kernel has 48 variants (rows per lane x non-temporal vector loads x
+vectors per lane x non-temporal stores x non-temporal matrix loads) and a
+work-group count per CU, set separately for the
 plain recursion and for the doubling form; jkpm_autotune times them on the
 caller's matrix and vectors.  Results are cached in a JSON table keyed by
 device, matrix shape (N, K), value type, batch size R and form, so a shape
@@ -85,7 +86,7 @@ def tuned(h: "nat.DeviceELL", v_r, steps: int = 8, retune: bool = False,
         return v, m
 
 
-def tuned_batch(bk, steps: int = 16, retune: bool = False, variants=range(24)):
+def tuned_batch(bk, steps: int = 16, retune: bool = False, variants=range(48)):
     """Step-kernel variant of both forms for a BatchKPM whose chunk tables are
     fixed (chunks of one lattice stride): times every variant on the batch
     itself, where the tables, not chunk_mult, set the work-group count.

@@ -1,4 +1,7 @@
 // complex128 instantiations of the Chebyshev step kernel and their dispatch.
+// Built twice: as itself (NTM = false: every kernel, and pick_step_c128)
+// and, included from cheb_c128_ntm.hip with JKPM_STEP_NTM_UNIT defined, for
+// the tuned shapes with non-temporal matrix loads (pick_step_c128_ntm).
 #include "cheb_step.h"
 
 namespace jkpm {
@@ -7,25 +10,31 @@ namespace {
 
 using Real = double;
 
-template <int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool NS = false>
+#ifdef JKPM_STEP_NTM_UNIT
+constexpr bool kNTM = true;
+#else
+constexpr bool kNTM = false;
+#endif
+
+template <int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool NS = false, bool NTM = false>
 const void *ptr() {
     return reinterpret_cast<const void *>(
-        &cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT, false, NS>);
+        &cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT, false, NS, NTM>);
 }
 
-template <int VPL, int LPR, int KT, int UNR, bool NT, bool NS = false>
+template <int VPL, int LPR, int KT, int UNR, bool NT, bool NS = false, bool NTM = false>
 const void *dbl_ptr() {
     return reinterpret_cast<const void *>(
-        &cheb_step_kernel<Real, VPL, LPR, 2, KT, UNR, NT, true, NS>);
+        &cheb_step_kernel<Real, VPL, LPR, 2, KT, UNR, NT, true, NS, NTM>);
 }
 
 template <int VPL, int LPR, int KT, bool NS>
 const void *dbl_tuned_ns(int v) {
     switch (v & 3) {
-        case 1: return dbl_ptr<VPL, LPR, KT, 2, false, NS>();
-        case 2: return dbl_ptr<VPL, LPR, KT, 1, true, NS>();
-        case 3: return dbl_ptr<VPL, LPR, KT, 2, true, NS>();
-        default: return dbl_ptr<VPL, LPR, KT, 1, false, NS>();
+        case 1: return dbl_ptr<VPL, LPR, KT, 2, false, NS, kNTM>();
+        case 2: return dbl_ptr<VPL, LPR, KT, 1, true, NS, kNTM>();
+        case 3: return dbl_ptr<VPL, LPR, KT, 2, true, NS, kNTM>();
+        default: return dbl_ptr<VPL, LPR, KT, 1, false, NS, kNTM>();
     }
 }
 
@@ -40,16 +49,17 @@ StepEntry dbl_entry(int k, int v) {
     const int u = (v & 1) ? 2 : 1;
     if (k == 4) return {dbl_tuned<VPL, LPR, 4>(v), VPL, LPR, u};
     if (k == 5) return {dbl_tuned<VPL, LPR, 5>(v), VPL, LPR, u};
-    return {dbl_ptr<VPL, LPR, 0, 1, false>(), VPL, LPR, 1};
+    if constexpr (kNTM) return {nullptr, 0, 0, 1};
+    else return {dbl_ptr<VPL, LPR, 0, 1, false>(), VPL, LPR, 1};
 }
 
 template <int VPL, int LPR, int KT, bool NS>
 const void *tuned_ns(int v) {
     switch (v & 3) {
-        case 1: return ptr<VPL, LPR, 1, KT, 2, false, NS>();
-        case 2: return ptr<VPL, LPR, 1, KT, 1, true, NS>();
-        case 3: return ptr<VPL, LPR, 1, KT, 2, true, NS>();
-        default: return ptr<VPL, LPR, 1, KT, 1, false, NS>();
+        case 1: return ptr<VPL, LPR, 1, KT, 2, false, NS, kNTM>();
+        case 2: return ptr<VPL, LPR, 1, KT, 1, true, NS, kNTM>();
+        case 3: return ptr<VPL, LPR, 1, KT, 2, true, NS, kNTM>();
+        default: return ptr<VPL, LPR, 1, KT, 1, false, NS, kNTM>();
     }
 }
 
@@ -58,23 +68,26 @@ const void *tuned(int v) {
     return v >= 12 ? tuned_ns<VPL, LPR, KT, true>(v) : tuned_ns<VPL, LPR, KT, false>(v);
 }
 
+// The NTM unit has only the tuned kernels (fn = NULL otherwise).
 template <int VPL, int LPR>
 StepEntry entry(int nl, int k, int v, bool dbl) {
     if (dbl) return dbl_entry<VPL, LPR>(k, v);
-    if (nl == 2) {
-        const void *f = k == 4 ? ptr<VPL, LPR, 2, 4, 1, false>()
-                      : k == 5 ? ptr<VPL, LPR, 2, 5, 1, false>()
-                               : ptr<VPL, LPR, 2, 0, 1, false>();
-        return {f, VPL, LPR, 1};
+    if (nl == 1 && k == 4) return {tuned<VPL, LPR, 4>(v), VPL, LPR, (v & 1) ? 2 : 1};
+    if (nl == 1 && k == 5) return {tuned<VPL, LPR, 5>(v), VPL, LPR, (v & 1) ? 2 : 1};
+    if constexpr (kNTM) {
+        return {nullptr, 0, 0, 1};
+    } else {
+        if (nl == 2) {
+            const void *f = k == 4 ? ptr<VPL, LPR, 2, 4, 1, false>()
+                          : k == 5 ? ptr<VPL, LPR, 2, 5, 1, false>()
+                                   : ptr<VPL, LPR, 2, 0, 1, false>();
+            return {f, VPL, LPR, 1};
+        }
+        return {ptr<VPL, LPR, 1, 0, 1, false>(), VPL, LPR, 1};
     }
-    if (k == 4) return {tuned<VPL, LPR, 4>(v), VPL, LPR, (v & 1) ? 2 : 1};
-    if (k == 5) return {tuned<VPL, LPR, 5>(v), VPL, LPR, (v & 1) ? 2 : 1};
-    return {ptr<VPL, LPR, 1, 0, 1, false>(), VPL, LPR, 1};
 }
 
-}  // namespace
-
-StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl) {
+StepEntry pick_c128(int rb, int nl, int k, int variant, bool dbl) {
     // 16-byte lanes and up: VPL = 1 << shape (capped by the group size)
     int shape = ((variant % 12) >> 2) & 3;
     while (shape > 0 && (1 << shape) > rb) --shape;
@@ -101,6 +114,23 @@ StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl) {
     }
     return {nullptr, 0, 0, 1};
 }
+
+}  // namespace
+
+#ifdef JKPM_STEP_NTM_UNIT
+StepEntry pick_step_c128_ntm(int rb, int nl, int k, int variant, bool dbl) {
+    return pick_c128(rb, nl, k, variant, dbl);
+}
+#else
+StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl) {
+    if (variant >= 24) {
+        const StepEntry e = pick_step_c128_ntm(rb, nl, k, variant - 24, dbl);
+        if (e.fn) return e;
+        variant -= 24;
+    }
+    return pick_c128(rb, nl, k, variant, dbl);
+}
+#endif
 
 }  // namespace step
 }  // namespace jkpm

@@ -1,4 +1,8 @@
 // complex64 instantiations of the Chebyshev step kernel and their dispatch.
+// Built twice: as itself (NTM = false: every kernel, and pick_step_c64) and,
+// included from cheb_c64_ntm.hip with JKPM_STEP_NTM_UNIT defined, for the
+// tuned shapes with non-temporal matrix loads (pick_step_c64_ntm), so the
+// two halves compile in parallel.
 #include "cheb_step.h"
 
 namespace jkpm {
@@ -7,25 +11,31 @@ namespace {
 
 using Real = float;
 
-template <int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool NS = false>
+#ifdef JKPM_STEP_NTM_UNIT
+constexpr bool kNTM = true;
+#else
+constexpr bool kNTM = false;
+#endif
+
+template <int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool NS = false, bool NTM = false>
 const void *ptr() {
     return reinterpret_cast<const void *>(
-        &cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT, false, NS>);
+        &cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT, false, NS, NTM>);
 }
 
-template <int VPL, int LPR, int KT, int UNR, bool NT, bool NS = false>
+template <int VPL, int LPR, int KT, int UNR, bool NT, bool NS = false, bool NTM = false>
 const void *dbl_ptr() {
     return reinterpret_cast<const void *>(
-        &cheb_step_kernel<Real, VPL, LPR, 2, KT, UNR, NT, true, NS>);
+        &cheb_step_kernel<Real, VPL, LPR, 2, KT, UNR, NT, true, NS, NTM>);
 }
 
 template <int VPL, int LPR, int KT, bool NS>
 const void *dbl_tuned_ns(int v) {
     switch (v & 3) {
-        case 1: return dbl_ptr<VPL, LPR, KT, 2, false, NS>();
-        case 2: return dbl_ptr<VPL, LPR, KT, 1, true, NS>();
-        case 3: return dbl_ptr<VPL, LPR, KT, 2, true, NS>();
-        default: return dbl_ptr<VPL, LPR, KT, 1, false, NS>();
+        case 1: return dbl_ptr<VPL, LPR, KT, 2, false, NS, kNTM>();
+        case 2: return dbl_ptr<VPL, LPR, KT, 1, true, NS, kNTM>();
+        case 3: return dbl_ptr<VPL, LPR, KT, 2, true, NS, kNTM>();
+        default: return dbl_ptr<VPL, LPR, KT, 1, false, NS, kNTM>();
     }
 }
 
@@ -40,16 +50,17 @@ StepEntry dbl_entry(int k, int v) {
     const int u = (v & 1) ? 2 : 1;
     if (k == 4) return {dbl_tuned<VPL, LPR, 4>(v), VPL, LPR, u};
     if (k == 5) return {dbl_tuned<VPL, LPR, 5>(v), VPL, LPR, u};
-    return {dbl_ptr<VPL, LPR, 0, 1, false>(), VPL, LPR, 1};
+    if constexpr (kNTM) return {nullptr, 0, 0, 1};
+    else return {dbl_ptr<VPL, LPR, 0, 1, false>(), VPL, LPR, 1};
 }
 
 template <int VPL, int LPR, int KT, bool NS>
 const void *tuned_ns(int v) {
     switch (v & 3) {
-        case 1: return ptr<VPL, LPR, 1, KT, 2, false, NS>();
-        case 2: return ptr<VPL, LPR, 1, KT, 1, true, NS>();
-        case 3: return ptr<VPL, LPR, 1, KT, 2, true, NS>();
-        default: return ptr<VPL, LPR, 1, KT, 1, false, NS>();
+        case 1: return ptr<VPL, LPR, 1, KT, 2, false, NS, kNTM>();
+        case 2: return ptr<VPL, LPR, 1, KT, 1, true, NS, kNTM>();
+        case 3: return ptr<VPL, LPR, 1, KT, 2, true, NS, kNTM>();
+        default: return ptr<VPL, LPR, 1, KT, 1, false, NS, kNTM>();
     }
 }
 
@@ -59,29 +70,40 @@ const void *tuned(int v) {
 }
 
 // Base kernels of every (left sets, width) and the tuned ones for nl = 1,
-// k = 4, 5.  `variants` = false instantiates only the base kernels.
+// k = 4, 5.  `variants` = false instantiates only the base kernels.  The NTM
+// unit has only the tuned ones (fn = NULL otherwise).
 template <int VPL, int LPR, bool variants>
 StepEntry entry(int nl, int k, int v, bool dbl) {
-    if (dbl) return dbl_entry<VPL, LPR>(k, variants ? v : 0);
-    if (nl == 2) {
-        const void *f = k == 4 ? ptr<VPL, LPR, 2, 4, 1, false>()
-                      : k == 5 ? ptr<VPL, LPR, 2, 5, 1, false>()
-                               : ptr<VPL, LPR, 2, 0, 1, false>();
-        return {f, VPL, LPR, 1};
+    if (dbl) {
+        if constexpr (kNTM && !variants) return {nullptr, 0, 0, 1};
+        else return dbl_entry<VPL, LPR>(k, variants ? v : 0);
     }
     if constexpr (variants) {
-        if (k == 4) return {tuned<VPL, LPR, 4>(v), VPL, LPR, (v & 1) ? 2 : 1};
-        if (k == 5) return {tuned<VPL, LPR, 5>(v), VPL, LPR, (v & 1) ? 2 : 1};
-    } else {
-        if (k == 4) return {ptr<VPL, LPR, 1, 4, 1, false>(), VPL, LPR, 1};
-        if (k == 5) return {ptr<VPL, LPR, 1, 5, 1, false>(), VPL, LPR, 1};
+        if (nl == 1 && k == 4) return {tuned<VPL, LPR, 4>(v), VPL, LPR, (v & 1) ? 2 : 1};
+        if (nl == 1 && k == 5) return {tuned<VPL, LPR, 5>(v), VPL, LPR, (v & 1) ? 2 : 1};
     }
-    return {ptr<VPL, LPR, 1, 0, 1, false>(), VPL, LPR, 1};
+    if constexpr (kNTM) {
+        return {nullptr, 0, 0, 1};
+    } else {
+        if (nl == 2) {
+            const void *f = k == 4 ? ptr<VPL, LPR, 2, 4, 1, false>()
+                          : k == 5 ? ptr<VPL, LPR, 2, 5, 1, false>()
+                                   : ptr<VPL, LPR, 2, 0, 1, false>();
+            return {f, VPL, LPR, 1};
+        }
+        if constexpr (!variants) {
+            if (k == 4) return {ptr<VPL, LPR, 1, 4, 1, false>(), VPL, LPR, 1};
+            if (k == 5) return {ptr<VPL, LPR, 1, 5, 1, false>(), VPL, LPR, 1};
+        }
+        return {ptr<VPL, LPR, 1, 0, 1, false>(), VPL, LPR, 1};
+    }
 }
 
 }  // namespace
 
-StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl) {
+namespace {
+
+StepEntry pick_c64(int rb, bool wide, int nl, int k, int variant, bool dbl) {
     if (wide) {
         // 16-byte lanes and up: VPL = 2 << shape (capped by the group size)
         int shape = ((variant % 12) >> 2) & 3;
@@ -117,6 +139,24 @@ StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl)
     }
     return {nullptr, 0, 0, 1};
 }
+
+}  // namespace
+
+#ifdef JKPM_STEP_NTM_UNIT
+StepEntry pick_step_c64_ntm(int rb, bool wide, int nl, int k, int variant, bool dbl) {
+    if (!wide) return {nullptr, 0, 0, 1};
+    return pick_c64(rb, wide, nl, k, variant, dbl);
+}
+#else
+StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl) {
+    if (variant >= 24) {
+        const StepEntry e = pick_step_c64_ntm(rb, wide, nl, k, variant - 24, dbl);
+        if (e.fn) return e;
+        variant -= 24;
+    }
+    return pick_c64(rb, wide, nl, k, variant, dbl);
+}
+#endif
 
 }  // namespace step
 }  // namespace jkpm

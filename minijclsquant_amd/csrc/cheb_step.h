@@ -14,9 +14,9 @@ constexpr int kXcd = 8;
 template <typename Real, int N>
 using vec_t = Real __attribute__((ext_vector_type(N)));
 
-// Loads of data read once per step (the matrix values and columns, T_{n-1},
-// v_l): non-temporal when NT, so they do not displace the gather window of
-// T_n from the caches.
+// Loads of data read once per step (T_{n-1} and v_l when NT, the matrix
+// values and columns when NTM): non-temporal, so they do not displace the
+// gather window of T_n from the caches.
 template <bool NT, typename V>
 __device__ __forceinline__ V load_stream(const V *p) {
     if constexpr (NT)
@@ -34,8 +34,10 @@ __device__ __forceinline__ V load_stream(const V *p) {
 // then give two moments per step with no v_l stream.  The first step's
 // step-0 partials are <T_0|T_0>.
 // NS: the T_{n+1} stream is written with non-temporal stores.
+// NTM: the matrix values and columns are loaded non-temporally (pays when
+// the step's working set is far beyond the caches, costs when it is not).
 template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool DBL = false,
-          bool NS = false>
+          bool NS = false, bool NTM = false>
 __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const Real *__restrict__ val, const int *__restrict__ col, int n, int k_rt,
     const Real *__restrict__ xcur, const Real *xprev, Real *xout,
@@ -97,8 +99,8 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
 #pragma unroll
                 for (int s = 0; s < KT; ++s) {
                     const long e = (long)s * n + rows[u];
-                    j[u][s] = load_stream<NT>(col + e);
-                    h[u][s] = load_stream<NT>(reinterpret_cast<const C2 *>(val + 2 * e));
+                    j[u][s] = load_stream<NTM>(col + e);
+                    h[u][s] = load_stream<NTM>(reinterpret_cast<const C2 *>(val + 2 * e));
                 }
             V x[UNR][KT];
 #pragma unroll
@@ -120,8 +122,8 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
             for (int u = 0; u < UNR; ++u)
                 for (int s = 0; s < K; ++s) {
                     const long e = (long)s * n + rows[u];
-                    const int j = load_stream<NT>(col + e);
-                    const C2 h = load_stream<NT>(reinterpret_cast<const C2 *>(val + 2 * e));
+                    const int j = load_stream<NTM>(col + e);
+                    const C2 h = load_stream<NTM>(reinterpret_cast<const C2 *>(val + 2 * e));
                     const V x = *reinterpret_cast<const V *>(xcur + (long)j * ld * 2 + voff);
 #pragma unroll
                     for (int v = 0; v < VPL; ++v) {
@@ -259,8 +261,13 @@ struct StepEntry {
 // apply to single-left-set kernels of ELL width 4 and 5; every other shape
 // uses the base kernel.  Returns fn = nullptr for an unsupported group size.
 // dbl selects the doubling form (two partial sets, see DBL above).
+// variant 0..47: 24..47 are 0..23 with non-temporal matrix loads (NTM).
 StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl = false);
 StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl = false);
+// The NTM kernels of the tuned shapes (variant 0..23 here); fn is NULL for
+// shapes that have none.  Defined in their own translation units.
+StepEntry pick_step_c64_ntm(int rb, bool wide, int nl, int k, int variant, bool dbl);
+StepEntry pick_step_c128_ntm(int rb, int nl, int k, int variant, bool dbl);
 
 }  // namespace step
 }  // namespace jkpm

@@ -56,9 +56,9 @@ struct Ell {
     int vtype = JKPM_C128;
     double scale = 1.0;    // the stored values are scale * (uploaded values)
     // Launch shape of the step kernel (jkpm_ell_set_launch / the autotuner):
-    // variant 0..23 = unroll | non-temporal loads << 1 | lane shape << 2,
-    // plus 12 for non-temporal stores of T_{n+1}; chunk_mult work-groups
-    // per CU.
+    // variant 0..47 = unroll | non-temporal loads << 1 | lane shape << 2,
+    // plus 12 for non-temporal stores of T_{n+1}, plus 24 for non-temporal
+    // matrix loads; chunk_mult work-groups per CU.
     int variant = 0;
     int chunk_mult = 8;
     // the same for the doubling form of the recursion
