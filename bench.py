@@ -358,9 +358,11 @@ def bench_grad(args, torch, jcl, nat, _kpm):
     eps = torch.tensor(eps0, device="cuda", requires_grad=True)
     hop = torch.tensor(t0, device="cuda", requires_grad=True)
 
+    seg = [None]  # None: every T_n kept when it fits (it does: 34 GB), else sqrt(M) checkpoints
+
     def forward():
         vals = lattice_values(eps, hop, slot_param)
-        mu = chebyshev_moments(vals, idx, rd, M, scale=1.0 / B)
+        mu = chebyshev_moments(vals, idx, rd, M, scale=1.0 / B, seg=seg[0])
         return (w * kpm_dos_torch(mu, es)).sum()
 
     def both():
@@ -389,6 +391,15 @@ def bench_grad(args, torch, jcl, nat, _kpm):
     out["forward_roofline"] = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "achieved": round(bytes_launch * (M - 1) / (ms_f / 1e3) / 1e9, 1),
                                "note": "forward step bytes x (M-1) over the forward time"}
+    # the same with sqrt(M) checkpoints and recomputation (small memory)
+    seg[0] = 0
+    both()
+    torch.cuda.synchronize()
+    ms_fb_ck = event_ms(torch, both, args.steps)
+    out["backward_mode"] = ("the forward keeps every T_n (M vector batches, 34 GB); the backward "
+                            "runs the adjoint steps and folds only")
+    out["sqrtM_checkpoints"] = {"ms_per_step": ms_fb_ck, "backward_ms": ms_fb_ck - ms_f,
+                                "backward_over_forward": (ms_fb_ck - ms_f) / ms_f}
     print(json.dumps(out), flush=True)
 
 

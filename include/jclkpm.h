@@ -324,9 +324,13 @@ int jkpm_dos(const double *mu, int M, int R, const double *e, int n_e, int kerne
  * derivatives).
  * ------------------------------------------------------------------- */
 
-/* Workspace bytes of jkpm_moments_grad for segment length seg (<= 0:
+/* Workspace bytes of jkpm_moments_grad for segment length seg (0:
  * ceil(sqrt(M-1))): 2*ceil((M-1)/seg) checkpoint vectors, 2*seg segment
- * vectors, 4 rolling vectors (batches of R) and the N*K slot gradient. */
+ * vectors, 4 rolling vectors (batches of R) and the N*K slot gradient.
+ * seg < 0 keeps every T_0 .. T_{M-2} instead (M-1 vectors, no checkpoints,
+ * no recomputation) with ceil(sqrt(M-1)) lambda vectors: one forward pass
+ * fewer, for (M-1) vector batches of memory (34 GB for N = 262,144, R = 8,
+ * complex128, M = 1024). */
 int jkpm_grad_workspace(const jkpm_ell *h, int R, int M, int seg, size_t *bytes);
 
 /* Adjoint of jkpm_moments (one left set, v_l = v_r = r): given
@@ -335,10 +339,33 @@ int jkpm_grad_workspace(const jkpm_ell *h, int R, int M, int seg, size_t *bytes)
  * complex128 values the handle was created from (device, N*K).  Reverse
  * Chebyshev recursion lambda_k = g_k r + 2 A lambda_{k+1} - lambda_{k+2}
  * (A^H = A: Hermitian matrices) with checkpoints every seg steps and
- * recomputation of each segment.  r: device [N][R] of the handle's type. */
+ * recomputation of each segment (seg >= 0), or with every T kept (seg < 0).
+ * r: device [N][R] of the handle's type. */
 int jkpm_moments_grad(const jkpm_ell *h, const void *r, int R, int M, const double *grad_mu,
                       int seg, double *grad_values, void *workspace, size_t workspace_bytes,
                       void *stream);
+
+/* Moments that keep their Chebyshev vectors: as jkpm_moments with one left
+ * set v_l = v_r, and T_0 .. T_{M-1} left in `stack` (device, max(M,2)
+ * vector batches of N*R elements of the handle's type, T_n at n*N*R).
+ * T_{n+1} is written to its own slot instead of over T_{n-1}: the same HBM
+ * traffic per step.  For the adjoint without recomputation
+ * (jkpm_moments_grad_kept).  No reference counterpart (the reference has no
+ * gradients). */
+int jkpm_moments_keep(const jkpm_ell *h, const void *v_r, int R, int M, double *mu,
+                      void *stack, size_t stack_bytes, void *workspace, size_t workspace_bytes,
+                      void *stream);
+
+/* Workspace bytes of jkpm_moments_grad_kept: ceil(sqrt(M-1)) lambda
+ * vectors, 2 rolling vectors and the N*K slot gradient. */
+int jkpm_grad_kept_workspace(const jkpm_ell *h, int R, int M, size_t *bytes);
+
+/* jkpm_moments_grad over the vectors a jkpm_moments_keep call left in
+ * `stack` (same h, r, R, M): the adjoint steps and folds only, no forward
+ * pass.  No reference counterpart. */
+int jkpm_moments_grad_kept(const jkpm_ell *h, const void *r, int R, int M, const double *grad_mu,
+                           const void *stack, double *grad_values, void *workspace,
+                           size_t workspace_bytes, void *stream);
 
 /* Transpose of jkpm_dos: grad_mu[n][r] = (1/R) g_n (2-[n==0])
  * sum_i w[i] cos(n acos e_i) / (pi sqrt(1-e_i^2)) (imaginary part 0), for

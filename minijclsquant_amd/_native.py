@@ -68,6 +68,9 @@ SIGNATURES = {
     "jkpm_profile_read": (_int, [_dp, _c.POINTER(_c.c_longlong), _int]),
     "jkpm_copy": (_int, [_vp, _vp, _size, _vp]),
     "jkpm_ell_create_like": (_int, [_vp, _vp, _dbl, _int, _int, _vp, _c.POINTER(_vp)]),
+    "jkpm_moments_keep": (_int, [_vp, _vp, _int, _int, _vp, _vp, _size, _vp, _size, _vp]),
+    "jkpm_grad_kept_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
+    "jkpm_moments_grad_kept": (_int, [_vp, _vp, _int, _int, _vp, _vp, _vp, _vp, _size, _vp]),
     "jkpm_kubo_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
     "jkpm_kubo_mu_nm": (_int, [_vp, _vp, _vp, _int, _int, _vp, _vp, _size, _vp]),
     "jkpm_kubo_sigma": (_int, [_vp, _int, _vp, _int, _int, _dbl, _vp, _vp]),
@@ -488,6 +491,44 @@ class DeviceELL:
                                   ws.numel(), stream_ptr(stream), _c.byref(best)))
         return float(best.value)
 
+    def moments_keep(self, v_r, M: int, stack=None, stream=None):
+        """(mu [M, R] complex128, stack): the moments <v_r|T_m|v_r> with every
+        T_0 .. T_{M-1} left in stack [max(M,2), N, R] (jkpm_moments_keep)."""
+        torch = _torch()
+        R = self._check_vecs(v_r)
+        M = int(M)
+        shape = (max(M, 2), self.n, R)
+        if stack is None:
+            stack = torch.empty(shape, dtype=v_r.dtype, device=v_r.device)
+        elif tuple(stack.shape) != shape or stack.dtype != v_r.dtype or not stack.is_contiguous():
+            raise ValueError(f"stack must be a contiguous {v_r.dtype} tensor {shape}")
+        ws = torch.empty(self.workspace_bytes(R, 1), dtype=torch.uint8, device=v_r.device)
+        mu = torch.empty((M, R), dtype=torch.complex128, device=v_r.device)
+        check(lib().jkpm_moments_keep(self._h, v_r.data_ptr(), R, M, mu.data_ptr(),
+                                      stack.data_ptr(), stack.numel() * stack.element_size(),
+                                      ws.data_ptr(), ws.numel(), stream_ptr(stream)))
+        return mu, stack
+
+    def grad_values_kept(self, r, M: int, grad_mu, stack, stream=None):
+        """grad_values from the vectors moments_keep left in stack: the
+        adjoint without a forward pass (jkpm_moments_grad_kept)."""
+        torch = _torch()
+        R = self._check_vecs(r)
+        M = int(M)
+        g = grad_mu.to(dtype=torch.complex128).contiguous()
+        if g.shape != (M, R) or not g.is_cuda:
+            raise ValueError(f"grad_mu must be a complex128 device tensor [M={M}, R={R}]")
+        if tuple(stack.shape) != (max(M, 2), self.n, R) or not stack.is_contiguous():
+            raise ValueError("stack must be the one moments_keep filled for these r and M")
+        need = _size()
+        check(lib().jkpm_grad_kept_workspace(self._h, R, M, _c.byref(need)))
+        ws = torch.empty(int(need.value), dtype=torch.uint8, device=r.device)
+        out = torch.empty(self.n * self.k, dtype=torch.complex128, device=r.device)
+        check(lib().jkpm_moments_grad_kept(self._h, r.data_ptr(), R, M, g.data_ptr(),
+                                           stack.data_ptr(), out.data_ptr(), ws.data_ptr(),
+                                           ws.numel(), stream_ptr(stream)))
+        return out
+
     def grad_workspace_bytes(self, R: int, M: int, seg: int = 0) -> int:
         out = _size()
         check(lib().jkpm_grad_workspace(self._h, int(R), int(M), int(seg), _c.byref(out)))
@@ -496,7 +537,8 @@ class DeviceELL:
     def grad_values(self, r, M: int, grad_mu, seg: int = 0, stream=None):
         """dL/dRe(values) + i dL/dIm(values) for the row-major complex128
         values this handle was made from, given grad_mu [M, R] (complex128)
-        of mu = moments(r, M)[0].  seg: checkpoint spacing (0: sqrt(M-1))."""
+        of mu = moments(r, M)[0].  seg: checkpoint spacing (0: sqrt(M-1));
+        < 0 keeps every T_n (no recomputation, M-1 vector batches)."""
         torch = _torch()
         R = self._check_vecs(r)
         M = int(M)

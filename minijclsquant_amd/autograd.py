@@ -30,11 +30,16 @@ __all__ = ["chebyshev_moments", "kpm_dos_torch", "lattice_values", "param_slots"
 
 class _Moments(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, values, indices, r, M, scale, seg):
+    def forward(ctx, values, indices, r, M, scale, seg, keep):
         N = r.shape[0]
         K = values.numel() // N
         h = nat.DeviceELL(values.detach().contiguous(), indices, N, K, scale, "c128")
-        mu = h.moments(r, M)[0]  # [M, R]
+        ctx.stack = None
+        if keep:
+            # the forward keeps every T_n: the backward runs no forward pass
+            mu, ctx.stack = h.moments_keep(r, M)
+        else:
+            mu = h.moments(r, M)[0]  # [M, R]
         ctx.h = h
         ctx.M = int(M)
         ctx.seg = int(seg)
@@ -45,18 +50,43 @@ class _Moments(torch.autograd.Function):
     def backward(ctx, grad):
         (r,) = ctx.saved_tensors
         g = grad.transpose(0, 1).contiguous()
-        gv = ctx.h.grad_values(r, ctx.M, g, ctx.seg)
-        return gv, None, None, None, None, None
+        if ctx.stack is not None:
+            gv = ctx.h.grad_values_kept(r, ctx.M, g, ctx.stack)
+            ctx.stack = None
+            return gv, None, None, None, None, None, None
+        seg = ctx.seg
+        if seg == _AUTO:
+            # keep every T_n when that fits in half the free device memory
+            R = r.shape[1]
+            need = ctx.h.grad_workspace_bytes(R, ctx.M, -1)
+            free, _ = torch.cuda.mem_get_info(r.device)
+            seg = -1 if need <= free // 2 else 0
+        gv = ctx.h.grad_values(r, ctx.M, g, seg)
+        return gv, None, None, None, None, None, None
 
 
-def chebyshev_moments(values, indices, r, M, scale=1.0, seg=0):
+_AUTO = -(2**31)
+
+
+def chebyshev_moments(values, indices, r, M, scale=1.0, seg=None):
     """mu [R, M] complex128 of the ELL (values, indices) times scale.
 
     values: device complex128 [N*K] (row-major ELL slots); indices: device
-    int32 [N*K]; r: device complex128 [N, R] (row-interleaved vectors)."""
+    int32 [N*K]; r: device complex128 [N, R] (row-interleaved vectors).
+    seg (backward): checkpoints every seg steps with recomputation (0:
+    sqrt(M-1)), every T_n kept by the backward's own forward pass (< 0), or
+    None: when a gradient will be taken and max(M,2) vector batches fit in
+    half the free device memory, the forward itself keeps every T_n and the
+    backward runs no forward pass; otherwise sqrt(M-1) checkpoints."""
     if r.dtype != torch.complex128:
         raise ValueError("the differentiable path runs in complex128")
-    return _Moments.apply(values, indices, r, int(M), float(scale), int(seg))
+    keep = False
+    if seg is None and torch.is_grad_enabled() and values.requires_grad:
+        need = max(int(M), 2) * r.numel() * r.element_size()
+        free, _ = torch.cuda.mem_get_info(r.device)
+        keep = need <= free // 2
+    return _Moments.apply(values, indices, r, int(M), float(scale),
+                          _AUTO if seg is None else int(seg), keep)
 
 
 class _Dos(torch.autograd.Function):

@@ -152,9 +152,13 @@ int device_info(DeviceInfo *out) {
 // handle is a block-diagonal batch and mu is [n_left][G][M][R].  With
 // doubling (v_l = v_r, Hermitian A), S = max(1, M/2) steps produce the M
 // moments through the raw dots <T_k|T_k>, <T_k|T_{k+1}>.
+// stack (plain form only): T_n is written to stack + n * (N*R elements)
+// instead of the two rotating buffers, so every T_1 .. T_{M-1} stays (the
+// caller puts T_0 in slot 0).
 static int run_moments(const Ell &H, const void *vr, const void *vl0, const void *vl1, int R,
                        int M, double2 *mu, void *ws, size_t ws_bytes, hipStream_t s,
-                       const Groups *grp = nullptr, bool doubling = false) {
+                       const Groups *grp = nullptr, bool doubling = false,
+                       void *stack = nullptr) {
     DeviceInfo dev;
     int rc = device_info(&dev);
     if (rc) return rc;
@@ -194,9 +198,17 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
         StepArgs a{};
         a.H = &H;
         a.first = (n == 1);
-        a.xcur = (n == 1) ? vr : B[(n - 1) & 1];
-        a.xprev = (n == 1) ? nullptr : (n == 2 ? vr : B[n & 1]);
-        a.xout = B[n & 1];
+        if (stack) {
+            const size_t vb = (size_t)H.n * R * cplx_bytes(H.vtype);
+            char *st = (char *)stack;
+            a.xcur = (n == 1) ? vr : (const void *)(st + (size_t)(n - 1) * vb);
+            a.xprev = (n == 1) ? nullptr : (n == 2 ? vr : (const void *)(st + (size_t)(n - 2) * vb));
+            a.xout = st + (size_t)n * vb;
+        } else {
+            a.xcur = (n == 1) ? vr : B[(n - 1) & 1];
+            a.xprev = (n == 1) ? nullptr : (n == 2 ? vr : B[n & 1]);
+            a.xout = B[n & 1];
+        }
         a.vl0 = doubling ? nullptr : vl0;
         a.vl1 = doubling ? nullptr : vl1;
         a.doubling = doubling;
@@ -768,6 +780,26 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
                        (hipStream_t)stream);
 }
 
+int jkpm_moments_keep(const jkpm_ell *h, const void *v_r, int R, int M, double *mu,
+                      void *stack, size_t stack_bytes, void *workspace, size_t workspace_bytes,
+                      void *stream) {
+    clear_error();
+    if (!h) return arg_error("handle is NULL");
+    if (!v_r || !mu || !stack || !workspace) return arg_error("NULL vector/mu/stack/workspace");
+    if (R <= 0) return arg_error("R must be positive");
+    if (M <= 0) return arg_error("M must be positive");
+    if (!aligned16(v_r) || !aligned16(stack) || !aligned16(workspace) || !aligned16(mu))
+        return arg_error("device pointers must be 16-byte aligned");
+    const size_t vb = (size_t)h->e.n * R * cplx_bytes(h->e.vtype);
+    const size_t need = (size_t)std::max(M, 2) * vb;
+    if (stack_bytes < need)
+        return arg_error("stack too small: need " + std::to_string(need) + " bytes");
+    hipStream_t s = (hipStream_t)stream;
+    JKPM_HIP(hipMemcpyAsync(stack, v_r, vb, hipMemcpyDeviceToDevice, s));
+    return run_moments(h->e, v_r, v_r, nullptr, R, M, (double2 *)mu, workspace, workspace_bytes,
+                       s, nullptr, false, stack);
+}
+
 // ---- launch shape and autotuner ----
 
 namespace {
@@ -1203,11 +1235,17 @@ struct GradLayout {
     size_t ckpt = 0, F0 = 0, F1 = 0, tseg = 0, lam = 0, C1 = 0, C2 = 0, G = 0, total = 0;
     size_t vec = 0;
     int S = 1, nseg = 0, nchunks = 1;
+    bool keep_all = false;
 };
 
+// seg > 0: checkpoint pairs every seg steps, T recomputed per segment;
+// seg == 0: the same with seg = ceil(sqrt(M-1));
+// seg < 0: every T_0 .. T_{M-2} kept (tseg holds them all, no checkpoints
+// and no recomputation), lambda still folded in segments of ceil(sqrt(M-1)).
 GradLayout grad_layout(const Ell &H, int R, int M, int seg, const DeviceInfo &dev) {
     GradLayout L;
     L.vec = align_up((size_t)H.n * R * cplx_bytes(H.vtype), 256);
+    L.keep_all = seg < 0;
     int S = seg > 0 ? seg : 1;
     if (seg <= 0)
         while ((long)S * S < (long)(M - 1)) ++S;  // ceil(sqrt(M - 1))
@@ -1216,13 +1254,13 @@ GradLayout grad_layout(const Ell &H, int R, int M, int seg, const DeviceInfo &de
     L.nchunks = step_chunks(H, R, dev);
     size_t o = 0;
     L.ckpt = o;
-    o += (size_t)2 * L.nseg * L.vec;
+    if (!L.keep_all) o += (size_t)2 * L.nseg * L.vec;
     L.F0 = o;
-    o += L.vec;
+    if (!L.keep_all) o += L.vec;
     L.F1 = o;
-    o += L.vec;
+    if (!L.keep_all) o += L.vec;
     L.tseg = o;
-    o += (size_t)S * L.vec;
+    o += (size_t)(L.keep_all ? std::max(M - 1, 1) : S) * L.vec;
     L.lam = o;
     o += (size_t)S * L.vec;
     L.C1 = o;
@@ -1233,6 +1271,41 @@ GradLayout grad_layout(const Ell &H, int R, int M, int seg, const DeviceInfo &de
     o += align_up((size_t)H.n * H.k * sizeof(double2), 256);
     L.total = o;
     return L;
+}
+}  // namespace
+
+namespace {
+// Reverse sweep over every T_0 .. T_{M-2} kept in `stack` (vector stride
+// N*R elements): segments of L.S lambda vectors, lambda_lo .. lambda_hi by
+// the adjoint steps, then one fold against T_{lo-1} .. T_{hi-1}.  Uses the
+// workspace regions lam, C1, C2 and G of L (G zeroed by the caller).
+int reverse_sweep_kept(const Ell &H, const void *r, int R, int M, const double2 *gmu,
+                       const void *stack, const GradLayout &L, char *w, hipStream_t s) {
+    const size_t vb = (size_t)H.n * R * cplx_bytes(H.vtype);
+    const int S = L.S;
+    auto lm = [&](int q) { return (void *)(w + L.lam + (size_t)q * vb); };
+    auto ts = [&](int q) { return (const void *)((const char *)stack + (size_t)q * vb); };
+    void *C1 = w + L.C1, *C2 = w + L.C2;
+    double2 *G = (double2 *)(w + L.G);
+    int rc = launch_zero((double2 *)C1, (long)(vb / 16), s);
+    if (!rc) rc = launch_zero((double2 *)C2, (long)(vb / 16), s);
+    if (rc) return rc;
+    for (int c = L.nseg - 1; c >= 0; --c) {
+        const int lo = 1 + c * S, hi = std::min((c + 1) * S, M - 1);
+        const int count = hi - lo + 1;
+        for (int k = hi; k >= lo; --k) {
+            const void *x = (k + 1 <= hi) ? lm(k + 1 - lo) : C1;
+            const void *prev = (k + 2 <= hi) ? lm(k + 2 - lo) : (k + 2 == hi + 1 ? C1 : C2);
+            rc = launch_plain_step(H, x, prev, lm(k - lo), gmu + (size_t)k * R, r, R, 2.0, -1.0,
+                                   L.nchunks, s);
+            if (rc) return rc;
+        }
+        rc = launch_grad_fold(H, lm(0), ts(lo - 1), count, lo, G, R, L.nchunks, s);
+        if (rc) return rc;
+        JKPM_HIP(hipMemcpyAsync(C2, count >= 2 ? lm(1) : C1, vb, hipMemcpyDeviceToDevice, s));
+        JKPM_HIP(hipMemcpyAsync(C1, lm(0), vb, hipMemcpyDeviceToDevice, s));
+    }
+    return JKPM_OK;
 }
 }  // namespace
 
@@ -1280,7 +1353,19 @@ int jkpm_moments_grad(const jkpm_ell *h, const void *r, int R, int M, const doub
 
     rc = launch_zero(G, gcount, s);
     if (rc) return rc;
-    if (M >= 2) {
+    if (M >= 2 && L.keep_all) {
+        // Forward pass straight into the stack: T_n in ts(n), n = 0 .. M-2.
+        const int last = M - 2;
+        JKPM_HIP(hipMemcpyAsync(ts(0), r, vb, hipMemcpyDeviceToDevice, s));
+        for (int n = 0; n < last; ++n) {
+            rc = launch_plain_step(H, ts(n), n == 0 ? nullptr : ts(n - 1), ts(n + 1), nullptr,
+                                   nullptr, R, n == 0 ? 1.0 : 2.0, n == 0 ? 0.0 : -1.0,
+                                   L.nchunks, s);
+            if (rc) return rc;
+        }
+        rc = reverse_sweep_kept(H, r, R, M, gmu, w + L.tseg, L, w, s);
+        if (rc) return rc;
+    } else if (M >= 2) {
         // Forward pass: T_0 = r, T_n (n >= 1) in F[n & 1]; keep T_{cS}, T_{cS+1}.
         const int last = M - 2;  // highest T index the gradient needs
         JKPM_HIP(hipMemcpyAsync(ck(0), r, vb, hipMemcpyDeviceToDevice, s));
@@ -1325,6 +1410,56 @@ int jkpm_moments_grad(const jkpm_ell *h, const void *r, int R, int M, const doub
         }
     }
     // the handle holds scale * values, so dL/dvalues = scale * dL/d(handle values)
+    return launch_grad_out(G, H.n, H.k, H.scale, (double2 *)grad_values, s);
+}
+
+int jkpm_grad_kept_workspace(const jkpm_ell *h, int R, int M, size_t *bytes) {
+    clear_error();
+    if (!h || !bytes) return arg_error("NULL argument");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const GradLayout L = grad_layout(h->e, R, M, -1, dev);
+    *bytes = L.total - (L.lam - L.tseg);  // everything but the stack
+    return JKPM_OK;
+}
+
+int jkpm_moments_grad_kept(const jkpm_ell *h, const void *r, int R, int M, const double *grad_mu,
+                           const void *stack, double *grad_values, void *workspace,
+                           size_t workspace_bytes, void *stream) {
+    clear_error();
+    if (!h) return arg_error("handle is NULL");
+    if (!r || !grad_mu || !stack || !grad_values || !workspace) return arg_error("NULL argument");
+    if (R <= 0 || M <= 0) return arg_error("R and M must be positive");
+    if (!aligned16(r) || !aligned16(grad_mu) || !aligned16(stack) || !aligned16(grad_values) ||
+        !aligned16(workspace))
+        return arg_error("device pointers must be 16-byte aligned");
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const Ell &H = h->e;
+    GradLayout L = grad_layout(H, R, M, -1, dev);
+    // this entry's workspace holds no stack: shift the regions after it down
+    const size_t drop = L.lam - L.tseg;
+    L.lam -= drop;
+    L.C1 -= drop;
+    L.C2 -= drop;
+    L.G -= drop;
+    L.total -= drop;
+    if (workspace_bytes < L.total) {
+        set_error("workspace too small: need " + std::to_string(L.total) + " bytes");
+        return JKPM_ERR_WORKSPACE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    char *w = (char *)workspace;
+    double2 *G = (double2 *)(w + L.G);
+    rc = launch_zero(G, (long)H.n * H.k, s);
+    if (rc) return rc;
+    if (M >= 2) {
+        rc = reverse_sweep_kept(H, r, R, M, (const double2 *)grad_mu, stack, L, w, s);
+        if (rc) return rc;
+    }
     return launch_grad_out(G, H.n, H.k, H.scale, (double2 *)grad_values, s);
 }
 

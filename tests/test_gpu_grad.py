@@ -41,7 +41,8 @@ def test_grad_values_vs_oracle(torch_gpu):
     gd = torch.from_numpy(g).cuda()
     want = sum(O.grad_values(M, A.data * (1 / B), A.indices, K, r[k], np.conj(g[:, k]))
                for k in range(R)) * (1 / B)
-    for seg in (0, 1, 3, M):
+    # checkpointed (0, 1, 3, M) and keep-every-T (-1) forms
+    for seg in (0, 1, 3, M, -1):
         got = h.grad_values(rd, M, gd, seg=seg).cpu().numpy()
         err = np.max(np.abs(got - want)) / np.max(np.abs(want))
         assert err <= 1e-10, (seg, err)
@@ -55,11 +56,12 @@ def test_grad_small_M(torch_gpu, M):
     r = jcl.random_vector_generator(N, 1)
     g = np.random.default_rng(M).normal(size=(M, 1)) + 0j
     h = nat.DeviceELL(A.data, A.indices, N, K, 1 / B, "c128")
-    got = h.grad_values(torch.from_numpy(r.reshape(N, 1)).cuda(), M,
-                        torch.from_numpy(g).cuda()).cpu().numpy()
     want = O.grad_values(M, A.data / B, A.indices, K, r, np.conj(g[:, 0])) / B
     scale = max(np.max(np.abs(want)), 1e-300)
-    assert np.max(np.abs(got - want)) / scale <= 1e-10 or np.max(np.abs(want)) == 0
+    for seg in (0, -1):
+        got = h.grad_values(torch.from_numpy(r.reshape(N, 1)).cuda(), M,
+                            torch.from_numpy(g).cuda(), seg=seg).cpu().numpy()
+        assert np.max(np.abs(got - want)) / scale <= 1e-10 or np.max(np.abs(want)) == 0, seg
 
 
 def _loss_setup(torch, L, M, R, n_e, seed):
@@ -141,3 +143,55 @@ def test_dos_grad_is_transpose(torch_gpu):
     gm = nat.dos_grad(w, M, R, e)
     rhs = float((gm.real * mu.real).sum() + (gm.imag * mu.imag).sum())
     assert abs(lhs - rhs) <= 1e-10 * max(1.0, abs(lhs))
+
+
+def test_keep_stack_forward_and_kept_adjoint(torch_gpu):
+    """jkpm_moments_keep leaves T_0 .. T_{M-1} in the stack and the same
+    moments as jkpm_moments; the adjoint over that stack equals the
+    checkpointed adjoint and the oracle's."""
+    torch = torch_gpu
+    A, B = square_params(6, seed=11)
+    N, K, M, R = A.shape[0], A.len_row, 33, 2
+    r = np.stack([jcl.random_vector_generator(N, 70 + s) for s in range(R)])
+    g = np.random.default_rng(5).normal(size=(M, R)) + 1j * np.random.default_rng(6).normal(size=(M, R))
+    h = nat.DeviceELL(A.data, A.indices, N, K, 1 / B, "c128")
+    rd = torch.from_numpy(r.T.copy()).cuda()
+    gd = torch.from_numpy(g).cuda()
+    mu, stack = h.moments_keep(rd, M)
+    assert np.array_equal(mu.cpu().numpy(), h.moments(rd, M)[0].cpu().numpy())
+    # the kept vectors are the Chebyshev vectors of the dense matrix
+    Hd = jcl.ell_to_csr(A).toarray() / B
+    T0, T1 = r.T, Hd @ r.T
+    st = stack.cpu().numpy()
+    assert np.allclose(st[0], T0, rtol=0, atol=1e-14)
+    assert np.allclose(st[1], T1, rtol=0, atol=1e-13)
+    Tm2, Tm1 = T0, T1
+    for n in range(2, M):
+        Tm2, Tm1 = Tm1, 2 * Hd @ Tm1 - Tm2
+    assert np.max(np.abs(st[M - 1] - Tm1)) <= 1e-9 * np.max(np.abs(Tm1))
+    got = h.grad_values_kept(rd, M, gd, stack).cpu().numpy()
+    ref = h.grad_values(rd, M, gd, seg=0).cpu().numpy()
+    assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) <= 1e-12
+    want = sum(O.grad_values(M, A.data * (1 / B), A.indices, K, r[k], np.conj(g[:, k]))
+               for k in range(R)) * (1 / B)
+    assert np.max(np.abs(got - want)) / np.max(np.abs(want)) <= 1e-10
+
+
+def test_autograd_keep_mode_matches_checkpointed(torch_gpu):
+    """chebyshev_moments with seg=None (the forward keeps every T_n) gives the
+    gradients of the sqrt(M)-checkpointed backward."""
+    torch = torch_gpu
+    from minijclsquant_amd.autograd import chebyshev_moments, lattice_values
+
+    A, B, slot_param, eps0, t0, rd, idx, es, w = _loss_setup(torch, 8, 60, 3, 40, 21)
+    grads = []
+    for seg in (None, 0):
+        eps = torch.tensor(eps0, device="cuda", requires_grad=True)
+        hop = torch.tensor(t0, device="cuda", requires_grad=True)
+        vals = lattice_values(eps, hop, slot_param)
+        mu = chebyshev_moments(vals, idx, rd, 60, scale=1.0 / B, seg=seg)
+        wm = torch.linspace(-1.0, 1.0, mu.shape[1], dtype=torch.float64, device="cuda")
+        (mu.real * wm).sum().backward()
+        grads.append((eps.grad.cpu().numpy(), hop.grad.cpu().numpy()))
+    for a, b in zip(grads[0], grads[1]):
+        assert np.max(np.abs(a - b)) <= 1e-12 * max(1.0, np.max(np.abs(b)))
