@@ -30,10 +30,15 @@ __all__ = ["chebyshev_moments", "kpm_dos_torch", "lattice_values", "param_slots"
 
 class _Moments(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, values, indices, r, M, scale, seg, keep):
+    def forward(ctx, values, indices, r, M, scale, seg, keep, tune):
         N = r.shape[0]
         K = values.numel() // N
         h = nat.DeviceELL(values.detach().contiguous(), indices, N, K, scale, "c128")
+        if tune:
+            # launch shape from the persistent table (timed once per shape)
+            from . import autotune
+
+            autotune.tuned(h, r)
         ctx.stack = None
         if keep:
             # the forward keeps every T_n: the backward runs no forward pass
@@ -53,7 +58,7 @@ class _Moments(torch.autograd.Function):
         if ctx.stack is not None:
             gv = ctx.h.grad_values_kept(r, ctx.M, g, ctx.stack)
             ctx.stack = None
-            return gv, None, None, None, None, None, None
+            return gv, None, None, None, None, None, None, None
         seg = ctx.seg
         if seg == _AUTO:
             # keep every T_n when that fits in half the free device memory
@@ -62,13 +67,13 @@ class _Moments(torch.autograd.Function):
             free, _ = torch.cuda.mem_get_info(r.device)
             seg = -1 if need <= free // 2 else 0
         gv = ctx.h.grad_values(r, ctx.M, g, seg)
-        return gv, None, None, None, None, None, None
+        return gv, None, None, None, None, None, None, None
 
 
 _AUTO = -(2**31)
 
 
-def chebyshev_moments(values, indices, r, M, scale=1.0, seg=None):
+def chebyshev_moments(values, indices, r, M, scale=1.0, seg=None, tune=True):
     """mu [R, M] complex128 of the ELL (values, indices) times scale.
 
     values: device complex128 [N*K] (row-major ELL slots); indices: device
@@ -77,7 +82,9 @@ def chebyshev_moments(values, indices, r, M, scale=1.0, seg=None):
     sqrt(M-1)), every T_n kept by the backward's own forward pass (< 0), or
     None: when a gradient will be taken and max(M,2) vector batches fit in
     half the free device memory, the forward itself keeps every T_n and the
-    backward runs no forward pass; otherwise sqrt(M-1) checkpoints."""
+    backward runs no forward pass; otherwise sqrt(M-1) checkpoints.
+    tune: run the forward with the autotuner's launch shape for this
+    (N, K, R) (timed on the first call, then read from its table)."""
     if r.dtype != torch.complex128:
         raise ValueError("the differentiable path runs in complex128")
     keep = False
@@ -86,7 +93,7 @@ def chebyshev_moments(values, indices, r, M, scale=1.0, seg=None):
         free, _ = torch.cuda.mem_get_info(r.device)
         keep = need <= free // 2
     return _Moments.apply(values, indices, r, int(M), float(scale),
-                          _AUTO if seg is None else int(seg), keep)
+                          _AUTO if seg is None else int(seg), keep, bool(tune))
 
 
 class _Dos(torch.autograd.Function):
