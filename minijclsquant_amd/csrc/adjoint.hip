@@ -236,8 +236,22 @@ struct Pick {
     int lpr;
 };
 
+// wide (an even batch R, so rows and groups start on two-vector
+// boundaries): groups of 2+ vectors run two vectors per lane (16-byte
+// complex64 or 32-byte complex128 lanes); otherwise one vector per lane.
+// The step kernel runs wide when it can, the fold never.
 template <typename Real>
-Pick pick_real(int rb, int k) {
+Pick pick_real(int rb, int k, bool wide) {
+    if (wide) {
+        switch (rb) {
+            case 2: return {Fns<Real, 2, 1>::step(), Fns<Real, 2, 1>::fold(k), 1};
+            case 4: return {Fns<Real, 2, 2>::step(), Fns<Real, 2, 2>::fold(k), 2};
+            case 8: return {Fns<Real, 2, 4>::step(), Fns<Real, 2, 4>::fold(k), 4};
+            case 16: return {Fns<Real, 2, 8>::step(), Fns<Real, 2, 8>::fold(k), 8};
+            case 32: return {Fns<Real, 2, 16>::step(), Fns<Real, 2, 16>::fold(k), 16};
+            default: break;
+        }
+    }
     switch (rb) {
         case 1: return {Fns<Real, 1, 1>::step(), Fns<Real, 1, 1>::fold(k), 1};
         case 2: return {Fns<Real, 1, 2>::step(), Fns<Real, 1, 2>::fold(k), 2};
@@ -249,8 +263,8 @@ Pick pick_real(int rb, int k) {
     }
 }
 
-Pick pick(int vtype, int rb, int k) {
-    return vtype == JKPM_C64 ? pick_real<float>(rb, k) : pick_real<double>(rb, k);
+Pick pick(int vtype, int rb, int k, bool wide) {
+    return vtype == JKPM_C64 ? pick_real<float>(rb, k, wide) : pick_real<double>(rb, k, wide);
 }
 
 long rows_per_chunk(int n, int nchunks, int lpr) {
@@ -268,7 +282,7 @@ int launch_plain_step(const Ell &H, const void *x, const void *prev, void *out,
     for (int r0 = 0; r0 < R;) {
         int rb = 32;
         while (rb > R - r0) rb >>= 1;
-        Pick p = pick(H.vtype, rb, H.k);
+        Pick p = pick(H.vtype, rb, H.k, R % 2 == 0);
         if (!p.step) return arg_error("unsupported vector group");
         const size_t off = (size_t)r0 * cb;
         const void *a_val = H.val;
@@ -301,7 +315,9 @@ int launch_grad_fold(const Ell &H, const void *lam, const void *tprev, int count
     for (int r0 = 0; r0 < R;) {
         int rb = 32;
         while (rb > R - r0) rb >>= 1;
-        Pick p = pick(H.vtype, rb, H.k);
+        // one vector per lane: faster for the fold (530 vs 679 us per
+        // 32-moment fold, square 512^2, R = 8, complex128)
+        Pick p = pick(H.vtype, rb, H.k, false);
         if (!p.fold) return arg_error("unsupported vector group");
         const size_t off = (size_t)r0 * cb;
         const int *a_col = H.col;
