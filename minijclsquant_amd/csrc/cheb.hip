@@ -38,7 +38,7 @@ constexpr int kXcd = 8;
 
 // One work-group per (vector, left set x group, step): fixed-order sum of the
 // chunk partials of that step over the group's chunks (all chunks when there
-// are no groups).
+// are no groups).  Used for batches of more than kBlock vectors.
 __global__ __launch_bounds__(kBlock) void reduce_kernel(const double2 *__restrict__ ring,
                                                          int ring_slots, int step_first,
                                                          int n_left, int nchunks, int ld,
@@ -70,6 +70,47 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(const double2 *__restric
     }
     if (threadIdx.x == 0)
         mu[(((long)l * G + g) * M + step) * ld + r] = make_double2(shr[0], shi[0]);
+}
+
+// The same sums with one work-group per (left set x group, step) for all
+// ld <= kBlock vectors at once: thread t adds chunks t/ld, t/ld + kBlock/ld,
+// ... of vector t % ld, so each pass reads kBlock/ld whole chunk rows of the
+// [chunk][ld] partials contiguously; the per-vector totals are then summed
+// over the kBlock/ld thread rows in a fixed order.
+__global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const double2 *__restrict__ ring,
+                                                              int ring_slots, int step_first,
+                                                              int n_left, int nchunks, int ld,
+                                                              double2 *__restrict__ mu, int M,
+                                                              const int *__restrict__ group_chunks,
+                                                              int G) {
+    const int l = blockIdx.x / G, g = blockIdx.x % G;
+    const int step = step_first + blockIdx.y;
+    const double2 *p = ring + ((long)(step % ring_slots) * n_left + l) * (long)nchunks * ld;
+    const int c_begin = group_chunks ? group_chunks[g] : 0;
+    const int c_end = group_chunks ? group_chunks[g + 1] : nchunks;
+    const int rows = kBlock / ld;  // chunk rows per pass
+    const int t = threadIdx.x;
+    const int r = t % ld, c0 = t / ld;
+    double sr = 0.0, si = 0.0;
+    if (c0 < rows) {
+        for (int c = c_begin + c0; c < c_end; c += rows) {
+            const double2 q = p[(long)c * ld + r];
+            sr += q.x;
+            si += q.y;
+        }
+    }
+    __shared__ double shr[kBlock], shi[kBlock];
+    shr[t] = sr;
+    shi[t] = si;
+    __syncthreads();
+    if (t < ld) {
+        double ar = 0.0, ai = 0.0;
+        for (int c = 0; c < rows; ++c) {
+            ar += shr[c * ld + t];
+            ai += shi[c * ld + t];
+        }
+        mu[(((long)l * G + g) * M + step) * ld + t] = make_double2(ar, ai);
+    }
 }
 
 // Row-major complex128 (device) -> slot-major value type, times `scale`.
@@ -224,6 +265,12 @@ int launch_reduce(const double2 *ring, int ring_slots, int step_first, int nstep
                   hipStream_t s) {
     if (nsteps <= 0) return JKPM_OK;
     if (G < 1) G = 1;
+    if (ld <= kBlock) {
+        hipLaunchKernelGGL(reduce_rows_kernel, dim3(n_left * G, nsteps), dim3(kBlock), 0, s, ring,
+                           ring_slots, step_first, n_left, nchunks, ld, mu, M, group_chunks, G);
+        JKPM_CHECK_LAUNCH("reduce_rows_kernel");
+        return JKPM_OK;
+    }
     hipLaunchKernelGGL(reduce_kernel, dim3(ld, n_left * G, nsteps), dim3(kBlock), 0, s, ring,
                        ring_slots, step_first, n_left, nchunks, ld, mu, M, group_chunks, G);
     JKPM_CHECK_LAUNCH("reduce_kernel");
