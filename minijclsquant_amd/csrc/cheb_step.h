@@ -90,6 +90,12 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
         for (int u = 0; u < UNR; ++u)
 #pragma unroll
             for (int v = 0; v < VPL; ++v) sr[u][v] = si[u][v] = Real(0);
+        // doubling form: T_n of the row itself, taken from the gather of the
+        // diagonal slot when the row has one (no second read of it)
+        V own[UNR];
+        bool have_own[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) have_own[u] = false;
 
         if constexpr (KT > 0) {
             int j[UNR][KT];
@@ -108,6 +114,16 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
 #pragma unroll
                 for (int s = 0; s < KT; ++s)
                     x[u][s] = *reinterpret_cast<const V *>(xcur + (long)j[u][s] * ld * 2 + voff);
+            if constexpr (DBL) {
+#pragma unroll
+                for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                    for (int s = 0; s < KT; ++s)
+                        if (j[u][s] == (int)rows[u]) {
+                            own[u] = x[u][s];
+                            have_own[u] = true;
+                        }
+            }
 #pragma unroll
             for (int u = 0; u < UNR; ++u)
 #pragma unroll
@@ -143,7 +159,12 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
             else
                 p[u] = load_stream<NT>(reinterpret_cast<const V *>(xprev + vo));
             if constexpr (DBL) {
-                if (!first) q[u][1] = *reinterpret_cast<const V *>(xcur + vo);  // T_n, cached
+                if (!first) {
+                    if (have_own[u])
+                        q[u][1] = own[u];
+                    else
+                        q[u][1] = *reinterpret_cast<const V *>(xcur + vo);  // T_n, cached
+                }
             } else {
 #pragma unroll
                 for (int l = 0; l < NL; ++l)
