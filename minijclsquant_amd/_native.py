@@ -563,17 +563,30 @@ class DeviceELL:
                                stream_ptr(stream)))
 
 
-def dos(mu, energies_scaled, kernel: int = JKPM_KERNEL_JACKSON, stream=None):
-    """DOS on device from complex128 moments mu [M, R] (averaged over R)."""
+def dos(mu, energies_scaled, kernel: int = JKPM_KERNEL_JACKSON, stream=None, out=None,
+        workspace=None, check_range: bool = True):
+    """DOS on device from complex128 moments mu [M, R] (averaged over R).
+    out (float64 [n_e]) and workspace (float64 [M]) may be given, and the
+    range check of the energies (a device-to-host read) skipped, so that the
+    call can be captured into a HIP graph."""
     torch = _torch()
     if mu.dtype != torch.complex128 or mu.dim() != 2 or not mu.is_contiguous():
         raise ValueError("mu must be contiguous complex128 [M, R]")
-    e = energies_scaled.to(device=mu.device, dtype=torch.float64).contiguous()
-    if e.numel() and float(e.abs().max()) >= 1.0:
+    e = energies_scaled
+    if e.device != mu.device or e.dtype != torch.float64 or not e.is_contiguous():
+        e = e.to(device=mu.device, dtype=torch.float64).contiguous()
+    if check_range and e.numel() and float(e.abs().max()) >= 1.0:
         raise ValueError("rescaled energies must lie strictly inside (-1, 1)")
     M, R = mu.shape
-    out = torch.empty(e.numel(), dtype=torch.float64, device=mu.device)
-    ws = torch.empty(M, dtype=torch.float64, device=mu.device)
+    if out is None:
+        out = torch.empty(e.numel(), dtype=torch.float64, device=mu.device)
+    elif out.numel() != e.numel() or out.dtype != torch.float64 or not out.is_contiguous():
+        raise ValueError("out must be contiguous float64 with one entry per energy")
+    ws = workspace
+    if ws is None:
+        ws = torch.empty(M, dtype=torch.float64, device=mu.device)
+    elif ws.numel() < M or ws.dtype != torch.float64:
+        raise ValueError("workspace must be float64 with at least M entries")
     check(lib().jkpm_dos(mu.data_ptr(), M, R, e.data_ptr(), e.numel(), int(kernel),
                          out.data_ptr(), ws.data_ptr(), ws.numel() * 8, stream_ptr(stream)))
     return out
