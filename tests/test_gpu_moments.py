@@ -329,3 +329,31 @@ def test_precision_form_full_size(torch_gpu):
     err = np.max(np.abs(d64 - d128)) / np.max(np.abs(d128))
     assert err <= 1e-5, err
     del torch
+
+
+def test_size_limits_and_degenerate_shapes(torch_gpu):
+    """Inputs past the 2^31 - 1 slot limit are refused before any
+    allocation; a one-row matrix and M = 1, 2 still give the oracle's
+    moments."""
+    torch = torch_gpu
+    lib = nat.lib()
+    data = np.zeros(8, np.complex128)
+    idx = np.zeros(8, np.int32)
+    h = nat._vp()
+    rc = lib.jkpm_ell_create(data.ctypes.data, idx.ctypes.data, 1 << 20, 1 << 12, 1.0,
+                             nat.JKPM_C128, 0, None, nat._c.byref(h))
+    assert rc == nat.JKPM_ERR_ARG and "2^31" in lib.jkpm_last_error().decode()
+    # CSR whose requested width makes n_rows * len_row too large
+    ip = np.array([0, 1, 2, 3, 4], np.int32)
+    ci = np.array([0, 1, 2, 3], np.int32)
+    dv = np.ones(4, np.complex128)
+    with pytest.raises(ValueError, match="2\\^31"):
+        nat.DeviceELL.from_csr(ip, ci, dv, 4, len_row=1 << 30)
+    # one row, one slot: T_n = a^n r
+    a = 0.3 - 0.4j
+    h1 = nat.DeviceELL(np.array([a]), np.array([0], np.int32), 1, 1, 1.0, "c128")
+    v = torch.tensor([[1.0 + 0j]], dtype=torch.complex128, device="cuda")
+    for M in (1, 2, 7):
+        mu = h1.moments(v, M)[0].cpu().numpy()[:, 0]
+        ref = O.rec_A_tab(M, np.array([a]), np.array([0], np.int32), 1, np.array([1.0 + 0j]))
+        assert np.allclose(mu, ref, rtol=0, atol=1e-14), M

@@ -9,7 +9,6 @@ through a throw-away module registered only for the dump.  ``load_ell`` must
 read it back as this package's ell_matrix, and must refuse any stream that
 names a constructor outside its list, without running it.
 """
-import io
 import os
 import pickle
 import sys
@@ -18,7 +17,6 @@ import types
 import joblib
 import numpy as np
 import pytest
-import scipy.sparse as sp
 
 import minijclsquant_amd as jcl
 
