@@ -557,7 +557,7 @@ def main():
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the KPM path "
               "shards over vectors and disorder realisations with one final all-reduce "
-              "of the moments, which this build does not run", file=sys.stderr)
+              "of the moments, which this package does not run", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -177,9 +177,9 @@ def modifier_hoppings_c(A, data_new, dx_vec_new, dy_vec_new, modifier_id=None,
                         modifier_params=None, t=None):
     """Real-space hopping modifier of the light modes
     (jclsquant/modifiers.py:362-443); the phonon mode and k-space matrices
-    are not part of this build."""
+    are not part of this package."""
     if getattr(A, "space", "r") == "k":
-        raise NotImplementedError("k-space hopping modifiers are not part of this build")
+        raise NotImplementedError("k-space hopping modifiers are not part of this package")
     if modifier_id == "phonons_packed":
-        raise NotImplementedError("the phonon hopping modifier is not part of this build")
+        raise NotImplementedError("the phonon hopping modifier is not part of this package")
     return modifier_light(A, data_new, modifier_id, modifier_params, t)

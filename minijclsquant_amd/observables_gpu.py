@@ -110,7 +110,7 @@ def kpm_rho_neq_gpu(H, t_vec=None, tau=0, modifier_id=None, modifier_params=None
     from .modifiers import modifier_bounds, modifier_hoppings_c
 
     if getattr(H, "space", "r") == "k":
-        raise NotImplementedError("k-space time evolution is not part of this build")
+        raise NotImplementedError("k-space time evolution is not part of this package")
     if M is None:
         M = int(np.sqrt(H.shape[0]))
     if Ef is None:
@@ -135,7 +135,7 @@ def kpm_rho_neq_gpu(H, t_vec=None, tau=0, modifier_id=None, modifier_params=None
               'linear packed , circle packed .')
         return None
     if modifier_id not in ("circle_packed", "linear_packed"):
-        raise NotImplementedError(f"modifier {modifier_id!r} is not part of this build")
+        raise NotImplementedError(f"modifier {modifier_id!r} is not part of this package")
     t_vec = np.asarray(t_vec)
     moments_Uv, _ = _time_moments(t_vec, H, M)
     H_kpm = H.modifier(modifier_bounds)

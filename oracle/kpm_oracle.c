@@ -3,7 +3,7 @@
  * restatement of the reference algorithm in complex128.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * cpu_baseline leg of bench.py may load this library, and only to check results
  * (or the timed CPU baseline) -- never as the product path.  The product path
  * is the HIP library in minijclsquant_amd/ and fails loudly without it.
  *

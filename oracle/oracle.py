@@ -1,7 +1,7 @@
 """Python face of the CPU oracle (libkpm_oracle.so built from kpm_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.  The
+cpu_baseline leg of bench.py, to check results or as the timed CPU baseline.  The
 product (minijclsquant_amd) never imports it.
 
 Every function restates the reference algorithm in complex128; see the
