@@ -1539,22 +1539,25 @@ int jkpm_kubo_mu_nm(const jkpm_ell *H, const jkpm_ell *V, const void *r, int R, 
     const size_t vec = (size_t)H->e.n * R * 8;
     char *Ls = w + K.L, *Rs = w + K.Rk;
     void *Tb[2] = {w + K.T0, w + K.T1};
-    // Right recursion T_n(H) r (T_0 = r, T_n in Tb[n & 1]); the fused dual
-    // output is Rk_n = v T_n.
-    for (int n = 0; n < M; ++n) {
-        const void *x = n == 0 ? r : Tb[n & 1];
-        const void *prev = n == 1 ? r : Tb[(n + 1) & 1];
-        rc = launch_kubo_step(H->e, &V->e, x, n == 0 ? nullptr : prev, Tb[(n + 1) & 1],
-                              Rs + (size_t)n * vec, R, n == 0 ? 1.f : 2.f, n == 0 ? 0.f : -1.f,
-                              K.nchunks, s);
-        if (rc) return rc;
-    }
-    // Left recursion L_m = T_m(H) (v r), L_0 = Rk_0, out of place into the stack.
+    // Right recursion T_n(H) r (T_0 = r, T_n in Tb[n & 1]) with the fused dual
+    // output Rk_n = v T_n, and the left recursion L_m = T_m(H) (v r), L_0 =
+    // Rk_0, out of place into its stack.  Step n >= 1 advances both on one
+    // pass over the slots: the right one to T_{n+1}, Rk_n and the left one to
+    // L_n.
+    rc = launch_kubo_step(H->e, &V->e, r, nullptr, Tb[1], Rs, R, 1.f, 0.f, K.nchunks, s);
+    if (rc) return rc;
     JKPM_HIP(hipMemcpyAsync(Ls, Rs, vec, hipMemcpyDeviceToDevice, s));
-    for (int m = 0; m + 1 < M; ++m) {
-        rc = launch_kubo_step(H->e, nullptr, Ls + (size_t)m * vec,
-                              m == 0 ? nullptr : Ls + (size_t)(m - 1) * vec, Ls + (size_t)(m + 1) * vec,
-                              nullptr, R, m == 0 ? 1.f : 2.f, m == 0 ? 0.f : -1.f, K.nchunks, s);
+    for (int n = 1; n < M; ++n) {
+        const int m = n - 1;
+        KuboLeft left;
+        left.x = Ls + (size_t)m * vec;
+        left.prev = m == 0 ? nullptr : Ls + (size_t)(m - 1) * vec;
+        left.out = Ls + (size_t)(m + 1) * vec;
+        left.alpha = m == 0 ? 1.f : 2.f;
+        left.beta = m == 0 ? 0.f : -1.f;
+        rc = launch_kubo_step(H->e, &V->e, Tb[n & 1], n == 1 ? r : Tb[(n + 1) & 1],
+                              Tb[(n + 1) & 1], Rs + (size_t)n * vec, R, 2.f, -1.f, K.nchunks, s,
+                              &left);
         if (rc) return rc;
     }
     rc = launch_kubo_gemm((const float2 *)Ls, (const float2 *)Rs, (long)H->e.n * R, M, K.nslices,

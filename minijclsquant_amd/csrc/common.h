@@ -202,10 +202,20 @@ int run_series(const Ell &H, const Ell *V, const void *v, int R, int M, const do
 
 // ---- Kubo double moments (kubo.hip) ----
 int kubo_gemm_slices(long K, int M, const DeviceInfo &dev);
+// A second recursion run on the same slot loads as a dual Kubo step:
+// out = alpha*H*x + beta*prev (prev unused when beta == 0).
+struct KuboLeft {
+    const void *x = nullptr;
+    const void *prev = nullptr;
+    void *out = nullptr;
+    float alpha = 1.f, beta = 0.f;
+};
 // out = alpha*H*x + beta*prev (prev unused when beta == 0); when Vop is set,
-// also wout = Vop*x on the same gathered x (Vop shares H's columns).
+// also wout = Vop*x on the same gathered x (Vop shares H's columns), and when
+// left is set (Vop required) the left recursion as well.
 int launch_kubo_step(const Ell &H, const Ell *Vop, const void *x, const void *prev, void *out,
-                     void *wout, int R, float alpha, float beta, int nchunks, hipStream_t s);
+                     void *wout, int R, float alpha, float beta, int nchunks, hipStream_t s,
+                     const KuboLeft *left = nullptr);
 int launch_kubo_gemm(const float2 *L, const float2 *Rk, long K, int M, int nslices, float2 *part,
                      hipStream_t s);
 int launch_kubo_reduce(const float2 *part, int nslices, int M, double scale, double2 *mu,

@@ -41,11 +41,19 @@ using floatx4 = vec_t<float, 4>;
 
 // ------------------------------------------------------------ stack steps
 
-template <int VPL, int LPR, bool DUAL>
+// LEFT (with DUAL only): a second, independent recursion on the same column
+// and H-value loads, outl = alpha_l H xl + beta_l prevl (prevl unused when
+// beta_l == 0).  The Kubo driver runs the right recursion (T_n, v T_n) and the
+// left one (L_m) side by side this way, so the slots are read once per step
+// for both.
+template <int VPL, int LPR, bool DUAL, bool LEFT>
 __global__ __launch_bounds__(kBlock) void kubo_step_kernel(
     const float *__restrict__ hval, const float *__restrict__ vval, const int *__restrict__ col,
     int n, int k, const float *__restrict__ x, const float *prev, float *out,
-    float *__restrict__ wout, int ld, float alpha, float beta, int rows_per_chunk, int nchunks) {
+    float *__restrict__ wout, const float *__restrict__ xl, const float *prevl,
+    float *__restrict__ outl, int ld, float alpha, float beta, float alpha_l, float beta_l,
+    int rows_per_chunk, int nchunks) {
+    static_assert(DUAL || !LEFT, "the left recursion rides on the dual step");
     using V = vec_t<float, 2 * VPL>;
     using C2 = vec_t<float, 2>;
     constexpr int RPW = 64 / LPR;
@@ -57,9 +65,9 @@ __global__ __launch_bounds__(kBlock) void kubo_step_kernel(
     const long row_end = min((long)n, row_begin + rows_per_chunk);
     const int voff = lv * VPL * 2;
     for (long row = row_begin + wave * RPW + lr; row < row_end; row += kWaves * RPW) {
-        float sr[VPL], si[VPL], wr[VPL], wi[VPL];
+        float sr[VPL], si[VPL], wr[VPL], wi[VPL], qr[VPL], qi[VPL];
 #pragma unroll
-        for (int v = 0; v < VPL; ++v) sr[v] = si[v] = wr[v] = wi[v] = 0.f;
+        for (int v = 0; v < VPL; ++v) sr[v] = si[v] = wr[v] = wi[v] = qr[v] = qi[v] = 0.f;
         for (int s = 0; s < k; ++s) {
             const long e = (long)s * n + row;
             const int j = col[e];
@@ -67,6 +75,8 @@ __global__ __launch_bounds__(kBlock) void kubo_step_kernel(
             const V xv = *reinterpret_cast<const V *>(x + (long)j * ld * 2 + voff);
             C2 w;
             if constexpr (DUAL) w = *reinterpret_cast<const C2 *>(vval + 2 * e);
+            V xq;
+            if constexpr (LEFT) xq = *reinterpret_cast<const V *>(xl + (long)j * ld * 2 + voff);
 #pragma unroll
             for (int v = 0; v < VPL; ++v) {
                 sr[v] += h.x * xv[2 * v] - h.y * xv[2 * v + 1];
@@ -74,6 +84,10 @@ __global__ __launch_bounds__(kBlock) void kubo_step_kernel(
                 if constexpr (DUAL) {
                     wr[v] += w.x * xv[2 * v] - w.y * xv[2 * v + 1];
                     wi[v] += w.x * xv[2 * v + 1] + w.y * xv[2 * v];
+                }
+                if constexpr (LEFT) {
+                    qr[v] += h.x * xq[2 * v] - h.y * xq[2 * v + 1];
+                    qi[v] += h.x * xq[2 * v + 1] + h.y * xq[2 * v];
                 }
             }
         }
@@ -103,13 +117,32 @@ __global__ __launch_bounds__(kBlock) void kubo_step_kernel(
             }
             *reinterpret_cast<V *>(wout + vo) = wv;
         }
+        if constexpr (LEFT) {
+            V yl;
+            if (beta_l != 0.f) {
+                const V p = *reinterpret_cast<const V *>(prevl + vo);
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    yl[2 * v] = alpha_l * qr[v] + beta_l * p[2 * v];
+                    yl[2 * v + 1] = alpha_l * qi[v] + beta_l * p[2 * v + 1];
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    yl[2 * v] = alpha_l * qr[v];
+                    yl[2 * v + 1] = alpha_l * qi[v];
+                }
+            }
+            *reinterpret_cast<V *>(outl + vo) = yl;
+        }
     }
 }
 
 template <int VPL, int LPR>
-const void *step_fn(bool dual) {
-    return dual ? reinterpret_cast<const void *>(&kubo_step_kernel<VPL, LPR, true>)
-                : reinterpret_cast<const void *>(&kubo_step_kernel<VPL, LPR, false>);
+const void *step_fn(bool dual, bool left) {
+    if (left) return reinterpret_cast<const void *>(&kubo_step_kernel<VPL, LPR, true, true>);
+    return dual ? reinterpret_cast<const void *>(&kubo_step_kernel<VPL, LPR, true, false>)
+                : reinterpret_cast<const void *>(&kubo_step_kernel<VPL, LPR, false, false>);
 }
 
 struct KStep {
@@ -117,24 +150,24 @@ struct KStep {
     int lpr;
 };
 
-KStep pick(int rb, bool wide, bool dual) {
+KStep pick(int rb, bool wide, bool dual, bool left) {
     if (wide) {
         switch (rb) {
-            case 2: return {step_fn<2, 1>(dual), 1};
-            case 4: return {step_fn<2, 2>(dual), 2};
-            case 8: return {step_fn<2, 4>(dual), 4};
-            case 16: return {step_fn<2, 8>(dual), 8};
-            case 32: return {step_fn<2, 16>(dual), 16};
+            case 2: return {step_fn<2, 1>(dual, left), 1};
+            case 4: return {step_fn<2, 2>(dual, left), 2};
+            case 8: return {step_fn<2, 4>(dual, left), 4};
+            case 16: return {step_fn<2, 8>(dual, left), 8};
+            case 32: return {step_fn<2, 16>(dual, left), 16};
             default: break;
         }
     }
     switch (rb) {
-        case 1: return {step_fn<1, 1>(dual), 1};
-        case 2: return {step_fn<1, 2>(dual), 2};
-        case 4: return {step_fn<1, 4>(dual), 4};
-        case 8: return {step_fn<1, 8>(dual), 8};
-        case 16: return {step_fn<1, 16>(dual), 16};
-        case 32: return {step_fn<1, 32>(dual), 32};
+        case 1: return {step_fn<1, 1>(dual, left), 1};
+        case 2: return {step_fn<1, 2>(dual, left), 2};
+        case 4: return {step_fn<1, 4>(dual, left), 4};
+        case 8: return {step_fn<1, 8>(dual, left), 8};
+        case 16: return {step_fn<1, 16>(dual, left), 16};
+        case 32: return {step_fn<1, 32>(dual, left), 32};
         default: break;
     }
     return {nullptr, 0};
@@ -399,13 +432,15 @@ int kubo_gemm_slices(long K, int M, const DeviceInfo &dev) {
 }
 
 int launch_kubo_step(const Ell &H, const Ell *Vop, const void *x, const void *prev, void *out,
-                     void *wout, int R, float alpha, float beta, int nchunks, hipStream_t s) {
+                     void *wout, int R, float alpha, float beta, int nchunks, hipStream_t s,
+                     const KuboLeft *left) {
     const bool dual = Vop != nullptr;
+    if (left && !dual) return arg_error("internal: the left recursion needs the dual step");
     for (int r0 = 0; r0 < R;) {
         int rb = 32;
         while (rb > R - r0) rb >>= 1;
         const bool wide = (R % 2 == 0) && (r0 % 2 == 0);
-        KStep e = pick(rb, wide, dual);
+        KStep e = pick(rb, wide, dual, left != nullptr);
         if (!e.fn) return arg_error("unsupported vector group");
         const int rpb = kWaves * (64 / e.lpr);
         long rpc = ((long)H.n + nchunks - 1) / nchunks;
@@ -419,11 +454,16 @@ int launch_kubo_step(const Ell &H, const Ell *Vop, const void *x, const void *pr
         const void *p_prev = prev ? (const char *)prev + off : nullptr;
         void *p_out = (char *)out + off;
         void *p_w = wout ? (char *)wout + off : nullptr;
+        const void *p_xl = left ? (const char *)left->x + off : nullptr;
+        const void *p_prevl = left && left->prev ? (const char *)left->prev + off : nullptr;
+        void *p_outl = left ? (char *)left->out + off : nullptr;
         int p_ld = R;
         float p_a = alpha, p_b = beta;
+        float p_al = left ? left->alpha : 0.f, p_bl = left ? left->beta : 0.f;
         int p_rpc = (int)rpc, p_nch = nchunks;
-        void *args[] = {&p_h, &p_v, &p_c, &p_n, &p_k, &p_x, &p_prev, &p_out,
-                        &p_w, &p_ld, &p_a, &p_b, &p_rpc, &p_nch};
+        void *args[] = {&p_h,  &p_v,     &p_c,    &p_n,  &p_k,   &p_x,  &p_prev,
+                        &p_out, &p_w,    &p_xl,   &p_prevl, &p_outl, &p_ld, &p_a,
+                        &p_b,  &p_al,    &p_bl,   &p_rpc, &p_nch};
         JKPM_HIP(hipLaunchKernel(e.fn, dim3(nchunks), dim3(kBlock), args, 0, s));
         r0 += rb;
     }
