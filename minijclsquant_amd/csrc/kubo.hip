@@ -22,6 +22,8 @@
 //                      MFMA registers (no LDS, no barriers).  Work-groups of
 //                      one K slice sit on one XCD so the slice is fetched
 //                      from HBM once per XCD L2.
+//   kubo_gemm3_kernel  the same GEMM with three real products per complex
+//                      one (Gauss); the default.
 //   kubo_reduce_kernel FP64 sum of the FP32 slice partials in fixed order.
 //   kubo_sigma_kernel  sigma_i = sum_mn (a_m b_n + c_m d_n) mu_mn per energy.
 #include <cstdlib>
@@ -334,6 +336,156 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
             }
 }
 
+// The same contraction with three real products per complex one (Gauss):
+// for conj(a) b,
+//   k1 = (ar - ai) br,   k2 = ar (bi - br),   k3 = ai (br + bi),
+//   re = k1 + k3 = ar br + ai bi,   im = k1 + k2 = ar bi - ai br,
+// so each block takes three MFMAs per k-set instead of four.  The third
+// accumulator set (192 registers) does not fit two waves per SIMD, so this
+// kernel runs one wave per SIMD, accumulators in AGPRs, with NB3 register
+// batches in rotation and NB3 - 1 of them in flight (3 measured fastest of
+// 3..6).  Operand layout, tiles and slice mapping are those of
+// kubo_gemm_kernel.  The error against the FP64 oracle is that of the
+// four-product form: both are set by the complex64 stacks.
+constexpr int NB3 = 3;
+
+template <bool ALIGNED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) void kubo_gemm3_kernel(
+    const float2 *__restrict__ L, const float2 *__restrict__ Rk, long K, int M, int ntm,
+    int ntn, int nslices, long k_per_slice, float2 *__restrict__ part) {
+    const int ntiles = ntm * ntn;
+    const int b = blockIdx.x;
+    int slice, tile;
+    if (nslices % kXcd == 0) {
+        const int xcd = b % kXcd, local = b / kXcd;
+        slice = (local / ntiles) * kXcd + xcd;
+        tile = local % ntiles;
+    } else {
+        slice = b / ntiles;
+        tile = b % ntiles;
+    }
+    const int m0 = (tile / ntn) * TM, n0 = (tile % ntn) * TN;
+    const long k_begin = (long)slice * k_per_slice;
+    const long k_end = min(K, k_begin + k_per_slice);
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int quarter = lane >> 4, l16 = lane & 15;
+
+    const float2 *pa[4], *pb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = m0 + wm * 64 + i * 16 + l16;
+        pa[i] = L + (long)(m < M ? m : 0) * K;
+        const int nn = n0 + wn * 64 + i * 16 + l16;
+        pb[i] = Rk + (long)(nn < M ? nn : 0) * K;
+    }
+
+    floatx4 k1[4][4], k2[4][4], k3[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) k1[i][j][r] = k2[i][j][r] = k3[i][j][r] = 0.f;
+
+    float4 A[NB3][4], B[NB3][4];
+    const long k_last = k_end - 2;
+    auto load = [&](long kc, float4 (&a)[4], float4 (&bb)[4], bool masked) {
+        const long kk = kc + quarter * 2;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if constexpr (ALIGNED) {
+                const long kc_ = kk < k_end ? kk : k_last;
+                float4 x = *reinterpret_cast<const float4 *>(pa[i] + kc_);
+                float4 y = *reinterpret_cast<const float4 *>(pb[i] + kc_);
+                if (masked && kk >= k_end) x = y = make_float4(0.f, 0.f, 0.f, 0.f);
+                a[i] = x;
+                bb[i] = y;
+            } else {
+                const long k0 = kk < k_end ? kk : k_end - 1;
+                const long k1_ = kk + 1 < k_end ? kk + 1 : k_end - 1;
+                float2 x0 = pa[i][k0], x1 = pa[i][k1_], y0 = pb[i][k0], y1 = pb[i][k1_];
+                if (masked && kk >= k_end) x0 = y0 = make_float2(0.f, 0.f);
+                if (masked && kk + 1 >= k_end) x1 = y1 = make_float2(0.f, 0.f);
+                a[i] = make_float4(x0.x, x0.y, x1.x, x1.y);
+                bb[i] = make_float4(y0.x, y0.y, y1.x, y1.y);
+            }
+        }
+    };
+    auto mul = [&](const float4 (&a)[4], const float4 (&bb)[4]) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            float ar[4], ai[4], d[4], br[4], u[4], w[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                ar[i] = g ? a[i].z : a[i].x;
+                ai[i] = g ? a[i].w : a[i].y;
+                d[i] = ar[i] - ai[i];
+                br[i] = g ? bb[i].z : bb[i].x;
+                const float bi = g ? bb[i].w : bb[i].y;
+                u[i] = bi - br[i];
+                w[i] = br[i] + bi;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    k1[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(d[i], br[j], k1[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    k2[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], u[j], k2[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    k3[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], w[j], k3[i][j], 0, 0, 0);
+        }
+    };
+    if (k_begin < k_end) {
+        const long nb = (k_end - k_begin + DK - 1) / DK;
+        const long nfull = (k_end - k_begin) / DK;
+        const long nmain = nfull / NB3 * NB3;
+        // Prefetches past the last full batch read clamped, in-range k and
+        // are never multiplied; the tail reloads with zero masks.
+#pragma unroll
+        for (int u = 0; u < NB3 - 1; ++u) {
+            load(k_begin + u * DK, A[u], B[u], false);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        for (long t = 0; t < nmain; t += NB3) {
+#pragma unroll
+            for (int u = 0; u < NB3; ++u) {
+                load(k_begin + (t + u + NB3 - 1) * DK, A[(u + NB3 - 1) % NB3],
+                     B[(u + NB3 - 1) % NB3], false);
+                __builtin_amdgcn_sched_barrier(0);
+                mul(A[u], B[u]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        for (long t = nmain; t < nb; ++t) {
+            load(k_begin + t * DK, A[0], B[0], true);
+            mul(A[0], B[0]);
+        }
+    }
+
+    float2 *P = part + (long)slice * M * M;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm * 64 + i * 16 + 4 * quarter + r;
+                const int cc = n0 + wn * 64 + j * 16 + l16;
+                if (row < M && cc < M)
+                    P[(long)row * M + cc] =
+                        make_float2(k1[i][j][r] + k3[i][j][r], k1[i][j][r] + k2[i][j][r]);
+            }
+}
+
 __global__ void kubo_reduce_kernel(const float2 *__restrict__ part, int nslices, long mm,
                                    double scale, double2 *__restrict__ mu) {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -474,12 +626,25 @@ int launch_kubo_gemm(const float2 *L, const float2 *Rk, long K, int M, int nslic
                      hipStream_t s) {
     const int ntm = (M + TM - 1) / TM, ntn = (M + TN - 1) / TN;
     const long kps = ((K + nslices - 1) / nslices + TK - 1) / TK * TK;
-    if (K % 2 == 0)
-        hipLaunchKernelGGL(kubo_gemm_kernel<true>, dim3(ntm * ntn * nslices), dim3(kBlock), 0, s, L,
-                           Rk, K, M, ntm, ntn, nslices, kps, part);
-    else
-        hipLaunchKernelGGL(kubo_gemm_kernel<false>, dim3(ntm * ntn * nslices), dim3(kBlock), 0, s, L,
-                           Rk, K, M, ntm, ntn, nslices, kps, part);
+    // JKPM_KUBO_GEMM=4 selects the four-product kernel (comparison, tuning)
+    const char *env = getenv("JKPM_KUBO_GEMM");
+    const bool four = env && atoi(env) == 4;
+    const dim3 grid(ntm * ntn * nslices), block(kBlock);
+    const bool al = K % 2 == 0;
+    if (four) {
+        if (al)
+            hipLaunchKernelGGL(kubo_gemm_kernel<true>, grid, block, 0, s, L, Rk, K, M, ntm, ntn,
+                               nslices, kps, part);
+        else
+            hipLaunchKernelGGL(kubo_gemm_kernel<false>, grid, block, 0, s, L, Rk, K, M, ntm, ntn,
+                               nslices, kps, part);
+    } else if (al) {
+        hipLaunchKernelGGL(kubo_gemm3_kernel<true>, grid, block, 0, s, L, Rk, K, M, ntm, ntn,
+                           nslices, kps, part);
+    } else {
+        hipLaunchKernelGGL(kubo_gemm3_kernel<false>, grid, block, 0, s, L, Rk, K, M, ntm, ntn,
+                           nslices, kps, part);
+    }
     JKPM_CHECK_LAUNCH("kubo_gemm_kernel");
     return JKPM_OK;
 }

@@ -137,3 +137,26 @@ def test_mu_nm_odd_k(torch_gpu, L, R, M):
     m = np.arange(M)
     want = 0.5 * (single[m[:, None] + m[None, :]] + single[np.abs(m[:, None] - m[None, :])])
     assert np.max(np.abs(mu2 - want)) / N <= TOL_MU
+
+
+@pytest.mark.parametrize("L,R,M,odd", [(24, 4, 256, False), (7, 3, 33, True)])
+def test_gemm_three_and_four_products_agree(torch_gpu, monkeypatch, L, R, M, odd):
+    """The default GEMM (three real products per complex one) against the
+    four-product kernel (JKPM_KUBO_GEMM=4) on the same stacks: both sum the
+    same FP32 products in a different form, so they agree to FP32 rounding
+    over N*R terms, 2e-6 of max |mu|; aligned and unaligned (odd K) rows."""
+    if odd:
+        H, _ = jcl.square_lattice(L, W=1.0, seed=L)
+        A = jcl.ell_matrix(H)
+        A.bounds = np.array([-4.5 * 1.01, 4.5 * 1.01])
+        assert (A.shape[0] * R) % 2 == 1
+        v = identity_velocity(A)
+    else:
+        A, _ = lattice(L, seed=L)
+        v = jcl.modifier_velocity(A, "x")
+    r = vectors(A.shape[0], R, 31)
+    monkeypatch.delenv("JKPM_KUBO_GEMM", raising=False)
+    mu3 = jcl.kpm_kubo_moments(A, M, r, v_data=v).cpu().numpy()
+    monkeypatch.setenv("JKPM_KUBO_GEMM", "4")
+    mu4 = jcl.kpm_kubo_moments(A, M, r, v_data=v).cpu().numpy()
+    assert np.max(np.abs(mu3 - mu4)) / np.max(np.abs(mu4)) <= 2e-6
