@@ -230,8 +230,9 @@ int launch_step(const StepArgs &a, hipStream_t s) {
     const step::StepEntry e =
         H.vtype == JKPM_C64
             ? step::pick_step_c64(a.rb, wide, nl, H.k, a.doubling ? H.dvariant : H.variant,
-                                  a.doubling)
-            : step::pick_step_c128(a.rb, nl, H.k, a.doubling ? H.dvariant : H.variant, a.doubling);
+                                  a.doubling, a.first)
+            : step::pick_step_c128(a.rb, nl, H.k, a.doubling ? H.dvariant : H.variant, a.doubling,
+                                   a.first);
     if (!e.fn) return arg_error("unsupported vector group size " + std::to_string(a.rb));
     const int rpb = kWaves * (64 / e.lpr) * e.unr;
     long rpc = ((long)H.n + a.nchunks - 1) / a.nchunks;

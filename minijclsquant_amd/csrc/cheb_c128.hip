@@ -16,16 +16,43 @@ constexpr bool kNTM = true;
 constexpr bool kNTM = false;
 #endif
 
-template <int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool NS = false, bool NTM = false>
+// Single-form kernels keep the first-step accumulators (F0) in every
+// variant: without them the tuned plain kernel measured slower (232 against
+// 210 us per launch on the headline workload, at 68 against 80 VGPRs).  The
+// doubling-form kernels drop them (86 against 104 VGPRs, 5 waves per SIMD
+// instead of 4, 179 against 186 us per launch).
+template <int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool NS = false, bool NTM = false,
+          bool F0 = true>
 const void *ptr() {
     return reinterpret_cast<const void *>(
-        &cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT, false, NS, NTM>);
+        &cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT, false, NS, NTM, F0>);
 }
 
-template <int VPL, int LPR, int KT, int UNR, bool NT, bool NS = false, bool NTM = false>
+template <int VPL, int LPR, int KT, int UNR, bool NT, bool NS = false, bool NTM = false,
+          bool F0 = false>
 const void *dbl_ptr() {
     return reinterpret_cast<const void *>(
-        &cheb_step_kernel<Real, VPL, LPR, 2, KT, UNR, NT, true, NS, NTM>);
+        &cheb_step_kernel<Real, VPL, LPR, 2, KT, UNR, NT, true, NS, NTM, F0>);
+}
+
+// First-step kernels: the base kernel of the shape with the step-0
+// accumulators (F0), whatever the tuned variant (one launch per recursion).
+template <int VPL, int LPR>
+StepEntry first_entry(int nl, int k, bool dbl) {
+    const void *f;
+    if (dbl)
+        f = k == 4 ? dbl_ptr<VPL, LPR, 4, 1, false, false, false, true>()
+          : k == 5 ? dbl_ptr<VPL, LPR, 5, 1, false, false, false, true>()
+                   : dbl_ptr<VPL, LPR, 0, 1, false, false, false, true>();
+    else if (nl == 2)
+        f = k == 4 ? ptr<VPL, LPR, 2, 4, 1, false, false, false, true>()
+          : k == 5 ? ptr<VPL, LPR, 2, 5, 1, false, false, false, true>()
+                   : ptr<VPL, LPR, 2, 0, 1, false, false, false, true>();
+    else
+        f = k == 4 ? ptr<VPL, LPR, 1, 4, 1, false, false, false, true>()
+          : k == 5 ? ptr<VPL, LPR, 1, 5, 1, false, false, false, true>()
+                   : ptr<VPL, LPR, 1, 0, 1, false, false, false, true>();
+    return {f, VPL, LPR, 1};
 }
 
 template <int VPL, int LPR, int KT, bool NS>
@@ -70,7 +97,11 @@ const void *tuned(int v) {
 
 // The NTM unit has only the tuned kernels (fn = NULL otherwise).
 template <int VPL, int LPR>
-StepEntry entry(int nl, int k, int v, bool dbl) {
+StepEntry entry(int nl, int k, int v, bool dbl, bool first) {
+    if (first) {
+        if constexpr (kNTM) return {nullptr, 0, 0, 1};
+        else return first_entry<VPL, LPR>(nl, k, dbl);
+    }
     if (dbl) return dbl_entry<VPL, LPR>(k, v);
     if (nl == 1 && k == 4) return {tuned<VPL, LPR, 4>(v), VPL, LPR, (v & 1) ? 2 : 1};
     if (nl == 1 && k == 5) return {tuned<VPL, LPR, 5>(v), VPL, LPR, (v & 1) ? 2 : 1};
@@ -87,29 +118,29 @@ StepEntry entry(int nl, int k, int v, bool dbl) {
     }
 }
 
-StepEntry pick_c128(int rb, int nl, int k, int variant, bool dbl) {
+StepEntry pick_c128(int rb, int nl, int k, int variant, bool dbl, bool first) {
     // 16-byte lanes and up: VPL = 1 << shape (capped by the group size)
-    int shape = ((variant % 12) >> 2) & 3;
+    int shape = first ? 0 : ((variant % 12) >> 2) & 3;
     while (shape > 0 && (1 << shape) > rb) --shape;
     switch (rb) {
-        case 1: return entry<1, 1>(nl, k, variant, dbl);
-        case 2: return shape == 0 ? entry<1, 2>(nl, k, variant, dbl) : entry<2, 1>(nl, k, variant, dbl);
+        case 1: return entry<1, 1>(nl, k, variant, dbl, first);
+        case 2: return shape == 0 ? entry<1, 2>(nl, k, variant, dbl, first) : entry<2, 1>(nl, k, variant, dbl, first);
         case 4:
-            return shape == 0   ? entry<1, 4>(nl, k, variant, dbl)
-                   : shape == 1 ? entry<2, 2>(nl, k, variant, dbl)
-                                : entry<4, 1>(nl, k, variant, dbl);
+            return shape == 0   ? entry<1, 4>(nl, k, variant, dbl, first)
+                   : shape == 1 ? entry<2, 2>(nl, k, variant, dbl, first)
+                                : entry<4, 1>(nl, k, variant, dbl, first);
         case 8:
-            return shape == 0   ? entry<1, 8>(nl, k, variant, dbl)
-                   : shape == 1 ? entry<2, 4>(nl, k, variant, dbl)
-                                : entry<4, 2>(nl, k, variant, dbl);
+            return shape == 0   ? entry<1, 8>(nl, k, variant, dbl, first)
+                   : shape == 1 ? entry<2, 4>(nl, k, variant, dbl, first)
+                                : entry<4, 2>(nl, k, variant, dbl, first);
         case 16:
-            return shape == 0   ? entry<1, 16>(nl, k, variant, dbl)
-                   : shape == 1 ? entry<2, 8>(nl, k, variant, dbl)
-                                : entry<4, 4>(nl, k, variant, dbl);
+            return shape == 0   ? entry<1, 16>(nl, k, variant, dbl, first)
+                   : shape == 1 ? entry<2, 8>(nl, k, variant, dbl, first)
+                                : entry<4, 4>(nl, k, variant, dbl, first);
         case 32:
-            return shape == 0   ? entry<1, 32>(nl, k, variant, dbl)
-                   : shape == 1 ? entry<2, 16>(nl, k, variant, dbl)
-                                : entry<4, 8>(nl, k, variant, dbl);
+            return shape == 0   ? entry<1, 32>(nl, k, variant, dbl, first)
+                   : shape == 1 ? entry<2, 16>(nl, k, variant, dbl, first)
+                                : entry<4, 8>(nl, k, variant, dbl, first);
         default: break;
     }
     return {nullptr, 0, 0, 1};
@@ -119,16 +150,17 @@ StepEntry pick_c128(int rb, int nl, int k, int variant, bool dbl) {
 
 #ifdef JKPM_STEP_NTM_UNIT
 StepEntry pick_step_c128_ntm(int rb, int nl, int k, int variant, bool dbl) {
-    return pick_c128(rb, nl, k, variant, dbl);
+    return pick_c128(rb, nl, k, variant, dbl, false);
 }
 #else
-StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl) {
+StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl, bool first) {
+    if (first) return pick_c128(rb, nl, k, 0, dbl, true);
     if (variant >= 24) {
         const StepEntry e = pick_step_c128_ntm(rb, nl, k, variant - 24, dbl);
         if (e.fn) return e;
         variant -= 24;
     }
-    return pick_c128(rb, nl, k, variant, dbl);
+    return pick_c128(rb, nl, k, variant, dbl, false);
 }
 #endif
 

@@ -17,16 +17,43 @@ constexpr bool kNTM = true;
 constexpr bool kNTM = false;
 #endif
 
-template <int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool NS = false, bool NTM = false>
+// Single-form kernels keep the first-step accumulators (F0) in every
+// variant: without them the tuned plain kernel measured slower (232 against
+// 210 us per launch on the headline workload, at 68 against 80 VGPRs).  The
+// doubling-form kernels drop them (86 against 104 VGPRs, 5 waves per SIMD
+// instead of 4, 179 against 186 us per launch).
+template <int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool NS = false, bool NTM = false,
+          bool F0 = true>
 const void *ptr() {
     return reinterpret_cast<const void *>(
-        &cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT, false, NS, NTM>);
+        &cheb_step_kernel<Real, VPL, LPR, NL, KT, UNR, NT, false, NS, NTM, F0>);
 }
 
-template <int VPL, int LPR, int KT, int UNR, bool NT, bool NS = false, bool NTM = false>
+template <int VPL, int LPR, int KT, int UNR, bool NT, bool NS = false, bool NTM = false,
+          bool F0 = false>
 const void *dbl_ptr() {
     return reinterpret_cast<const void *>(
-        &cheb_step_kernel<Real, VPL, LPR, 2, KT, UNR, NT, true, NS, NTM>);
+        &cheb_step_kernel<Real, VPL, LPR, 2, KT, UNR, NT, true, NS, NTM, F0>);
+}
+
+// First-step kernels: the base kernel of the shape with the step-0
+// accumulators (F0), whatever the tuned variant (one launch per recursion).
+template <int VPL, int LPR>
+StepEntry first_entry(int nl, int k, bool dbl) {
+    const void *f;
+    if (dbl)
+        f = k == 4 ? dbl_ptr<VPL, LPR, 4, 1, false, false, false, true>()
+          : k == 5 ? dbl_ptr<VPL, LPR, 5, 1, false, false, false, true>()
+                   : dbl_ptr<VPL, LPR, 0, 1, false, false, false, true>();
+    else if (nl == 2)
+        f = k == 4 ? ptr<VPL, LPR, 2, 4, 1, false, false, false, true>()
+          : k == 5 ? ptr<VPL, LPR, 2, 5, 1, false, false, false, true>()
+                   : ptr<VPL, LPR, 2, 0, 1, false, false, false, true>();
+    else
+        f = k == 4 ? ptr<VPL, LPR, 1, 4, 1, false, false, false, true>()
+          : k == 5 ? ptr<VPL, LPR, 1, 5, 1, false, false, false, true>()
+                   : ptr<VPL, LPR, 1, 0, 1, false, false, false, true>();
+    return {f, VPL, LPR, 1};
 }
 
 template <int VPL, int LPR, int KT, bool NS>
@@ -73,7 +100,11 @@ const void *tuned(int v) {
 // k = 4, 5.  `variants` = false instantiates only the base kernels.  The NTM
 // unit has only the tuned ones (fn = NULL otherwise).
 template <int VPL, int LPR, bool variants>
-StepEntry entry(int nl, int k, int v, bool dbl) {
+StepEntry entry(int nl, int k, int v, bool dbl, bool first) {
+    if (first) {
+        if constexpr (kNTM) return {nullptr, 0, 0, 1};
+        else return first_entry<VPL, LPR>(nl, k, dbl);
+    }
     if (dbl) {
         if constexpr (kNTM && !variants) return {nullptr, 0, 0, 1};
         else return dbl_entry<VPL, LPR>(k, variants ? v : 0);
@@ -103,38 +134,38 @@ StepEntry entry(int nl, int k, int v, bool dbl) {
 
 namespace {
 
-StepEntry pick_c64(int rb, bool wide, int nl, int k, int variant, bool dbl) {
+StepEntry pick_c64(int rb, bool wide, int nl, int k, int variant, bool dbl, bool first) {
     if (wide) {
         // 16-byte lanes and up: VPL = 2 << shape (capped by the group size)
-        int shape = ((variant % 12) >> 2) & 3;
+        int shape = first ? 0 : ((variant % 12) >> 2) & 3;
         while (shape > 0 && (2 << shape) > rb) --shape;
         switch (rb) {
-            case 2: return entry<2, 1, true>(nl, k, variant, dbl);
+            case 2: return entry<2, 1, true>(nl, k, variant, dbl, first);
             case 4:
-                return shape == 0 ? entry<2, 2, true>(nl, k, variant, dbl) : entry<4, 1, true>(nl, k, variant, dbl);
+                return shape == 0 ? entry<2, 2, true>(nl, k, variant, dbl, first) : entry<4, 1, true>(nl, k, variant, dbl, first);
             case 8:
-                return shape == 0   ? entry<2, 4, true>(nl, k, variant, dbl)
-                       : shape == 1 ? entry<4, 2, true>(nl, k, variant, dbl)
-                                    : entry<8, 1, true>(nl, k, variant, dbl);
+                return shape == 0   ? entry<2, 4, true>(nl, k, variant, dbl, first)
+                       : shape == 1 ? entry<4, 2, true>(nl, k, variant, dbl, first)
+                                    : entry<8, 1, true>(nl, k, variant, dbl, first);
             case 16:
-                return shape == 0   ? entry<2, 8, true>(nl, k, variant, dbl)
-                       : shape == 1 ? entry<4, 4, true>(nl, k, variant, dbl)
-                                    : entry<8, 2, true>(nl, k, variant, dbl);
+                return shape == 0   ? entry<2, 8, true>(nl, k, variant, dbl, first)
+                       : shape == 1 ? entry<4, 4, true>(nl, k, variant, dbl, first)
+                                    : entry<8, 2, true>(nl, k, variant, dbl, first);
             case 32:
-                return shape == 0   ? entry<2, 16, true>(nl, k, variant, dbl)
-                       : shape == 1 ? entry<4, 8, true>(nl, k, variant, dbl)
-                                    : entry<8, 4, true>(nl, k, variant, dbl);
+                return shape == 0   ? entry<2, 16, true>(nl, k, variant, dbl, first)
+                       : shape == 1 ? entry<4, 8, true>(nl, k, variant, dbl, first)
+                                    : entry<8, 4, true>(nl, k, variant, dbl, first);
             default: break;
         }
     }
     // 8-byte lanes (odd leading dimension or a single vector)
     switch (rb) {
-        case 1: return entry<1, 1, false>(nl, k, 0, dbl);
-        case 2: return entry<1, 2, false>(nl, k, 0, dbl);
-        case 4: return entry<1, 4, false>(nl, k, 0, dbl);
-        case 8: return entry<1, 8, false>(nl, k, 0, dbl);
-        case 16: return entry<1, 16, false>(nl, k, 0, dbl);
-        case 32: return entry<1, 32, false>(nl, k, 0, dbl);
+        case 1: return entry<1, 1, false>(nl, k, 0, dbl, first);
+        case 2: return entry<1, 2, false>(nl, k, 0, dbl, first);
+        case 4: return entry<1, 4, false>(nl, k, 0, dbl, first);
+        case 8: return entry<1, 8, false>(nl, k, 0, dbl, first);
+        case 16: return entry<1, 16, false>(nl, k, 0, dbl, first);
+        case 32: return entry<1, 32, false>(nl, k, 0, dbl, first);
         default: break;
     }
     return {nullptr, 0, 0, 1};
@@ -145,16 +176,17 @@ StepEntry pick_c64(int rb, bool wide, int nl, int k, int variant, bool dbl) {
 #ifdef JKPM_STEP_NTM_UNIT
 StepEntry pick_step_c64_ntm(int rb, bool wide, int nl, int k, int variant, bool dbl) {
     if (!wide) return {nullptr, 0, 0, 1};
-    return pick_c64(rb, wide, nl, k, variant, dbl);
+    return pick_c64(rb, wide, nl, k, variant, dbl, false);
 }
 #else
-StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl) {
+StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl, bool first) {
+    if (first) return pick_c64(rb, wide, nl, k, 0, dbl, true);
     if (variant >= 24) {
         const StepEntry e = pick_step_c64_ntm(rb, wide, nl, k, variant - 24, dbl);
         if (e.fn) return e;
         variant -= 24;
     }
-    return pick_c64(rb, wide, nl, k, variant, dbl);
+    return pick_c64(rb, wide, nl, k, variant, dbl, false);
 }
 #endif
 

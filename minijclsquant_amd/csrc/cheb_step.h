@@ -36,14 +36,18 @@ __device__ __forceinline__ V load_stream(const V *p) {
 // NS: the T_{n+1} stream is written with non-temporal stores.
 // NTM: the matrix values and columns are loaded non-temporally (pays when
 // the step's working set is far beyond the caches, costs when it is not).
+// F0: the kernel can run the first step (T_1 and the step-0 partials); its
+// step-0 accumulators cost registers, so only the base kernel of each shape
+// has it, and every other launch takes a kernel without (first must be 0).
 template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool DBL = false,
-          bool NS = false, bool NTM = false>
+          bool NS = false, bool NTM = false, bool F0 = false>
 __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const Real *__restrict__ val, const int *__restrict__ col, int n, int k_rt,
     const Real *__restrict__ xcur, const Real *xprev, Real *xout,
     const Real *__restrict__ vl0, const Real *__restrict__ vl1, int ld,
-    double2 *__restrict__ part, double2 *__restrict__ part0, int first,
+    double2 *__restrict__ part, double2 *__restrict__ part0, int first_arg,
     int rows_per_chunk, int nchunks, const long *__restrict__ chunk_rows) {
+    const bool first = F0 && first_arg;
     using V = vec_t<Real, 2 * VPL>;
     using C2 = vec_t<Real, 2>;
     constexpr int RPW = 64 / LPR;
@@ -64,11 +68,15 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
         chunk_rows ? chunk_rows[chunk + 1] : min((long)n, row_begin + rows_per_chunk);
 
     double ar[NL][VPL], ai[NL][VPL];
-    double a0r[NL][VPL], a0i[NL][VPL];
+    double a0r[F0 ? NL : 1][VPL], a0i[F0 ? NL : 1][VPL];
 #pragma unroll
     for (int l = 0; l < NL; ++l)
 #pragma unroll
-        for (int v = 0; v < VPL; ++v) ar[l][v] = ai[l][v] = a0r[l][v] = a0i[l][v] = 0.0;
+        for (int v = 0; v < VPL; ++v) ar[l][v] = ai[l][v] = 0.0;
+#pragma unroll
+    for (int l = 0; l < (F0 ? NL : 1); ++l)
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) a0r[l][v] = a0i[l][v] = 0.0;
 
     const int voff = lv * VPL * 2;
     const Real *vls[2] = {vl0, vl1};
@@ -188,15 +196,17 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
                     y[2 * v] = sr[u][v];
                     y[2 * v + 1] = si[u][v];
                 }
+                if constexpr (F0) {
 #pragma unroll
-                for (int l = 0; l < NL; ++l)
+                    for (int l = 0; l < NL; ++l)
 #pragma unroll
-                    for (int v = 0; v < VPL; ++v) {
-                        const double qr = q[u][l][2 * v], qi = q[u][l][2 * v + 1];
-                        const double tr = t0[u][2 * v], ti = t0[u][2 * v + 1];
-                        a0r[l][v] += qr * tr + qi * ti;
-                        a0i[l][v] += qr * ti - qi * tr;
-                    }
+                        for (int v = 0; v < VPL; ++v) {
+                            const double qr = q[u][l][2 * v], qi = q[u][l][2 * v + 1];
+                            const double tr = t0[u][2 * v], ti = t0[u][2 * v + 1];
+                            a0r[l][v] += qr * tr + qi * ti;
+                            a0i[l][v] += qr * ti - qi * tr;
+                        }
+                }
             } else {
 #pragma unroll
                 for (int v = 0; v < VPL; ++v) {
@@ -216,7 +226,9 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
                     const double qr = q[u][l][2 * v], qi = q[u][l][2 * v + 1];
                     const double yr = y[2 * v], yi = y[2 * v + 1];
                     ar[l][v] += qr * yr + qi * yi;
-                    ai[l][v] += qr * yi - qi * yr;
+                    // <T_{n+1}|T_{n+1}> of the doubling form is real: no
+                    // imaginary accumulator for it
+                    if (!(DBL && l == 0)) ai[l][v] += qr * yi - qi * yr;
                 }
         }
     }
@@ -230,9 +242,11 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
             for (int v = 0; v < VPL; ++v) {
                 ar[l][v] += __shfl_xor(ar[l][v], o);
                 ai[l][v] += __shfl_xor(ai[l][v], o);
-                if (first) {
-                    a0r[l][v] += __shfl_xor(a0r[l][v], o);
-                    a0i[l][v] += __shfl_xor(a0i[l][v], o);
+                if constexpr (F0) {
+                    if (first) {
+                        a0r[l][v] += __shfl_xor(a0r[l][v], o);
+                        a0i[l][v] += __shfl_xor(a0i[l][v], o);
+                    }
                 }
             }
     }
@@ -244,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
 #pragma unroll
             for (int v = 0; v < VPL; ++v) {
                 sh[0][wave][l][lv * VPL + v] = make_double2(ar[l][v], ai[l][v]);
-                sh[1][wave][l][lv * VPL + v] = make_double2(a0r[l][v], a0i[l][v]);
+                if constexpr (F0) sh[1][wave][l][lv * VPL + v] = make_double2(a0r[l][v], a0i[l][v]);
             }
     }
     __syncthreads();
@@ -257,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
             si2 += sh[0][w][l][r].y;
         }
         part[((long)l * nchunks + chunk) * ld + r] = make_double2(sr2, si2);
-        if (first) {
+        if (F0 && first) {
             double s0r = 0.0, s0i = 0.0;
 #pragma unroll
             for (int w = 0; w < kWaves; ++w) {
@@ -281,10 +295,13 @@ struct StepEntry {
 // vectors per lane (VPL) by 1, 2 or 4.  Variants
 // apply to single-left-set kernels of ELL width 4 and 5; every other shape
 // uses the base kernel.  Returns fn = nullptr for an unsupported group size.
-// dbl selects the doubling form (two partial sets, see DBL above).
+// dbl selects the doubling form (two partial sets, see DBL above); first
+// selects the base kernel of the shape with the first-step accumulators (F0).
 // variant 0..47: 24..47 are 0..23 with non-temporal matrix loads (NTM).
-StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl = false);
-StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl = false);
+StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl = false,
+                        bool first = false);
+StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl = false,
+                         bool first = false);
 // The NTM kernels of the tuned shapes (variant 0..23 here); fn is NULL for
 // shapes that have none.  Defined in their own translation units.
 StepEntry pick_step_c64_ntm(int rb, bool wide, int nl, int k, int variant, bool dbl);
