@@ -1251,7 +1251,16 @@ GradLayout grad_layout(const Ell &H, int R, int M, int seg, const DeviceInfo &de
         while ((long)S * S < (long)(M - 1)) ++S;  // ceil(sqrt(M - 1))
     L.S = S;
     L.nseg = M > 1 ? (M - 1 + S - 1) / S : 0;
-    L.nchunks = step_chunks(H, R, dev);
+    // The adjoint steps, folds and recomputed steps take a fixed wide launch
+    // (up to 32 work-groups per CU of at least 64 rows), not the forward's
+    // tuned chunk count: the forward tunes to anything from 2 to 32
+    // work-groups per CU at near-equal times, and at 2 the backward of the
+    // square 512 x 512, M = 1024, R = 8 gradient took 59.7 ms against 46.3.
+    {
+        long c = std::min<long>((long)dev.cu_count * 32, ((long)H.n + 63) / 64);
+        if (c >= 8) c = c / 8 * 8;
+        L.nchunks = (int)std::max<long>(c, 1);
+    }
     size_t o = 0;
     L.ckpt = o;
     if (!L.keep_all) o += (size_t)2 * L.nseg * L.vec;
