@@ -1495,7 +1495,20 @@ KuboLayout kubo_layout(const Ell &H, int R, int M, const DeviceInfo &dev) {
     const size_t vec = (size_t)H.n * R * 8;
     const size_t stack = align_up((size_t)M * vec, 256);
     K.nslices = kubo_gemm_slices((long)H.n * R, M, dev);
-    K.nchunks = step_chunks(H, R, dev);
+    // Stack steps: 16 work-groups per CU (at least 64 rows each).  On the
+    // honeycomb 512 x 512 workload (R = 4) the whole step measured 44.9-45.1
+    // ms at 16, 47.3-47.8 at 8, 46.1 at 4 and 51.9 at 32.
+    // JKPM_KUBO_CHUNK_MULT overrides the work-groups per CU (tuning).
+    long mult = 16;
+    if (const char *env = getenv("JKPM_KUBO_CHUNK_MULT")) {
+        const long m = atol(env);
+        if (m > 0) mult = m;
+    }
+    {
+        long c = std::min<long>((long)dev.cu_count * mult, ((long)H.n + 63) / 64);
+        if (c >= 8) c = c / 8 * 8;
+        K.nchunks = (int)std::max<long>(c, 1);
+    }
     K.L = 0;
     K.Rk = stack;
     K.T0 = 2 * stack;
