@@ -12,7 +12,8 @@ vectors) plus the DOS reconstruction, inputs resident in HBM.  The metric is
 N*M*R units per second.  The JSON line also carries
   roofline      the fused step kernel's algorithmic bytes per launch divided
                 by its mean launch duration (HIP events on the launch
-                stream, every `--profile-every`-th launch of the timed steps)
+                stream around runs of 8 consecutive step launches, one run
+                every `--profile-every` steps of the timed region)
                 against the 8 TB/s HBM peak, plus HBM traffic per launch from
                 the committed rocprofv3 counter summary;
   cpu_baseline  the complex128 CPU oracle (the reference algorithm, one

@@ -412,10 +412,12 @@ int jkpm_kubo_sigma(const double *mu, int M, const double *e, int n_e, int kind,
  * Measurement support.
  * ------------------------------------------------------------------- */
 
-/* every > 0: jkpm_moments records a HIP event pair around every every-th
- * Chebyshev step launch on its stream (0 = off); jkpm_profile_read sums the
- * event durations of the recorded steps (ms) and counts them.
- * reset != 0 clears the totals. */
+/* every > 0: the moment calls record HIP event pairs on their stream around
+ * runs of up to 8 consecutive Chebyshev step launches, one run started every
+ * `every` steps (not at a recursion's first step, and closed before any
+ * reduction launch); the two-step tile calls record every every-th launch
+ * alone (0 = off).  jkpm_profile_read sums the recorded spans (ms) and the
+ * launches they cover.  reset != 0 clears the totals. */
 int jkpm_profile_enable(int every);
 int jkpm_profile_read(double *step_ms, long long *step_launches, int reset);
 

@@ -25,11 +25,19 @@ thread_local std::string g_err;
 constexpr int kRing = 64;  // steps of partials held before one reduction launch
 
 // ---- profiling of the step kernel (events on the launch stream) ----
+// A recorded run: events before its first and after its last step launch,
+// and the number of steps in between.
+struct ProfRun {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int steps = 0;
+};
+constexpr int kProfRun = 8;  // steps per recorded run (at most)
+
 struct Prof {
     std::mutex mu;
-    int every = 0;  // record every `every`-th step (0 = off)
+    int every = 0;  // start a recorded run every `every` steps (0 = off)
     long long counter = 0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<ProfRun> pending;
     std::vector<hipEvent_t> free_events;
     double ms = 0.0;
     long long launches = 0;
@@ -50,16 +58,16 @@ hipEvent_t take_event(Prof &p) {
     return e;
 }
 
-// Folds the durations of all recorded pairs into the totals (waits for them).
+// Folds the durations of all recorded runs into the totals (waits for them).
 int drain_profile(Prof &p) {
     for (auto &pr : p.pending) {
-        JKPM_HIP(hipEventSynchronize(pr.second));
+        JKPM_HIP(hipEventSynchronize(pr.e1));
         float ms = 0.f;
-        JKPM_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+        JKPM_HIP(hipEventElapsedTime(&ms, pr.e0, pr.e1));
         p.ms += ms;
-        p.launches += 1;
-        p.free_events.push_back(pr.first);
-        p.free_events.push_back(pr.second);
+        p.launches += pr.steps;
+        p.free_events.push_back(pr.e0);
+        p.free_events.push_back(pr.e1);
     }
     p.pending.clear();
     return JKPM_OK;
@@ -217,27 +225,11 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
         a.part0 = (n == 1) ? ring : nullptr;  // step 0 lives in ring slot 0
         a.nchunks = L.nchunks;
         a.chunk_rows = grp ? grp->chunk_rows : nullptr;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        const bool rec = profiling && (P.counter++ % every == 0);
-        if (rec) {
-            e0 = take_event(P);
-            e1 = take_event(P);
-            if (!e0 || !e1) return hip_fail(hipErrorOutOfMemory, "hipEventCreate", __FILE__, __LINE__);
-            JKPM_HIP(hipEventRecord(e0, s));
-        }
         for (auto &g : groups) {
             a.r0 = g.first;
             a.rb = g.second;
             int rc2 = launch_step(a, s);
             if (rc2) return rc2;
-        }
-        if (rec) {
-            JKPM_HIP(hipEventRecord(e1, s));
-            P.pending.push_back({e0, e1});
-            if (P.pending.size() >= 8192) {
-                int rc3 = drain_profile(P);
-                if (rc3) return rc3;
-            }
         }
         return JKPM_OK;
     };
@@ -251,11 +243,38 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
     double2 *target = doubling ? (double2 *)(w + L.raw) : mu;
     const int target_M = doubling ? nsteps + 1 : M;
     int pending_lo = 0;
+    // Profiling: event pairs around runs of up to kProfRun consecutive steps,
+    // one run started every `every` steps (never at the first step, whose
+    // kernel may differ), each closed before a reduction launch so that no
+    // reduction is timed.  A run's span over its step count is the per-launch
+    // cost of the recursion, inter-kernel gaps included.
+    ProfRun run;
+    bool run_open = false;
     for (int n = 1; n <= nsteps; ++n) {
+        if (profiling && !run_open && n > 1 && P.counter % every == 0) {
+            run.e0 = take_event(P);
+            run.e1 = take_event(P);
+            if (!run.e0 || !run.e1)
+                return hip_fail(hipErrorOutOfMemory, "hipEventCreate", __FILE__, __LINE__);
+            JKPM_HIP(hipEventRecord(run.e0, s));
+            run.steps = 0;
+            run_open = true;
+        }
+        if (profiling) ++P.counter;
         rc = step(n);
         if (rc) return rc;
         const int last = std::min(n, last_slot);
-        if (n == nsteps || n - pending_lo + 1 == kRing) {
+        const bool reduce_now = n == nsteps || n - pending_lo + 1 == kRing;
+        if (run_open && (++run.steps == kProfRun || reduce_now)) {
+            JKPM_HIP(hipEventRecord(run.e1, s));
+            P.pending.push_back(run);
+            run_open = false;
+            if (P.pending.size() >= 8192) {
+                rc = drain_profile(P);
+                if (rc) return rc;
+            }
+        }
+        if (reduce_now) {
             rc = launch_reduce(ring, kRing, pending_lo, last - pending_lo + 1, n_left, L.nchunks,
                                R, target, target_M, grp ? grp->group_chunks : nullptr, G, s);
             if (rc) return rc;
@@ -363,7 +382,7 @@ int run_tiles(const jkpm_tiles &T, const void *v_r, int R, int M, bool doubling,
         }
         if (rec) {
             JKPM_HIP(hipEventRecord(e1, s));
-            P.pending.push_back({e0, e1});
+            P.pending.push_back({e0, e1, 1});
             if (P.pending.size() >= 8192) {
                 rc = drain_profile(P);
                 if (rc) return rc;

@@ -154,7 +154,9 @@ StepEntry pick_step_c128_ntm(int rb, int nl, int k, int variant, bool dbl) {
 }
 #else
 StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl, bool first) {
-    if (first) return pick_c128(rb, nl, k, 0, dbl, true);
+    // The doubling kernels lack the first-step accumulators: its first launch
+    // takes the base kernel of the shape (the single-form kernels all have them).
+    if (first && dbl) return pick_c128(rb, nl, k, 0, dbl, true);
     if (variant >= 24) {
         const StepEntry e = pick_step_c128_ntm(rb, nl, k, variant - 24, dbl);
         if (e.fn) return e;

@@ -180,7 +180,9 @@ StepEntry pick_step_c64_ntm(int rb, bool wide, int nl, int k, int variant, bool 
 }
 #else
 StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl, bool first) {
-    if (first) return pick_c64(rb, wide, nl, k, 0, dbl, true);
+    // The doubling kernels lack the first-step accumulators: its first launch
+    // takes the base kernel of the shape (the single-form kernels all have them).
+    if (first && dbl) return pick_c64(rb, wide, nl, k, 0, dbl, true);
     if (variant >= 24) {
         const StepEntry e = pick_step_c64_ntm(rb, wide, nl, k, variant - 24, dbl);
         if (e.fn) return e;
