@@ -357,3 +357,31 @@ def test_size_limits_and_degenerate_shapes(torch_gpu):
         mu = h1.moments(v, M)[0].cpu().numpy()[:, 0]
         ref = O.rec_A_tab(M, np.array([a]), np.array([0], np.int32), 1, np.array([1.0 + 0j]))
         assert np.allclose(mu, ref, rtol=0, atol=1e-14), M
+
+
+def test_profiler_times_runs_of_consecutive_steps(torch_gpu):
+    """jkpm_profile_enable(16): runs of up to 8 consecutive step launches,
+    one started every 16 steps, never at the first step and closed before a
+    reduction.  M = 130 is 129 steps with reductions after steps 63, 127 and
+    129: runs of 8 start at steps 17, 33, 49, 65, 81, 97 and 113, and a run
+    of 1 at step 129, closed by the last reduction: 57 launches.  The spans
+    must fit inside the whole call."""
+    torch = torch_gpu
+    A = honeycomb(32, seed=4)
+    h = A.device("c64", 1.0 / A.bounds[1])
+    v = torch.ones((A.shape[0], 16), dtype=torch.complex64, device="cuda")
+    h.moments(v, 130)
+    torch.cuda.synchronize()
+    nat.profile_read(reset=True)
+    nat.profile_enable(16)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    h.moments(v, 130)
+    t1.record()
+    t1.synchronize()
+    nat.profile_enable(0)
+    ms, launches = nat.profile_read(reset=True)
+    assert launches == 57
+    assert 0.0 < ms < t0.elapsed_time(t1)
+    nat.profile_enable(0)
+    assert nat.profile_read(reset=True) == (0.0, 0)
