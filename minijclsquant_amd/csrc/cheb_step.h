@@ -36,9 +36,11 @@ __device__ __forceinline__ V load_stream(const V *p) {
 // NS: the T_{n+1} stream is written with non-temporal stores.
 // NTM: the matrix values and columns are loaded non-temporally (pays when
 // the step's working set is far beyond the caches, costs when it is not).
-// F0: the kernel can run the first step (T_1 and the step-0 partials); its
-// step-0 accumulators cost registers, so only the base kernel of each shape
-// has it, and every other launch takes a kernel without (first must be 0).
+// F0: the kernel can run the first step (T_1 and the step-0 partials).  The
+// step-0 accumulators cost registers: every single-form kernel keeps them,
+// but of the doubling-form kernels only the base kernel of each shape has
+// them, and the first step of a doubling recursion runs that one.  A kernel
+// without F0 treats first as 0.
 template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool DBL = false,
           bool NS = false, bool NTM = false, bool F0 = false>
 __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
