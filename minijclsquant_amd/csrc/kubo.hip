@@ -349,8 +349,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
 // four-product form: both are set by the complex64 stacks.
 constexpr int NB3 = 3;
 
-template <bool ALIGNED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) void kubo_gemm3_kernel(
+// WN: 16-column blocks per wave along n.  Only 4 (wave tile 64 x 64, one
+// wave per SIMD) is launched: 2 (64 x 32, whose 96 accumulators fit two
+// waves per SIMD) measured 55.1 against 42.9 ms per Kubo step, its operand
+// rows feeding half as many MFMAs per load.
+template <bool ALIGNED, int WN>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WN == 4 ? 1 : 2, WN == 4 ? 1 : 2))) void kubo_gemm3_kernel(
     const float2 *__restrict__ L, const float2 *__restrict__ Rk, long K, int M, int ntm,
     int ntn, int nslices, long k_per_slice, float2 *__restrict__ part) {
     const int ntiles = ntm * ntn;
@@ -364,7 +368,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
         slice = b / ntiles;
         tile = b % ntiles;
     }
-    const int m0 = (tile / ntn) * TM, n0 = (tile % ntn) * TN;
+    constexpr int TN3 = 2 * 16 * WN;
+    const int m0 = (tile / ntn) * TM, n0 = (tile % ntn) * TN3;
     const long k_begin = (long)slice * k_per_slice;
     const long k_end = min(K, k_begin + k_per_slice);
 
@@ -372,75 +377,81 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
     const int wm = wave >> 1, wn = wave & 1;
     const int quarter = lane >> 4, l16 = lane & 15;
 
-    const float2 *pa[4], *pb[4];
+    const float2 *pa[4], *pb[WN];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int m = m0 + wm * 64 + i * 16 + l16;
         pa[i] = L + (long)(m < M ? m : 0) * K;
-        const int nn = n0 + wn * 64 + i * 16 + l16;
-        pb[i] = Rk + (long)(nn < M ? nn : 0) * K;
+    }
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+        const int nn = n0 + wn * 16 * WN + j * 16 + l16;
+        pb[j] = Rk + (long)(nn < M ? nn : 0) * K;
     }
 
-    floatx4 k1[4][4], k2[4][4], k3[4][4];
+    floatx4 k1[4][WN], k2[4][WN], k3[4][WN];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < WN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) k1[i][j][r] = k2[i][j][r] = k3[i][j][r] = 0.f;
 
-    float4 A[NB3][4], B[NB3][4];
+    float4 A[NB3][4], B[NB3][WN];
     const long k_last = k_end - 2;
-    auto load = [&](long kc, float4 (&a)[4], float4 (&bb)[4], bool masked) {
+    auto load = [&](long kc, float4 (&a)[4], float4 (&bb)[WN], bool masked) {
         const long kk = kc + quarter * 2;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        auto one = [&](const float2 *row) -> float4 {
             if constexpr (ALIGNED) {
                 const long kc_ = kk < k_end ? kk : k_last;
-                float4 x = *reinterpret_cast<const float4 *>(pa[i] + kc_);
-                float4 y = *reinterpret_cast<const float4 *>(pb[i] + kc_);
-                if (masked && kk >= k_end) x = y = make_float4(0.f, 0.f, 0.f, 0.f);
-                a[i] = x;
-                bb[i] = y;
+                float4 x = *reinterpret_cast<const float4 *>(row + kc_);
+                if (masked && kk >= k_end) x = make_float4(0.f, 0.f, 0.f, 0.f);
+                return x;
             } else {
                 const long k0 = kk < k_end ? kk : k_end - 1;
                 const long k1_ = kk + 1 < k_end ? kk + 1 : k_end - 1;
-                float2 x0 = pa[i][k0], x1 = pa[i][k1_], y0 = pb[i][k0], y1 = pb[i][k1_];
-                if (masked && kk >= k_end) x0 = y0 = make_float2(0.f, 0.f);
-                if (masked && kk + 1 >= k_end) x1 = y1 = make_float2(0.f, 0.f);
-                a[i] = make_float4(x0.x, x0.y, x1.x, x1.y);
-                bb[i] = make_float4(y0.x, y0.y, y1.x, y1.y);
+                float2 x0 = row[k0], x1 = row[k1_];
+                if (masked && kk >= k_end) x0 = make_float2(0.f, 0.f);
+                if (masked && kk + 1 >= k_end) x1 = make_float2(0.f, 0.f);
+                return make_float4(x0.x, x0.y, x1.x, x1.y);
             }
-        }
+        };
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = one(pa[i]);
+#pragma unroll
+        for (int j = 0; j < WN; ++j) bb[j] = one(pb[j]);
     };
-    auto mul = [&](const float4 (&a)[4], const float4 (&bb)[4]) {
+    auto mul = [&](const float4 (&a)[4], const float4 (&bb)[WN]) {
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
-            float ar[4], ai[4], d[4], br[4], u[4], w[4];
+            float ar[4], ai[4], d[4], br[WN], u[WN], w[WN];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 ar[i] = g ? a[i].z : a[i].x;
                 ai[i] = g ? a[i].w : a[i].y;
                 d[i] = ar[i] - ai[i];
-                br[i] = g ? bb[i].z : bb[i].x;
-                const float bi = g ? bb[i].w : bb[i].y;
-                u[i] = bi - br[i];
-                w[i] = br[i] + bi;
+            }
+#pragma unroll
+            for (int j = 0; j < WN; ++j) {
+                br[j] = g ? bb[j].z : bb[j].x;
+                const float bi = g ? bb[j].w : bb[j].y;
+                u[j] = bi - br[j];
+                w[j] = br[j] + bi;
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < WN; ++j)
                     k1[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(d[i], br[j], k1[i][j], 0, 0, 0);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < WN; ++j)
                     k2[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], u[j], k2[i][j], 0, 0, 0);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < WN; ++j)
                     k3[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], w[j], k3[i][j], 0, 0, 0);
         }
     };
@@ -475,11 +486,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < WN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = m0 + wm * 64 + i * 16 + 4 * quarter + r;
-                const int cc = n0 + wn * 64 + j * 16 + l16;
+                const int cc = n0 + wn * 16 * WN + j * 16 + l16;
                 if (row < M && cc < M)
                     P[(long)row * M + cc] =
                         make_float2(k1[i][j][r] + k3[i][j][r], k1[i][j][r] + k2[i][j][r]);
@@ -629,8 +640,8 @@ int launch_kubo_gemm(const float2 *L, const float2 *Rk, long K, int M, int nslic
     // JKPM_KUBO_GEMM=4 selects the four-product kernel (comparison, tuning)
     const char *env = getenv("JKPM_KUBO_GEMM");
     const bool four = env && atoi(env) == 4;
-    const dim3 grid(ntm * ntn * nslices), block(kBlock);
     const bool al = K % 2 == 0;
+    const dim3 grid(ntm * ntn * nslices), block(kBlock);
     if (four) {
         if (al)
             hipLaunchKernelGGL(kubo_gemm_kernel<true>, grid, block, 0, s, L, Rk, K, M, ntm, ntn,
@@ -639,10 +650,10 @@ int launch_kubo_gemm(const float2 *L, const float2 *Rk, long K, int M, int nslic
             hipLaunchKernelGGL(kubo_gemm_kernel<false>, grid, block, 0, s, L, Rk, K, M, ntm, ntn,
                                nslices, kps, part);
     } else if (al) {
-        hipLaunchKernelGGL(kubo_gemm3_kernel<true>, grid, block, 0, s, L, Rk, K, M, ntm, ntn,
+        hipLaunchKernelGGL((kubo_gemm3_kernel<true, 4>), grid, block, 0, s, L, Rk, K, M, ntm, ntn,
                            nslices, kps, part);
     } else {
-        hipLaunchKernelGGL(kubo_gemm3_kernel<false>, grid, block, 0, s, L, Rk, K, M, ntm, ntn,
+        hipLaunchKernelGGL((kubo_gemm3_kernel<false, 4>), grid, block, 0, s, L, Rk, K, M, ntm, ntn,
                            nslices, kps, part);
     }
     JKPM_CHECK_LAUNCH("kubo_gemm_kernel");
