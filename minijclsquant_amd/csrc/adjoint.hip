@@ -140,10 +140,12 @@ __global__ __launch_bounds__(kBlock) void grad_fold_kernel(
                 j[s] = live ? col[(long)s * n + row] : 0;
             }
             if (live) {
+                // unrolled so that several moments' loads are in flight at once
+#pragma unroll 4
                 for (int q = 0; q < count; ++q) {
                     const double f = (k_lo + q == 1) ? 1.0 : 2.0;
-                    const V lv_ = *reinterpret_cast<const V *>(lam + q * vstride * 2 +
-                                                               row * ld * 2 + voff);
+                    const V lv_ = __builtin_nontemporal_load(reinterpret_cast<const V *>(
+                        lam + q * vstride * 2 + row * ld * 2 + voff));
                     V tv[KT];
 #pragma unroll
                     for (int s = 0; s < KT; ++s)
