@@ -62,6 +62,10 @@ def _store(tab):
         pass  # a read-only home just means re-tuning next time
 
 
+_MIN_TIMED_US = 2000.0  # timed span per candidate the tuner aims for
+_MAX_TUNE_STEPS = 256
+
+
 def tuned(h: "nat.DeviceELL", v_r, steps: int = 8, retune: bool = False,
           doubling: bool = False):
     """Applies the cached (or freshly tuned) launch shape to handle h for the
@@ -80,6 +84,12 @@ def tuned(h: "nat.DeviceELL", v_r, steps: int = 8, retune: bool = False,
             h.set_launch(v, m, doubling)
             return v, m
         us = h.autotune(v_r, steps, doubling=doubling)
+        # Short steps: time again over about 2 ms per candidate, so that
+        # shapes a few percent apart are told apart.
+        if us * steps < _MIN_TIMED_US:
+            steps2 = min(_MAX_TUNE_STEPS, int(_MIN_TIMED_US / max(us, 1e-3)) + 1)
+            if steps2 > steps:
+                us = h.autotune(v_r, steps2, doubling=doubling)
         v, m = h.get_launch(doubling)
         tab[key] = {"variant": v, "chunk_mult": m, "step_us": us}
         _store(tab)
