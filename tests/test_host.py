@@ -305,3 +305,46 @@ def test_lattice_stride():
     rows = np.repeat(np.arange(n), 4)
     cols = rng.integers(0, n, size=4 * n)
     assert lattice_stride(rows, cols, n) == 0
+
+
+def test_tuned_times_short_steps_longer(tmp_path, monkeypatch):
+    """autotune.tuned: a first pass whose timed span is under 2 ms (8 steps of
+    25 us) is repeated with enough steps for about 2 ms per candidate (81),
+    at most 256; a long step is tuned once; a cached shape is not re-tuned.
+    A stand-in handle records the step counts it is asked to time."""
+    from minijclsquant_amd import autotune
+
+    monkeypatch.setenv("JKPM_TUNE_TABLE", str(tmp_path / "table.json"))
+    monkeypatch.delenv("JKPM_LAUNCH", raising=False)
+    monkeypatch.setattr(autotune, "_key", lambda h, R, d: f"k|R={R}|{d}")
+
+    class Handle:
+        def __init__(self, us):
+            self.us, self.calls, self.launch = us, [], (0, 8)
+
+        def autotune(self, v_r, steps, doubling=False):
+            self.calls.append(steps)
+            self.launch = (5, 4)
+            return self.us
+
+        def get_launch(self, doubling=False):
+            return self.launch
+
+        def set_launch(self, v, m, doubling=False):
+            self.launch = (v, m)
+
+    class Vecs:
+        shape = (100, 4)
+
+    h = Handle(25.0)
+    assert autotune.tuned(h, Vecs()) == (5, 4)
+    assert h.calls == [8, 81]
+    h = Handle(300.0)
+    autotune.tuned(h, Vecs(), doubling=True)
+    assert h.calls == [8]
+    monkeypatch.setattr(autotune, "_key", lambda h, R, d: "tiny")
+    h = Handle(0.5)
+    autotune.tuned(h, Vecs())
+    assert h.calls == [8, 256]
+    h = Handle(25.0)
+    assert autotune.tuned(h, Vecs()) == (5, 4) and h.calls == []
