@@ -195,3 +195,30 @@ def test_autograd_keep_mode_matches_checkpointed(torch_gpu):
         grads.append((eps.grad.cpu().numpy(), hop.grad.cpu().numpy()))
     for a, b in zip(grads[0], grads[1]):
         assert np.max(np.abs(a - b)) <= 1e-12 * max(1.0, np.max(np.abs(b)))
+
+
+def test_gradients_independent_of_forward_launch_shape(torch_gpu):
+    """The adjoint steps and folds take their own launch width, so the
+    gradient is the same (to rounding of the forward's partial sums, which
+    the gradient does not use) whatever launch shape the forward is set to:
+    2 and 32 work-groups per CU, two variants; checkpointed and kept-stack."""
+    torch = torch_gpu
+    A, B = square_params(24, seed=7)
+    N, K, M, R = A.shape[0], A.len_row, 65, 8
+    r = np.stack([jcl.random_vector_generator(N, 90 + s) for s in range(R)])
+    rng = np.random.default_rng(9)
+    g = rng.normal(size=(M, R)) + 1j * rng.normal(size=(M, R))
+    h = nat.DeviceELL(A.data, A.indices, N, K, 1 / B, "c128")
+    rd = torch.from_numpy(r.T.copy()).cuda()
+    gd = torch.from_numpy(g).cuda()
+    want = sum(O.grad_values(M, A.data * (1 / B), A.indices, K, r[k], np.conj(g[:, k]))
+               for k in range(R)) * (1 / B)
+    ref = None
+    for variant, mult in ((0, 2), (0, 32), (4, 2), (4, 32)):
+        h.set_launch(variant, mult)
+        for seg in (0, -1):
+            got = h.grad_values(rd, M, gd, seg=seg).cpu().numpy()
+            assert np.max(np.abs(got - want)) / np.max(np.abs(want)) <= 1e-10, (variant, mult, seg)
+            if ref is None:
+                ref = got
+            assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) <= 1e-13, (variant, mult, seg)
