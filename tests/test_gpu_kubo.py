@@ -160,3 +160,24 @@ def test_gemm_three_and_four_products_agree(torch_gpu, monkeypatch, L, R, M, odd
     monkeypatch.setenv("JKPM_KUBO_GEMM", "4")
     mu4 = jcl.kpm_kubo_moments(A, M, r, v_data=v).cpu().numpy()
     assert np.max(np.abs(mu3 - mu4)) / np.max(np.abs(mu4)) <= 2e-6
+
+
+def test_mu_nm_independent_of_launch_widths(torch_gpu, monkeypatch):
+    """The stack steps' work-group count (JKPM_KUBO_CHUNK_MULT) leaves every
+    stack vector, and so mu_nm, bitwise unchanged; the GEMM's split-K slice
+    count (JKPM_KUBO_SLICES) only regroups FP32 partial sums: 2e-6 of
+    max |mu|."""
+    A, _ = lattice(24, seed=24)
+    r = vectors(A.shape[0], 4, 41)
+    v = jcl.modifier_velocity(A, "x")
+    for name in ("JKPM_KUBO_CHUNK_MULT", "JKPM_KUBO_SLICES", "JKPM_KUBO_GEMM"):
+        monkeypatch.delenv(name, raising=False)
+    ref = jcl.kpm_kubo_moments(A, 128, r, v_data=v).cpu().numpy()
+    for mult in ("2", "4", "32"):
+        monkeypatch.setenv("JKPM_KUBO_CHUNK_MULT", mult)
+        assert np.array_equal(jcl.kpm_kubo_moments(A, 128, r, v_data=v).cpu().numpy(), ref), mult
+    monkeypatch.delenv("JKPM_KUBO_CHUNK_MULT")
+    for slices in ("8", "64", "1024"):
+        monkeypatch.setenv("JKPM_KUBO_SLICES", slices)
+        got = jcl.kpm_kubo_moments(A, 128, r, v_data=v).cpu().numpy()
+        assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) <= 2e-6, slices
