@@ -18,10 +18,11 @@ constexpr bool kNTM = false;
 #endif
 
 // Single-form kernels keep the first-step accumulators (F0) in every
-// variant: without them the tuned plain kernel measured slower (232 against
-// 210 us per launch on the headline workload, at 68 against 80 VGPRs).  The
-// doubling-form kernels drop them (86 against 104 VGPRs, 5 waves per SIMD
-// instead of 4, 179 against 186 us per launch).
+// variant: without them the complex64 plain kernel measured slower (242.5
+// against 208.3 us per launch on the headline workload at the same launch
+// shape, with 68 against 80 VGPRs), and the complex128 ones gained nothing.
+// The doubling-form kernels drop them (86 against 104 VGPRs, 5 waves per
+// SIMD instead of 4, 179 against 186 us per launch).
 template <int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool NS = false, bool NTM = false,
           bool F0 = true>
 const void *ptr() {
