@@ -7,6 +7,8 @@ against the oracle's FP64 contraction of the same mu <= 1e-10 relative.
 """
 import numpy as np
 import pytest
+import scipy.sparse as sp
+from numpy.polynomial import chebyshev as C
 
 import minijclsquant_amd as jcl
 from minijclsquant_amd import _native as nat
@@ -99,6 +101,31 @@ def test_kpm_sigma_end_to_end(torch_gpu):
     s2 = jcl.kpm_sigma(A, M, r, "x", energies=E, kind="g_delta")
     want2 = O.kubo_sigma(ref_mu, E / B, B, "g_delta").real
     assert np.max(np.abs(s2[:, 1] - want2)) / np.max(np.abs(want2)) <= 1e-4
+
+
+def test_kpm_sigma_trace_vs_exact_diagonalisation(torch_gpu):
+    """Every basis vector as a random vector (R = N = 32): kpm_sigma at
+    physical energies E equals (1/N) sum_ab |<a|v|b>|^2 d_a d_b with
+    d_a = sum_m g_m delta_m(E/B) T_m(eps_a) from dense eigh of H/B, i.e.
+    (1/N) Tr[v delta_K(E/B - H~) v delta_K(E/B - H~)].  Catches energies not
+    rescaled by B and a kernel applied on one side only."""
+    A, B = lattice(4, seed=5)
+    N, K = A.shape[0], A.len_row
+    rows = np.arange(A.data.size) // K
+    Ht = sp.csr_matrix((A.data / B, (rows, A.indices)), shape=(N, N))
+    V = sp.csr_matrix((jcl.modifier_velocity(A, "x"), (rows, A.indices)), shape=(N, N))
+    M = 40
+    E = np.linspace(-B + 0.3, B - 0.3, 23)
+    s = jcl.kpm_sigma(A, M, np.eye(N, dtype=np.complex128), "x", energies=E)
+    lam, U = np.linalg.eigh(Ht.toarray())
+    vab2 = np.abs(U.conj().T @ V.toarray() @ U) ** 2
+    g = O.jackson(M)
+    want = []
+    for e in E / B:
+        d = C.chebval(lam, g * O.moments_delta(e, M))
+        want.append(d @ vab2 @ d / N)
+    want = np.array(want)
+    assert np.max(np.abs(s[:, 1] - want)) <= 1e-4 * np.max(np.abs(want))
 
 
 def test_kubo_argument_errors(torch_gpu):

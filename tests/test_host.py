@@ -136,6 +136,90 @@ def test_modifiers_match_oracle():
     assert np.allclose(got, shift / 4.0, rtol=1e-15, atol=0)
 
 
+def _reference_light_factor(mode, params, t, dx, dy):
+    """The elementwise factor each light mode of jclsquant/modifiers.py:133-181
+    hands to vec_mul, written out per mode as the reference spells it."""
+    if mode == "linear":
+        A0, w = params
+        return np.exp(1j * A0 * np.sin(w * t) * dy)
+    if mode == "linear_packed":
+        A0, w, Tp = params
+        return np.exp(1j * A0 * np.sin(w * t) * dy / np.cosh((t - 2 * Tp) / (0.5673 * Tp)))
+    if mode == "circle":
+        A0, w, pol = params
+        s = 1.0 if pol == "r" else -1.0
+        return np.exp(1j * A0 * (np.cos(w * t) * dx + s * np.sin(w * t) * dy))
+    A0, w, pol, Tp = params
+    s = 1.0 if pol == "r" else -1.0
+    return np.exp(1j * A0 * (np.cos(w * t) * dx + s * np.sin(w * t) * dy)
+                  / np.cosh((t - 2 * Tp) / (0.5673 * Tp)))
+
+
+def _check_light_phases(phases):
+    """Hand-computed (ax, ay) of each mode at special times, where cos/sin
+    and the envelope 1/cosh((t - 2 Tp)/(0.5673 Tp)) take known values."""
+    A0, w, Tp = 0.8, 1.7, 3.0
+    quarter = np.pi / (2 * w)  # cos(w t) = 0, sin(w t) = 1
+    sech1 = 1 / np.cosh(1.0)   # 0.6480542736638853...
+    cases = [
+        (("linear", (A0, w), 0.0), (0.0, 0.0)),
+        (("linear", (A0, w), quarter), (0.0, A0)),
+        (("circle", (A0, w, "r"), 0.0), (A0, 0.0)),
+        (("circle", (A0, w, "r"), quarter), (0.0, A0)),
+        (("circle", (A0, w, "l"), quarter), (0.0, -A0)),
+        (("circle_packed", (A0, w, "r", Tp), 2 * Tp), (A0 * np.cos(2 * w * Tp),
+                                                       A0 * np.sin(2 * w * Tp))),
+        (("circle_packed", (A0, w, "l", Tp), 2 * Tp + 0.5673 * Tp),
+         (A0 * np.cos(w * 2.5673 * Tp) * sech1, -A0 * np.sin(w * 2.5673 * Tp) * sech1)),
+        (("linear_packed", (A0, w, Tp), 2 * Tp - 0.5673 * Tp),
+         (0.0, A0 * np.sin(w * 1.4327 * Tp) * sech1)),
+    ]
+    for (mode, params, t), (ax, ay) in cases:
+        got = phases(mode, params, t)
+        if not (abs(got[0] - ax) <= 1e-14 and abs(got[1] - ay) <= 1e-14):
+            return False
+    return True
+
+
+def test_light_phases_pinned():
+    """light_phases against hand-computed values at special times and against
+    the reference's elementwise factor for every mode and polarisation; a
+    swapped 'l' sign, a changed envelope constant or cos/sin swapped fail."""
+    assert _check_light_phases(jcl.light_phases)
+    rng = np.random.default_rng(3)
+    dx, dy = rng.normal(size=64) * 0.14, rng.normal(size=64) * 0.14
+    for mode, params in (("linear", (2.1, 0.7)), ("linear_packed", (2.1, 0.7, 4.0)),
+                         ("circle", (2.1, 0.7, "r")), ("circle", (2.1, 0.7, "l")),
+                         ("circle_packed", (2.1, 0.7, "r", 4.0)),
+                         ("circle_packed", (2.1, 0.7, "l", 4.0))):
+        for t in (0.0, 0.9, 3.3, 8.0, 11.7):
+            ax, ay = jcl.light_phases(mode, params, t)
+            got = np.exp(1j * (ax * dx + ay * dy))
+            want = _reference_light_factor(mode, params, t, dx, dy)
+            assert np.max(np.abs(got - want)) <= 1e-14, (mode, params, t)
+    assert jcl.light_phases("circle", (1.0, 1.0, "x"), 0.0) is None
+    assert jcl.light_phases("square", (1.0, 1.0), 0.0) is None
+
+    def l_sign_swapped(mode, params, t):
+        ax, ay = jcl.light_phases(mode, params, t)
+        return (ax, -ay) if mode.startswith("circle") and params[2] == "l" else (ax, ay)
+
+    def envelope_constant(mode, params, t):
+        ax, ay = jcl.light_phases(mode, params, t)
+        if mode.endswith("packed"):
+            Tp = params[-1]
+            f = np.cosh((t - 2 * Tp) / (0.5673 * Tp)) / np.cosh((t - 2 * Tp) / (0.567 * Tp))
+            return ax * f, ay * f
+        return ax, ay
+
+    def cos_sin_swapped(mode, params, t):
+        ax, ay = jcl.light_phases(mode, params, t)
+        return (ay, ax) if mode.startswith("circle") else (ax, ay)
+
+    for bad in (l_sign_swapped, envelope_constant, cos_sin_swapped):
+        assert not _check_light_phases(bad), bad.__name__
+
+
 def test_kernel_and_moments_match_oracle():
     for M in (1, 3, 64, 1000):
         assert np.max(np.abs(jcl.JacksonKernel(M) - O.jackson(M))) <= 1e-14

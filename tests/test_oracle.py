@@ -11,6 +11,8 @@ so every oracle function is checked against an independent route:
   csr_to_ell      round trip to CSR + the padding rule
   aAxby/vdot/dot  scipy / numpy products
   mu_nm           dense eigh brute force, Hermitian symmetry, v = I identity
+  kubo_sigma      greenwood: eigenbasis form sum_ab |v_ab|^2 d_a d_b of the
+                  trace (dense eigh + chebval); g_delta: eval_chebyu rank-1 forms
   grad_values     central finite differences in complex128
 Each check is chosen so a dropped term, a wrong sign/index or a transposed
 operand fails it (see the planted-mistake tests at the end).
@@ -273,6 +275,47 @@ def test_table_G_delta_vs_chebyshev_u():
         dl = O.moments_delta(e, M) * g
         s_rank.append(G @ mu @ dl - dl @ mu @ G)
     assert np.max(np.abs(s_table - np.array(s_rank))) <= 1e-11 * np.max(np.abs(s_rank))
+
+
+def _greenwood_exact(Ht, V, E, M, g):
+    """Spectral form of the Kubo-Greenwood trace at rescaled energies E:
+    Tr[v D v D] = sum_ab |<a|v|b>|^2 d_a d_b with D = sum_m c_m T_m(H~),
+    c = g * delta(E), d_a = chebval(eps_a, c) from dense eigh."""
+    lam, U = np.linalg.eigh(Ht.toarray())
+    vab2 = np.abs(U.conj().T @ V.toarray() @ U) ** 2
+    out = []
+    for e in E:
+        d = C.chebval(lam, g * O.moments_delta(e, M))
+        out.append(d @ vab2 @ d)
+    return np.array(out)
+
+
+def test_kubo_sigma_greenwood_vs_exact_diagonalisation():
+    """sigma(E~) = sum_mn a_m a_n mu_mn (a = Jackson * delta) with mu_mn the
+    trace Tr[v T_m v T_n] (the oracle's mu_nm summed over every basis vector
+    r = e_i) equals the eigenbasis form sum_ab |v_ab|^2 d_a(E~) d_b(E~) of
+    Tr[v delta_K(E~-H~) v delta_K(E~-H~)]; a Jackson kernel dropped on one
+    side or a lost factor 2 of the n >= 1 terms fails it."""
+    Ht, V = _velocity_pair(4, W=1.3, seed=11)
+    N = Ht.shape[0]
+    hd, hi, _, hk = ell_of(Ht)
+    vd, vi, _, vk = ell_of(V)
+    M = 40
+    mu = sum(O.mu_nm(M, hd, hi, hk, vd, vi, vk, np.eye(N)[i]) for i in range(N))
+    E = np.array([-0.93, -0.61, -0.22, 0.0, 0.17, 0.44, 0.8, 0.95])
+    g = O.jackson(M)
+    want = _greenwood_exact(Ht, V, E, M, g)
+    got = O.kubo_sigma(mu, E, 3.1, "greenwood")
+    assert np.max(np.abs(got.imag)) <= 1e-10 * np.max(np.abs(want))
+    assert np.max(np.abs(got.real - want)) <= 1e-10 * np.max(np.abs(want))
+    # planted mistakes in the contraction
+    one_sided = np.array([(g * O.moments_delta(e, M)) @ mu @ O.moments_delta(e, M) for e in E])
+    assert np.max(np.abs(one_sided - want)) > 1e-2 * np.max(np.abs(want))
+    half = np.ones(M)
+    half[1:] = 0.5
+    no_two = np.array([(half * g * O.moments_delta(e, M)) @ mu @ (half * g * O.moments_delta(e, M))
+                       for e in E])
+    assert np.max(np.abs(no_two - want)) > 1e-2 * np.max(np.abs(want))
 
 
 def test_rec_A_vec_vs_eigh_chebval():
