@@ -221,6 +221,14 @@ __global__ void zero_kernel(double2 *__restrict__ p, long cnt) {
     if (e < cnt) p[e] = make_double2(0.0, 0.0);
 }
 
+// Clears cnt 16-byte units and, if tail8, the 8 bytes after them (a vector
+// batch of an odd number of complex64 elements).
+__global__ void zero_tail_kernel(double2 *__restrict__ p, long cnt, int tail8) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < cnt) p[e] = make_double2(0.0, 0.0);
+    else if (e == cnt && tail8) reinterpret_cast<double *>(p + cnt)[0] = 0.0;
+}
+
 template <typename Real, int VPL, int LPR>
 struct Fns {
     static const void *step() { return reinterpret_cast<const void *>(&plain_step_kernel<Real, VPL, LPR>); }
@@ -353,6 +361,21 @@ int launch_zero(double2 *p, long cnt, hipStream_t s) {
     hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((cnt + kBlock - 1) / kBlock)), dim3(kBlock),
                        0, s, p, cnt);
     JKPM_CHECK_LAUNCH("zero_kernel");
+    return JKPM_OK;
+}
+
+int launch_zero_bytes(void *p, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return JKPM_OK;
+    if (bytes % 8 != 0 || ((uintptr_t)p & 15u) != 0) {
+        set_error("launch_zero_bytes: 16-byte aligned start and a multiple of 8 bytes required");
+        return JKPM_ERR_ARG;
+    }
+    const long cnt = (long)(bytes / 16);
+    const int tail8 = (bytes % 16) != 0;
+    const long threads = cnt + tail8;
+    hipLaunchKernelGGL(zero_tail_kernel, dim3((unsigned)((threads + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, (double2 *)p, cnt, tail8);
+    JKPM_CHECK_LAUNCH("zero_tail_kernel");
     return JKPM_OK;
 }
 

@@ -188,7 +188,7 @@ int run_series(const Ell &H, const Ell *V, const void *v, int R, int M, const do
     const bool comm = V != nullptr;
     const size_t vb = (size_t)H.n * R * cplx_bytes(H.vtype);
     if (comm && tab) {
-        int rc = launch_zero((double2 *)tab, (long)(vb / 16), s);  // tab[0] = c_0 C_0 = 0
+        int rc = launch_zero_bytes(tab, vb, s);  // tab[0] = c_0 C_0 = 0
         if (rc) return rc;
     }
     auto tab_at = [&](int m) { return tab ? (void *)((char *)tab + (size_t)m * vb) : nullptr; };

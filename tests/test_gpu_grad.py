@@ -177,6 +177,38 @@ def test_keep_stack_forward_and_kept_adjoint(torch_gpu):
     assert np.max(np.abs(got - want)) / np.max(np.abs(want)) <= 1e-10
 
 
+@pytest.mark.parametrize("R", [1, 3])
+def test_grad_complex64_odd_batch_ignores_stale_workspace(torch_gpu, R):
+    """complex64 handle with N*R odd (square 7 x 7, N = 49): the adjoint's
+    zero carry vectors lambda_M, lambda_{M+1} are cleared to the last 8-byte
+    element, so a workspace left full of NaN does not leak into the gradient.
+    Checked against the complex128 gradient (complex64 recursion: 1e-4
+    relative) for the checkpointed, keep-all and kept-stack forms."""
+    torch = torch_gpu
+    A, B = square_params(7, seed=21)
+    N, K, M = A.shape[0], A.len_row, 29
+    assert (N * R) % 2 == 1
+    r = np.stack([jcl.random_vector_generator(N, 90 + s) for s in range(R)])
+    g = np.random.default_rng(R).normal(size=(M, R)) + 0.5j
+    gd = torch.from_numpy(g).cuda()
+    h128 = nat.DeviceELL(A.data, A.indices, N, K, 1 / B, "c128")
+    ref = h128.grad_values(torch.from_numpy(r.T.copy()).cuda(), M, gd).cpu().numpy()
+    h64 = nat.DeviceELL(A.data, A.indices, N, K, 1 / B, "c64")
+    r64 = torch.from_numpy(r.T.astype(np.complex64).copy()).cuda()
+    scale = np.max(np.abs(ref))
+    for seg in (0, -1):
+        ws = torch.full((h64.grad_workspace_bytes(R, M, seg),), 255, dtype=torch.uint8,
+                        device="cuda")  # every float a NaN
+        got = h64.grad_values(r64, M, gd, seg=seg, workspace=ws).cpu().numpy()
+        assert np.all(np.isfinite(got)), seg
+        assert np.max(np.abs(got - ref)) / scale <= 1e-4, seg
+    _, stack = h64.moments_keep(r64, M)
+    ws = torch.full((1 << 22,), 255, dtype=torch.uint8, device="cuda")
+    got = h64.grad_values_kept(r64, M, gd, stack, workspace=ws).cpu().numpy()
+    assert np.all(np.isfinite(got))
+    assert np.max(np.abs(got - ref)) / scale <= 1e-4
+
+
 def test_autograd_keep_mode_matches_checkpointed(torch_gpu):
     """chebyshev_moments with seg=None (the forward keeps every T_n) gives the
     gradients of the sqrt(M)-checkpointed backward."""
