@@ -435,8 +435,16 @@ TileEntry pick_tile(int vtype, int rb, int k, bool dbl, bool wide) {
                 default: break;
             }
         }
-        if (rb == 1) return {tile_fn2<float, 1, 1>(k, dbl), 1, 1};
-        return {nullptr, 0, 0};
+        // 8-byte lanes (one complex64 per lane): an odd leading dimension
+        // leaves a row's vectors only 8-byte aligned.
+        switch (rb) {
+            case 16: return {tile_fn2<float, 1, 16>(k, dbl), 1, 16};
+            case 8: return {tile_fn2<float, 1, 8>(k, dbl), 1, 8};
+            case 4: return {tile_fn2<float, 1, 4>(k, dbl), 1, 4};
+            case 2: return {tile_fn2<float, 1, 2>(k, dbl), 1, 2};
+            case 1: return {tile_fn2<float, 1, 1>(k, dbl), 1, 1};
+            default: return {nullptr, 0, 0};
+        }
     }
     switch (rb) {
         case 16: return {tile_fn2<double, 1, 16>(k, dbl), 1, 16};
@@ -670,7 +678,9 @@ int launch_halo_gather(const jkpm_tiles &T, bool with_cols, hipStream_t s) {
 
 int launch_tile2(const TileArgs &a, hipStream_t s) {
     const jkpm_tiles &T = *a.T;
-    const bool wide = (a.rb % 2 == 0) && (a.r0 % 2 == 0);
+    // 16-byte lanes need every row's group start 16-byte aligned: an even
+    // leading dimension and an even first vector (the rule of launch_step).
+    const bool wide = (a.ld % 2 == 0) && (a.r0 % 2 == 0);
     TileEntry e = pick_tile(T.vtype, a.rb, T.k, a.doubling, wide);
     if (!e.fn) return arg_error("unsupported vector group for the tiled kernel");
     if (T.max_ext > tile_cap_rows(T.vtype, a.rb))
