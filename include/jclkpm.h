@@ -107,6 +107,20 @@ int jkpm_ell_info(const jkpm_ell *h, int *n_rows, int *len_row, int *value_type)
 int jkpm_ell_set_values(jkpm_ell *h, const double *data, double scale,
                         int src_on_device, void *stream);
 
+/* Hermiticity of the stored matrix A (the handle's scaled values):
+ * *abs_err = max over slots of |a_ij - conj(a_ji)| (the slots of a row that
+ * share a column summed, so ELL padding counts once), *max_abs = max |a_ij|.
+ * Computed on the device once per value change and cached on the handle;
+ * synchronises `stream` when it computes, and is refused under stream capture
+ * until computed.  The doubling form (JKPM_MOMENTS_DOUBLING), the Kubo double
+ * moments and the adjoint entry points require abs_err <=
+ * JKPM_HERMITIAN_RTOL * max_abs and return JKPM_ERR_ARG otherwise.  No
+ * reference counterpart: the reference's H_graphene 'ELL' matrices are
+ * non-Hermitian by a 1e-6 cross term (jclsquant/hams.py:160-162), which its
+ * plain recursion tolerates and these forms do not. */
+#define JKPM_HERMITIAN_RTOL 1e-10
+int jkpm_ell_hermiticity(const jkpm_ell *h, double *abs_err, double *max_abs, void *stream);
+
 /* Row-major complex128 copy of the values in the units they were uploaded in
  * (stored values / scale), to host or device memory (dst_on_device).  The
  * device side of ell_matrix.data (jclsquant/ell_matrix.py:119-258) after a
@@ -133,6 +147,33 @@ int jkpm_ell_peierls(jkpm_ell *dst, const jkpm_ell *src, const jkpm_ell *bonds, 
  * a and b are (re, im). */
 int jkpm_aAxby(const jkpm_ell *h, const double *a, const double *b, const void *x,
                void *y, int R, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Complex128 vector helpers (device vectors of n elements, asynchronous on
+ * `stream`; a and b are host (re, im) pairs).  They replace the OpenMP
+ * loops of jclsquant/cython_modules/blas_funs.pyx that the reference's
+ * namespace exports:
+ *   JKPM_VOP_AXPBY  y <- a x + b y   scal (:15-25, x = y, b = 0),
+ *                                    axpy (:31-43, b = 1), axpy2 (:47-59)
+ *   JKPM_VOP_MUL    z <- a y x       vec_mul  (:113-121)
+ *   JKPM_VOP_MUL2   y <- a x y       vec_mul2 (:125-137)
+ *   JKPM_VOP_MUL3   z <- a y x w     vec_mul3 (:145-157)
+ * Unused pointers may be NULL.
+ * ------------------------------------------------------------------- */
+#define JKPM_VOP_AXPBY 0
+#define JKPM_VOP_MUL 1
+#define JKPM_VOP_MUL2 2
+#define JKPM_VOP_MUL3 3
+int jkpm_zvec(int op, long long n, const double *a, const double *b, const void *x, void *y,
+              const void *w, void *z, void *stream);
+
+/* out = a * sum_i conj(x_i) y_i (conj_x = 1: vdot, blas_funs.pyx:80-91) or
+ * a * sum_i x_i y_i (conj_x = 0: dot, :94-105), complex128, FP64 sums in a
+ * fixed order; out is a device complex128 scalar.  workspace: at least
+ * jkpm_zdot_workspace bytes of device memory. */
+int jkpm_zdot_workspace(size_t *bytes);
+int jkpm_zdot(long long n, const double *a, const void *x, const void *y, int conj_x,
+              double *out, void *workspace, size_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------------------------
  * Chebyshev moments (the hot path).
@@ -238,7 +279,8 @@ int jkpm_series(const jkpm_ell *h, const jkpm_ell *v_op, const void *v, int R, i
  * Single-left-set moments mu_m = <r|T_m(A)|r> with options.
  * ------------------------------------------------------------------- */
 
-/* Doubling form: for Hermitian A only (not checked), the M moments come from
+/* Doubling form: for Hermitian A only (checked: jkpm_ell_hermiticity), the
+ * M moments come from
  * S = max(1, M/2) Chebyshev steps through mu_2k = 2<T_k|T_k> - mu_0 and
  * mu_2k+1 = 2<T_k|T_{k+1}> - mu_1 (Weisse et al., Rev. Mod. Phys. 78, 275
  * (2006)); each step reads no left vectors.  Same moments as the plain

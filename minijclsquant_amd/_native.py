@@ -27,6 +27,13 @@ JKPM_ERR_WORKSPACE = 4
 JKPM_C128 = 0
 JKPM_C64 = 1
 
+JKPM_HERMITIAN_RTOL = HERMITIAN_RTOL = 1e-10
+
+JKPM_VOP_AXPBY = 0
+JKPM_VOP_MUL = 1
+JKPM_VOP_MUL2 = 2
+JKPM_VOP_MUL3 = 3
+
 JKPM_KERNEL_NONE = 0
 JKPM_KERNEL_JACKSON = 1
 
@@ -49,6 +56,10 @@ SIGNATURES = {
     "jkpm_ell_info": (_int, [_vp, _ip, _ip, _ip]),
     "jkpm_ell_set_values": (_int, [_vp, _vp, _dbl, _int, _vp]),
     "jkpm_ell_get_values": (_int, [_vp, _vp, _int, _vp]),
+    "jkpm_ell_hermiticity": (_int, [_vp, _dp, _dp, _vp]),
+    "jkpm_zvec": (_int, [_int, _c.c_longlong, _dp, _dp, _vp, _vp, _vp, _vp, _vp]),
+    "jkpm_zdot_workspace": (_int, [_c.POINTER(_size)]),
+    "jkpm_zdot": (_int, [_c.c_longlong, _dp, _vp, _vp, _int, _vp, _vp, _size, _vp]),
     "jkpm_ell_create_csr": (_int, [_vp, _vp, _vp, _int, _int, _dbl, _int, _int, _vp,
                                    _c.POINTER(_vp)]),
     "jkpm_ell_get_indices": (_int, [_vp, _vp, _int, _vp]),
@@ -439,6 +450,20 @@ class DeviceELL:
                                  stream_ptr(stream)))
         return out
 
+    def hermiticity(self, stream=None):
+        """(max |a_ij - conj(a_ji)|, max |a_ij|) of the stored values, computed
+        once per value change on the device (jkpm_ell_hermiticity)."""
+        err, mag = _dbl(), _dbl()
+        check(lib().jkpm_ell_hermiticity(self._h, _c.byref(err), _c.byref(mag),
+                                         stream_ptr(stream)))
+        return float(err.value), float(mag.value)
+
+    def is_hermitian(self, stream=None) -> bool:
+        """True when max |a_ij - conj(a_ji)| <= HERMITIAN_RTOL * max |a_ij|, the
+        condition the doubling form, the Kubo moments and the adjoint need."""
+        err, mag = self.hermiticity(stream)
+        return err <= HERMITIAN_RTOL * mag
+
     def values(self, stream=None):
         """Row-major complex128 values [n*k] (device tensor) in the units they
         were uploaded in."""
@@ -509,9 +534,10 @@ class DeviceELL:
                                       ws.data_ptr(), ws.numel(), stream_ptr(stream)))
         return mu, stack
 
-    def grad_values_kept(self, r, M: int, grad_mu, stack, stream=None):
+    def grad_values_kept(self, r, M: int, grad_mu, stack, workspace=None, stream=None):
         """grad_values from the vectors moments_keep left in stack: the
-        adjoint without a forward pass (jkpm_moments_grad_kept)."""
+        adjoint without a forward pass (jkpm_moments_grad_kept).  workspace:
+        optional uint8 device tensor of at least the required size."""
         torch = _torch()
         R = self._check_vecs(r)
         M = int(M)
@@ -522,7 +548,9 @@ class DeviceELL:
             raise ValueError("stack must be the one moments_keep filled for these r and M")
         need = _size()
         check(lib().jkpm_grad_kept_workspace(self._h, R, M, _c.byref(need)))
-        ws = torch.empty(int(need.value), dtype=torch.uint8, device=r.device)
+        ws = workspace
+        if ws is None or ws.numel() < int(need.value):
+            ws = torch.empty(int(need.value), dtype=torch.uint8, device=r.device)
         out = torch.empty(self.n * self.k, dtype=torch.complex128, device=r.device)
         check(lib().jkpm_moments_grad_kept(self._h, r.data_ptr(), R, M, g.data_ptr(),
                                            stack.data_ptr(), out.data_ptr(), ws.data_ptr(),
@@ -534,18 +562,22 @@ class DeviceELL:
         check(lib().jkpm_grad_workspace(self._h, int(R), int(M), int(seg), _c.byref(out)))
         return int(out.value)
 
-    def grad_values(self, r, M: int, grad_mu, seg: int = 0, stream=None):
+    def grad_values(self, r, M: int, grad_mu, seg: int = 0, workspace=None, stream=None):
         """dL/dRe(values) + i dL/dIm(values) for the row-major complex128
         values this handle was made from, given grad_mu [M, R] (complex128)
         of mu = moments(r, M)[0].  seg: checkpoint spacing (0: sqrt(M-1));
-        < 0 keeps every T_n (no recomputation, M-1 vector batches)."""
+        < 0 keeps every T_n (no recomputation, M-1 vector batches).
+        workspace: optional uint8 device tensor of at least the required size."""
         torch = _torch()
         R = self._check_vecs(r)
         M = int(M)
         g = grad_mu.to(dtype=torch.complex128).contiguous()
         if g.shape != (M, R) or not g.is_cuda:
             raise ValueError(f"grad_mu must be a complex128 device tensor [M={M}, R={R}]")
-        ws = torch.empty(self.grad_workspace_bytes(R, M, seg), dtype=torch.uint8, device=r.device)
+        ws = workspace
+        if ws is None or ws.numel() < self.grad_workspace_bytes(R, M, seg):
+            ws = torch.empty(self.grad_workspace_bytes(R, M, seg), dtype=torch.uint8,
+                             device=r.device)
         out = torch.empty(self.n * self.k, dtype=torch.complex128, device=r.device)
         check(lib().jkpm_moments_grad(self._h, r.data_ptr(), R, M, g.data_ptr(), int(seg),
                                       out.data_ptr(), ws.data_ptr(), ws.numel(),

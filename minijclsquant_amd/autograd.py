@@ -12,8 +12,9 @@ device DOS kernel and its transpose.
 hoppings by a gather, so gradients reach those parameters through torch.
 
 Gradients follow torch's convention for complex tensors:
-dL/dRe(z) + i dL/dIm(z).  The matrix is assumed Hermitian at the point of
-evaluation (the adjoint uses A^H = A); the spectral bound is a constant.
+dL/dRe(z) + i dL/dIm(z).  The matrix must be Hermitian at the point of
+evaluation (the adjoint uses A^H = A; a non-Hermitian one raises
+ValueError); the spectral bound is a constant.
 The reference computes no gradients; this extends its rec_A_tab
 (jclsquant/recurrence_functions.py:9-40) and kpm_dos
 (jclsquant/observables.py:26-54).
@@ -34,6 +35,14 @@ class _Moments(torch.autograd.Function):
         N = r.shape[0]
         K = values.numel() // N
         h = nat.DeviceELL(values.detach().contiguous(), indices, N, K, scale, "c128")
+        if values.requires_grad:
+            # the adjoint runs A in place of A^H: refuse before the forward pass
+            err, mag = h.hermiticity()
+            if not err <= nat.HERMITIAN_RTOL * mag:
+                raise ValueError(
+                    f"chebyshev_moments: the gradient needs a Hermitian matrix, but "
+                    f"max |A_ij - conj(A_ji)| = {err:.3e} exceeds "
+                    f"{nat.HERMITIAN_RTOL:.0e} * max |A_ij| = {nat.HERMITIAN_RTOL * mag:.3e}")
         if tune:
             # launch shape from the persistent table (timed once per shape)
             from . import autotune

@@ -718,10 +718,17 @@ int jkpm_ell_set_values(jkpm_ell *h, const double *data, double scale, int src_o
     clear_error();
     if (!h || !data) return arg_error("handle/data is NULL");
     // Values only: the slot-major column array stays as uploaded.
+    h->herm_err = h->herm_max = -1.0;
     int rc = upload_values(&h->e, data, scale, nullptr, src_on_device, (hipStream_t)stream);
     if (rc) return rc;
     JKPM_HIP(hipStreamSynchronize((hipStream_t)stream));
     return JKPM_OK;
+}
+
+int jkpm_ell_hermiticity(const jkpm_ell *h, double *abs_err, double *max_abs, void *stream) {
+    clear_error();
+    if (!h || !abs_err || !max_abs) return arg_error("NULL argument");
+    return hermiticity(h, (hipStream_t)stream, abs_err, max_abs);
 }
 
 int jkpm_ell_get_values(const jkpm_ell *h, double *data, int dst_on_device, void *stream) {
@@ -755,6 +762,7 @@ int jkpm_ell_peierls(jkpm_ell *dst, const jkpm_ell *src, const jkpm_ell *bonds, 
     if (src->e.col != dst->e.col || src->e.col != bonds->e.col)
         return arg_error("dst, src and bonds must share one slot pattern (jkpm_ell_create_like)");
     if (dst->e.vtype != src->e.vtype) return arg_error("dst and src must have one value type");
+    dst->herm_err = dst->herm_max = -1.0;
     int rc = launch_peierls(src->e, bonds->e, &dst->e, make_double2(a_re, a_im),
                             make_double2(ax_re, ax_im), make_double2(ay_re, ay_im),
                             (hipStream_t)stream);
@@ -771,6 +779,49 @@ int jkpm_aAxby(const jkpm_ell *h, const double *a, const double *b, const void *
     if (x == y) return arg_error("x must not alias y");
     return launch_aaxby(h->e, make_double2(a[0], a[1]), make_double2(b[0], b[1]), x, y, R,
                         (hipStream_t)stream);
+}
+
+int jkpm_zvec(int op, long long n, const double *a, const double *b, const void *x, void *y,
+              const void *w, void *z, void *stream) {
+    clear_error();
+    if (n < 0) return arg_error("n must be >= 0");
+    if (!a) return arg_error("a is NULL");
+    const bool need_x = true, need_y = true;
+    const bool need_z = op == JKPM_VOP_MUL || op == JKPM_VOP_MUL3;
+    const bool need_w = op == JKPM_VOP_MUL3;
+    if (op < JKPM_VOP_AXPBY || op > JKPM_VOP_MUL3) return arg_error("unknown vector op");
+    if (op == JKPM_VOP_AXPBY && !b) return arg_error("b is NULL");
+    if (n == 0) return JKPM_OK;
+    if ((need_x && !x) || (need_y && !y) || (need_z && !z) || (need_w && !w))
+        return arg_error("NULL vector");
+    if ((x && !aligned16(x)) || (y && !aligned16(y)) || (z && !aligned16(z)) ||
+        (w && !aligned16(w)))
+        return arg_error("device pointers must be 16-byte aligned");
+    const double2 av = make_double2(a[0], a[1]);
+    const double2 bv = b ? make_double2(b[0], b[1]) : make_double2(0.0, 0.0);
+    return zvec(op, (long)n, av, bv, x, y, w, z, (hipStream_t)stream);
+}
+
+int jkpm_zdot_workspace(size_t *bytes) {
+    clear_error();
+    if (!bytes) return arg_error("bytes is NULL");
+    *bytes = zdot_workspace();
+    return JKPM_OK;
+}
+
+int jkpm_zdot(long long n, const double *a, const void *x, const void *y, int conj_x,
+              double *out, void *workspace, size_t workspace_bytes, void *stream) {
+    clear_error();
+    if (n < 0) return arg_error("n must be >= 0");
+    if (!a || !out || !workspace || (n > 0 && (!x || !y))) return arg_error("NULL argument");
+    if (!aligned16(out) || !aligned16(workspace) || (x && !aligned16(x)) || (y && !aligned16(y)))
+        return arg_error("device pointers must be 16-byte aligned");
+    if (workspace_bytes < zdot_workspace()) {
+        set_error("workspace too small: need " + std::to_string(zdot_workspace()) + " bytes");
+        return JKPM_ERR_WORKSPACE;
+    }
+    return zdot((long)n, make_double2(a[0], a[1]), x, y, conj_x ? 1 : 0, (double2 *)out,
+                (double2 *)workspace, (hipStream_t)stream);
 }
 
 int jkpm_moments_workspace(const jkpm_ell *h, int R, int n_left, size_t *bytes) {
@@ -1169,6 +1220,10 @@ int jkpm_moments_ex(const jkpm_ell *h, const jkpm_batch *b, const void *v_r, int
     if (!aligned16(v_r) || !aligned16(workspace) || !aligned16(mu))
         return arg_error("device pointers must be 16-byte aligned");
     const bool dbl = (flags & JKPM_MOMENTS_DOUBLING) != 0;
+    if (dbl) {
+        int rc = require_hermitian(b ? b->h : h, (hipStream_t)stream, "the doubling form");
+        if (rc) return rc;
+    }
     if (b) {
         const Groups g = b->grp(dbl);
         return run_moments(b->h->e, v_r, v_r, nullptr, R, M, (double2 *)mu, workspace,
@@ -1315,8 +1370,9 @@ int reverse_sweep_kept(const Ell &H, const void *r, int R, int M, const double2 
     auto ts = [&](int q) { return (const void *)((const char *)stack + (size_t)q * vb); };
     void *C1 = w + L.C1, *C2 = w + L.C2;
     double2 *G = (double2 *)(w + L.G);
-    int rc = launch_zero((double2 *)C1, (long)(vb / 16), s);
-    if (!rc) rc = launch_zero((double2 *)C2, (long)(vb / 16), s);
+    // vb is a multiple of 8, not of 16, for an odd N*R complex64 batch
+    int rc = launch_zero_bytes(C1, vb, s);
+    if (!rc) rc = launch_zero_bytes(C2, vb, s);
     if (rc) return rc;
     for (int c = L.nseg - 1; c >= 0; --c) {
         const int lo = 1 + c * S, hi = std::min((c + 1) * S, M - 1);
@@ -1367,6 +1423,8 @@ int jkpm_moments_grad(const jkpm_ell *h, const void *r, int R, int M, const doub
         return JKPM_ERR_WORKSPACE;
     }
     hipStream_t s = (hipStream_t)stream;
+    rc = require_hermitian(h, s, "the adjoint recursion");
+    if (rc) return rc;
     char *w = (char *)workspace;
     const size_t vb = (size_t)H.n * R * cplx_bytes(H.vtype);  // bytes of one vector batch
     const int S = L.S;
@@ -1409,8 +1467,8 @@ int jkpm_moments_grad(const jkpm_ell *h, const void *r, int R, int M, const doub
                                         hipMemcpyDeviceToDevice, s));
         }
         // Reverse sweep over segments of k = 1 .. M-1.
-        rc = launch_zero((double2 *)C1, (long)(vb / 16), s);
-        if (!rc) rc = launch_zero((double2 *)C2, (long)(vb / 16), s);
+        rc = launch_zero_bytes(C1, vb, s);
+        if (!rc) rc = launch_zero_bytes(C2, vb, s);
         if (rc) return rc;
         for (int c = L.nseg - 1; c >= 0; --c) {
             const int lo = 1 + c * S, hi = std::min((c + 1) * S, M - 1);
@@ -1480,6 +1538,8 @@ int jkpm_moments_grad_kept(const jkpm_ell *h, const void *r, int R, int M, const
         return JKPM_ERR_WORKSPACE;
     }
     hipStream_t s = (hipStream_t)stream;
+    rc = require_hermitian(h, s, "the adjoint recursion");
+    if (rc) return rc;
     char *w = (char *)workspace;
     double2 *G = (double2 *)(w + L.G);
     rc = launch_zero(G, (long)H.n * H.k, s);
@@ -1576,6 +1636,9 @@ int jkpm_kubo_mu_nm(const jkpm_ell *H, const jkpm_ell *V, const void *r, int R, 
         return JKPM_ERR_WORKSPACE;
     }
     hipStream_t s = (hipStream_t)stream;
+    rc = require_hermitian(H, s, "the Kubo double moments (H)");
+    if (!rc) rc = require_hermitian(V, s, "the Kubo double moments (v)");
+    if (rc) return rc;
     char *w = (char *)workspace;
     const size_t vec = (size_t)H->e.n * R * 8;
     char *Ls = w + K.L, *Rs = w + K.Rk;
@@ -1853,6 +1916,10 @@ int jkpm_tiles_moments(const jkpm_tiles *t, const void *v_r, int R, int M, int f
     while (rb > R) rb >>= 1;
     if (rb > t->rb && t->max_ext > tile_cap_rows(t->vtype, rb))
         return arg_error("the plan was built for fewer vectors per group; rebuild it for this R");
+    if (flags & JKPM_MOMENTS_DOUBLING) {
+        int rc = require_hermitian(t->h, (hipStream_t)stream, "the doubling form");
+        if (rc) return rc;
+    }
     return run_tiles(*t, v_r, R, M, (flags & JKPM_MOMENTS_DOUBLING) != 0, (double2 *)mu,
                      workspace, workspace_bytes, (hipStream_t)stream);
 }

@@ -1,0 +1,128 @@
+// Input checks on device matrices: the Hermiticity of a handle's values.
+//
+// The doubling form of the moments, the Kubo double moments and the adjoint
+// recursion all rely on A^H = A (they use <T_k|T_k> for mu_2k, <r|v T_m v T_n|r>
+// as L^H Rk, and A in place of A^H in the reverse recursion).  The check runs
+// once per value upload over the slot pattern: for every slot (i, s) with
+// column j it compares a_ij (all slots of row i with column j summed) with
+// conj(a_ji) (all slots of row j with column i summed), so the padding slots
+// of the ELL layout, which repeat a column with value 0, do not count twice.
+// Cost O(N K^2) reads, one launch and one synchronisation per value change.
+#include "common.h"
+
+#include <cstdio>
+#include <cstring>
+
+namespace jkpm {
+namespace {
+
+constexpr int kBlock = 256;
+
+// out[0] = bits of max |a_ij - conj(a_ji)|, out[1] = bits of max |a_ij|.
+// Non-negative doubles order like their bit patterns as unsigned integers.
+template <typename Real>
+__global__ void __launch_bounds__(kBlock) herm_kernel(const Real *__restrict__ val,
+                                                      const int *__restrict__ col, int n, int k,
+                                                      unsigned long long *__restrict__ out) {
+    double err = 0.0, mag = 0.0;
+    const long tot = (long)n * k;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+         t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % n);
+        const int j = col[t];
+        double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+        for (int s = 0; s < k; ++s) {
+            const long u = (long)s * n + i;
+            if (col[u] == j) {
+                ar += (double)val[2 * u];
+                ai += (double)val[2 * u + 1];
+            }
+            const long w = (long)s * n + j;
+            if (col[w] == i) {
+                br += (double)val[2 * w];
+                bi += (double)val[2 * w + 1];
+            }
+        }
+        // a_ij - conj(a_ji)
+        const double dr = ar - br, di = ai + bi;
+        err = fmax(err, sqrt(dr * dr + di * di));
+        const double vr = (double)val[2 * t], vi = (double)val[2 * t + 1];
+        mag = fmax(mag, sqrt(vr * vr + vi * vi));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        err = fmax(err, __shfl_xor(err, off));
+        mag = fmax(mag, __shfl_xor(mag, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], (unsigned long long)__double_as_longlong(err));
+        atomicMax(&out[1], (unsigned long long)__double_as_longlong(mag));
+    }
+}
+
+}  // namespace
+
+int hermiticity(const jkpm_ell *h, hipStream_t s, double *abs_err, double *max_abs) {
+    if (h->herm_err >= 0.0) {
+        *abs_err = h->herm_err;
+        *max_abs = h->herm_max;
+        return JKPM_OK;
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    JKPM_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone)
+        return arg_error("the Hermiticity of this matrix is not known yet and cannot be checked "
+                         "under stream capture: call the entry point (or jkpm_ell_hermiticity) "
+                         "once before capturing");
+    const Ell &H = h->e;
+    unsigned long long *d = nullptr;
+    JKPM_HIP(hipMalloc(&d, 2 * sizeof(unsigned long long)));
+    unsigned long long host[2] = {0, 0};
+    hipError_t e = hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), s);
+    if (e == hipSuccess) {
+        DeviceInfo dev;
+        int rc = device_info(&dev);
+        if (rc) {
+            hipFree(d);
+            return rc;
+        }
+        const long tot = (long)H.n * H.k;
+        const long want = (tot + kBlock - 1) / kBlock;
+        const int grid = (int)std::max(1L, std::min(want, (long)dev.cu_count * 8));
+        if (H.vtype == JKPM_C64)
+            hipLaunchKernelGGL(herm_kernel<float>, dim3(grid), dim3(kBlock), 0, s,
+                               (const float *)H.val, H.col, H.n, H.k, d);
+        else
+            hipLaunchKernelGGL(herm_kernel<double>, dim3(grid), dim3(kBlock), 0, s,
+                               (const double *)H.val, H.col, H.n, H.k, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(host, d, sizeof(host), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "Hermiticity check", __FILE__, __LINE__);
+    double err, mag;
+    memcpy(&err, &host[0], sizeof(double));
+    memcpy(&mag, &host[1], sizeof(double));
+    h->herm_err = err;
+    h->herm_max = mag;
+    *abs_err = err;
+    *max_abs = mag;
+    return JKPM_OK;
+}
+
+int require_hermitian(const jkpm_ell *h, hipStream_t s, const char *what) {
+    double err = 0.0, mag = 0.0;
+    int rc = hermiticity(h, s, &err, &mag);
+    if (rc) return rc;
+    if (!(err <= JKPM_HERMITIAN_RTOL * mag)) {  // NaN values fail too
+        char buf[320];
+        snprintf(buf, sizeof(buf),
+                 "%s needs a Hermitian matrix: max |A_ij - conj(A_ji)| = %.3e exceeds "
+                 "%.0e * max |A_ij| = %.3e (use the plain recursion for non-Hermitian matrices)",
+                 what, err, JKPM_HERMITIAN_RTOL, JKPM_HERMITIAN_RTOL * mag);
+        return arg_error(buf);
+    }
+    return JKPM_OK;
+}
+
+}  // namespace jkpm

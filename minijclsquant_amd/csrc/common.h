@@ -136,6 +136,8 @@ int launch_grad_fold(const Ell &H, const void *lam, const void *tprev, int count
                      double2 *G, int R, int nchunks, hipStream_t s);
 int launch_grad_out(const double2 *G, int n, int k, double scale, double2 *out, hipStream_t s);
 int launch_zero(double2 *p, long cnt, hipStream_t s);
+// Clears `bytes` bytes (a multiple of 8) from a 16-byte aligned p, exactly.
+int launch_zero_bytes(void *p, size_t bytes, hipStream_t s);
 int launch_dos_grad(const double *w, int M, int R, const double *e, int n_e, int kernel,
                     double2 *grad_mu, hipStream_t s);
 
@@ -223,6 +225,21 @@ int launch_kubo_reduce(const float2 *part, int nslices, int M, double scale, dou
 int launch_kubo_sigma(const double2 *mu, int M, const double *e, int n_e, int kind, double bound,
                       double2 *sigma, hipStream_t s);
 
+// ---- complex128 vector helpers (blas.hip) ----
+int zvec(int op, long n, double2 a, double2 b, const void *x, void *y, const void *w, void *z,
+         hipStream_t s);
+size_t zdot_workspace();
+int zdot(long n, double2 a, const void *x, const void *y, int conj_x, double2 *out,
+         double2 *part, hipStream_t s);
+
+// ---- input checks (checks.hip) ----
+// Hermiticity of h's stored values (computed once per value change, cached on
+// the handle; synchronises s, refused under capture while unknown).
+int hermiticity(const jkpm_ell *h, hipStream_t s, double *abs_err, double *max_abs);
+// JKPM_ERR_ARG naming `what` unless max |a_ij - conj(a_ji)| <=
+// JKPM_HERMITIAN_RTOL * max |a_ij|.
+int require_hermitian(const jkpm_ell *h, hipStream_t s, const char *what);
+
 }  // namespace jkpm
 
 // The opaque handle of the C-ABI: the device matrix plus the ownership of its
@@ -231,6 +248,9 @@ int launch_kubo_sigma(const double2 *mu, int M, const double *e, int n_e, int ki
 struct jkpm_ell {
     jkpm::Ell e;
     std::shared_ptr<int> col_owner;
+    // Hermiticity of the stored values (checks.hip): max |a_ij - conj(a_ji)|
+    // and max |a_ij|, or -1 while unknown (reset whenever the values change).
+    mutable double herm_err = -1.0, herm_max = -1.0;
 };
 
 // A tile plan of a handle: rows renumbered into patches, the matrix in that
