@@ -31,7 +31,7 @@ import scipy.sparse as sp
 
 from .ell_matrix import ell_matrix
 
-__all__ = ["load_ell"]
+__all__ = ["load_ell", "H_graphene"]
 
 
 def _restore_ell(state):
@@ -168,3 +168,129 @@ def load_ell(path=None):
     if not isinstance(obj, ell_matrix):
         raise pickle.UnpicklingError(f"{path} does not hold an ell_matrix")
     return obj
+
+
+# ---------------------------------------------------------------- H_graphene
+
+_A_GRAPHENE = 0.24595             # nm, unit cell length (jclsquant/hams.py:81)
+_ACC_GRAPHENE = _A_GRAPHENE / np.sqrt(3)  # nm, carbon-carbon distance
+
+
+def _border_pairs(pos, border, Lx, Ly, a_cc, thr, chunk=2048):
+    """Periodic bonds across the flake's edges, by the rule of
+    jclsquant/hams.py:108-133: for border atoms p < q (sorted indices) with
+    |dx| >= Lx - a_cc or |dy| >= Ly - a_cc, the separation is shifted by
+    floor(|d| / L) periods (Lx + sqrt(3) a_cc / 2, Ly + a_cc / 2) towards zero
+    and kept when it is shorter than thr.  Vectorised over blocks of p.
+    Returns (p, q, dx, dy) with d = pos[p] - pos[q] after the shift."""
+    P, Q, DX, DY = [], [], [], []
+    b = border
+    nb = b.size
+    px_all, py_all = pos[b, 0], pos[b, 1]
+    for lo in range(0, nb, chunk):
+        hi = min(nb, lo + chunk)
+        ii = np.arange(lo, hi)[:, None]
+        jj = np.arange(nb)[None, :]
+        upper = jj > ii
+        dx = px_all[lo:hi, None] - px_all[None, :]
+        dy = py_all[lo:hi, None] - py_all[None, :]
+        far = upper & ((np.abs(dx) >= Lx - a_cc) | (np.abs(dy) >= Ly - a_cc))
+        if not far.any():
+            continue
+        ri, ci = np.nonzero(far)
+        dx = dx[ri, ci]
+        dy = dy[ri, ci]
+        dx = dx - np.sign(dx) * (np.abs(dx) // Lx) * (Lx + np.sqrt(3) * a_cc / 2)
+        dy = dy - np.sign(dy) * (np.abs(dy) // Ly) * (Ly + 0.5 * a_cc)
+        near = np.sqrt(dx ** 2 + dy ** 2) < thr
+        P.append(b[lo + ri[near]])
+        Q.append(b[ci[near]])
+        DX.append(dx[near])
+        DY.append(dy[near])
+    if not P:
+        e = np.zeros(0)
+        return e.astype(np.int64), e.astype(np.int64), e, e
+    return np.concatenate(P), np.concatenate(Q), np.concatenate(DX), np.concatenate(DY)
+
+
+def _conjugate_flags(H):
+    """Conjugate flags on H's own sparsity pattern: 0 for (i, j) with i < j,
+    1 for i > j, 2 on the diagonal (jclsquant/hams.py:75, 102, 141-149).  The
+    reference adds its diagonal with sparse `+=`, which drops the explicit
+    zero flags, so its flag array no longer lines up with H's values; here
+    every entry of H keeps its flag (the same matrix, zeros stored)."""
+    H = H.tocsr()
+    H.sort_indices()
+    rows = np.repeat(np.arange(H.shape[0]), np.diff(H.indptr))
+    cols = H.indices
+    flags = np.where(rows < cols, 0, np.where(rows > cols, 1, 2)).astype(np.int64)
+    return sp.csr_matrix((flags, H.indices.copy(), H.indptr.copy()), shape=H.shape)
+
+
+def H_graphene(positions, t, m=0, W=0, periodic=True, type_H='CSR', seed=None):
+    """Nearest-neighbour tight-binding graphene on the given positions, with
+    the signature and outputs of jclsquant/hams.py:63-168.
+
+    Bonds join atoms closer than 1.1 a_cc (a k-d tree, as the reference);
+    with ``periodic`` the border atoms are paired across the edges by the
+    reference's shift rule (``_border_pairs``).  Entry (i, j) of Dx/Dy is
+    x_j - x_i (y_j - y_i); the conjugate flags are 0 for i < j, 1 for i > j
+    and 2 on the diagonal, stored on H's sparsity pattern.  m adds +m on
+    even and -m on odd sites; W adds onsite W * (2 U[0,1) - 1) drawn from
+    numpy's global generator (``np.random.rand``) as the reference does, or
+    from ``default_rng(seed)`` when ``seed`` is given.
+
+    type_H 'CSR': returns (H, Dx, Dy, Omega, conjugates) with Omega = Lx Ly.
+    type_H 'ELL': adds the reference's cross terms H += 1e-6 (Dx + Dy), then
+    Dx += 1e-6 H and Dy += 1e-6 H, and returns an ell_matrix with .Omega.  That
+    matrix is not Hermitian (the 1e-6 Dx term is antisymmetric): the plain
+    recursion accepts it; the doubling form, the Kubo moments and gradients
+    refuse it."""
+    from scipy.spatial import cKDTree
+
+    pos = np.asarray(positions, dtype=np.float64)
+    N = pos.shape[0]
+    a_cc = _ACC_GRAPHENE
+    thr = 1.1 * a_cc
+    pairs = cKDTree(pos).query_pairs(thr, output_type='ndarray').astype(np.int64)
+    if pairs.size:
+        pairs = pairs[np.argsort(pairs[:, 0] * N + pairs[:, 1], kind='stable')]
+    i, j = pairs[:, 0], pairs[:, 1]
+    dx = pos[i, 0] - pos[j, 0]
+    dy = pos[i, 1] - pos[j, 1]
+    keep = np.sqrt(dx ** 2 + dy ** 2) <= thr
+    i, j, dx, dy = i[keep], j[keep], dx[keep], dy[keep]
+    Lx = np.max(pos[:, 0]) - np.min(pos[:, 0])
+    Ly = np.max(pos[:, 1]) - np.min(pos[:, 1])
+    if periodic:
+        border = np.unique(np.concatenate((
+            np.where(pos[:, 0] == np.min(pos[:, 0]))[0], np.where(pos[:, 0] == np.max(pos[:, 0]))[0],
+            np.where(pos[:, 1] == np.min(pos[:, 1]))[0], np.where(pos[:, 1] == np.max(pos[:, 1]))[0])))
+        bp, bq, bdx, bdy = _border_pairs(pos, border, Lx, Ly, a_cc, thr)
+        i = np.concatenate((i, bp))
+        j = np.concatenate((j, bq))
+        dx = np.concatenate((dx, bdx))
+        dy = np.concatenate((dy, bdy))
+    row = np.concatenate((i, j))
+    col = np.concatenate((j, i))
+    nb = i.size
+    hop = np.full(2 * nb, t)
+    H = sp.csr_matrix((hop, (row, col)), shape=(N, N))
+    Dx = sp.csr_matrix((np.concatenate((-dx, dx)), (row, col)), shape=(N, N))
+    Dy = sp.csr_matrix((np.concatenate((-dy, dy)), (row, col)), shape=(N, N))
+    if np.abs(m) != 0:
+        diag = np.where(np.arange(N) % 2 == 0, m, -m)
+        H = H + sp.diags(diag, offsets=0, shape=(N, N), format='csr')
+    if np.abs(W) != 0:
+        u = np.random.rand(N) if seed is None else np.random.default_rng(seed).random(N)
+        H = H + sp.diags(W * (2 * u - 1), offsets=0, format='csr')
+    if type_H == 'ELL':
+        H = H + 1e-6 * (Dx + Dy)
+        Dx = Dx + 1e-6 * H
+        Dy = Dy + 1e-6 * H
+        out = ell_matrix(H, Dx, Dy, _conjugate_flags(H))
+        out.Omega = Lx * Ly
+        return out
+    if type_H == 'CSR':
+        return H, Dx, Dy, Lx * Ly, _conjugate_flags(H)
+    return None

@@ -21,7 +21,6 @@ __all__ = [
     "modifier_velocity",
     "modifier_diagonal",
     "modifier_diagonal_anderson",
-    "diagonal_modifier",
     "light_phases",
     "modifier_light",
     "modifier_hoppings_c",
@@ -31,7 +30,7 @@ _LIGHT_MSG = (" You must introduce a valid id : light, circle,light_packed or ci
               "with irs valid parameters .")
 
 
-def diagonal_modifier(data, indices, valid_row_values, diagonal_array, len_row):
+def _diagonal_values(data, indices, valid_row_values, diagonal_array, len_row):
     """data_new[s] = diag[row] + data[s] on valid diagonal slots, else data[s]
     (jclsquant/cython_modules/Extra_funs.pyx:352-361)."""
     data = np.asarray(data)
@@ -50,7 +49,7 @@ def modifier_bounds(A):
     if np.abs(b[1]) == np.abs(b[0]):
         return np.copy(A.data) * (1 / ((b[1] - b[0]) / 2) + 0j)
     diag = (-0.5 * (b[1] + b[0])) * np.ones(int(A.shape[0]), dtype=np.complex128)
-    data_new = diagonal_modifier(A.data, A.indices, A.valid_row_values, diag, A.len_row)
+    data_new = _diagonal_values(A.data, A.indices, A.valid_row_values, diag, A.len_row)
     return data_new * (1 / (0.5 * (b[1] - b[0])) + 0j)
 
 
@@ -82,7 +81,7 @@ def modifier_diagonal(A, id=None, diagonal_array=None):
         return
     if len(diagonal_array) != A.shape[0]:
         print('You must introduce an array that is of the same shape as the matrix.')
-    return diagonal_modifier(A.data, A.indices, A.valid_row_values, diagonal_array, A.len_row)
+    return _diagonal_values(A.data, A.indices, A.valid_row_values, diagonal_array, A.len_row)
 
 
 def modifier_diagonal_anderson(A, W=None, seed=None):
@@ -92,7 +91,7 @@ def modifier_diagonal_anderson(A, W=None, seed=None):
         print('You must introduce a disorder strength to modify the system. ')
     rng = np.random.default_rng(seed)
     diagonal_array = (W) * (2 * rng.random(int(A.shape[0])) - 1)
-    return diagonal_modifier(A.data, A.indices, A.valid_row_values, diagonal_array, A.len_row)
+    return _diagonal_values(A.data, A.indices, A.valid_row_values, diagonal_array, A.len_row)
 
 
 def light_phases(modifier_id, modifier_params, t):

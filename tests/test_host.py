@@ -432,3 +432,124 @@ def test_tuned_times_short_steps_longer(tmp_path, monkeypatch):
     assert h.calls == [8, 256]
     h = Handle(25.0)
     assert autotune.tuned(h, Vecs()) == (5, 4) and h.calls == []
+
+
+def _min_image_bonds(S, ncx, ncy, thr):
+    """Brute-force nearest neighbours of the 4-atom-cell flake on its torus:
+    every pair, separation wrapped to the nearest periodic image (periods
+    ncx sqrt(3) a_cc and 3 ncy a_cc)."""
+    a_cc = jcl.A_CC
+    P = np.array([ncx * np.sqrt(3) * a_cc, 3 * ncy * a_cc])
+    d = S[None, :, :] - S[:, None, :]  # d[i, j] = pos_j - pos_i
+    d = d - P * np.round(d / P)
+    dist = np.sqrt((d ** 2).sum(-1))
+    i, j = np.nonzero((dist < thr) & (dist > 0))
+    return i, j, d[i, j]
+
+
+@pytest.mark.parametrize("N", [16, 64, 256, 1024])
+def test_H_graphene_periodic_bonds_vs_minimum_image(N):
+    """The border pairing of H_graphene (the shift rule of
+    jclsquant/hams.py:108-133) builds exactly the minimum-image torus: every
+    atom has three neighbours at a_cc, the hoppings are t and Dx/Dy hold the
+    wrapped x_j - x_i, y_j - y_i."""
+    S = jcl.lattice_hexagonal(N)
+    t = -2.7 + 0j
+    H, Dx, Dy, Omega, cj = jcl.H_graphene(S, t, 0, 0, True, 'CSR')
+    n = int(np.sqrt(N // 4))
+    i, j, d = _min_image_bonds(S, n, n, 1.1 * jcl.A_CC)
+    want = sp.csr_matrix((np.full(i.size, t), (i, j)), shape=(N, N))
+    assert abs(H - want).max() == 0 and H.nnz == want.nnz == 3 * N
+    wdx = sp.csr_matrix((d[:, 0], (i, j)), shape=(N, N))
+    wdy = sp.csr_matrix((d[:, 1], (i, j)), shape=(N, N))
+    assert abs(Dx - wdx).max() <= 1e-12 and abs(Dy - wdy).max() <= 1e-12
+    assert np.allclose(np.sqrt(Dx.data ** 2 + Dy.data ** 2), jcl.A_CC, rtol=1e-12)
+    Lx = (n - 0.5) * np.sqrt(3) * jcl.A_CC
+    Ly = (3 * n - 0.5) * jcl.A_CC
+    assert abs(Omega - Lx * Ly) <= 1e-12 * Omega
+    rows = np.repeat(np.arange(N), np.diff(cj.indptr))
+    assert np.array_equal(cj.indices, H.indices) and np.array_equal(cj.indptr, H.indptr)
+    assert np.array_equal(cj.data, (rows > cj.indices).astype(cj.data.dtype))
+
+
+def test_H_graphene_clean_spectrum():
+    """Clean graphene torus of 2 n^2 two-atom cells: eigenvalues
+    +-|t| |sum_d exp(i k.d)| over the NN vectors d at the k allowed by the
+    rectangular supercell (2 pi m / (n sqrt(3) a_cc), 2 pi q / (3 n a_cc)),
+    m < n, q < 2n."""
+    N = 64
+    n = 4
+    S = jcl.lattice_hexagonal(N)
+    H, _, _, _, _ = jcl.H_graphene(S, -2.7, 0, 0, True, 'CSR')
+    a = jcl.A_CC
+    deltas = np.array([[0.0, a], [np.sqrt(3) * a / 2, -a / 2], [-np.sqrt(3) * a / 2, -a / 2]])
+    ev = []
+    for mm in range(n):
+        for q in range(2 * n):
+            k = np.array([2 * np.pi * mm / (n * np.sqrt(3) * a), 2 * np.pi * q / (3 * n * a)])
+            f = abs(np.exp(1j * deltas @ k).sum())
+            ev += [2.7 * f, -2.7 * f]
+    got = np.linalg.eigvalsh(H.toarray())
+    assert np.max(np.abs(np.sort(ev) - got)) <= 1e-12
+
+
+def test_H_graphene_mass_disorder_and_ell_form():
+    """m: +m on even, -m on odd sites; W: onsite W (2u - 1) from numpy's
+    global generator (or default_rng(seed)); 'ELL': the reference's 1e-6
+    cross terms (H += 1e-6 (Dx + Dy), then Dx, Dy += 1e-6 H), Omega kept,
+    and the result is not Hermitian."""
+    N = 64
+    S = jcl.lattice_hexagonal(N)
+    H, Dx, Dy, Om, cj = jcl.H_graphene(S, -2.7, 0.3, 0, True, 'CSR')
+    assert np.array_equal(H.diagonal(), np.where(np.arange(N) % 2 == 0, 0.3, -0.3))
+    assert np.all(cj.diagonal() == 2)
+    np.random.seed(5)
+    Hw = jcl.H_graphene(S, -2.7, 0, 0.8, True, 'CSR')[0]
+    np.random.seed(5)
+    assert np.array_equal(Hw.diagonal(), 0.8 * (2 * np.random.rand(N) - 1))
+    Hs = jcl.H_graphene(S, -2.7, 0, 0.8, True, 'CSR', seed=9)[0]
+    assert np.array_equal(Hs.diagonal(), 0.8 * (2 * np.random.default_rng(9).random(N) - 1))
+    A = jcl.H_graphene(S, -2.7 + 0j, 0.3, 0, True, 'ELL')
+    assert abs(A.Omega - Om) == 0
+    He = H + 1e-6 * (Dx + Dy)
+    want = jcl.ell_matrix(He, Dx + 1e-6 * He, Dy + 1e-6 * He)
+    assert np.array_equal(A.data, want.data) and np.array_equal(A.indices, want.indices)
+    assert np.array_equal(A.dx_vec, want.dx_vec) and np.array_equal(A.dy_vec, want.dy_vec)
+    Hc = jcl.ell_to_csr(A)
+    assert abs(Hc - Hc.conj().T).max() > 1e-7
+    assert A.conjugates.size == A.data.size
+
+
+def test_extra_funs_match_oracle():
+    """The Extra_funs entries of the namespace: data_indices_c(_0) fill the
+    caller's arrays like the oracle's CSR -> ELL, diagonal_modifier writes
+    data_new in place like the oracle's, table_G_delta equals the oracle's
+    table (pinned by eval_chebyu) including the 1e-16 guards at E = +-1."""
+    H, Dx, Dy = jcl.honeycomb_lattice(5, W=1.0, seed=2)
+    N = H.shape[0]
+    d, i, v, K = O.csr_to_ell(H)
+    out_i, out_d, out_v = np.zeros(N * K, np.int32), np.zeros(N * K, complex), np.zeros(N * K, np.int32)
+    rd, ri, rv = jcl.data_indices_c_0(K, N, H.indptr, H.data, H.indices, out_i, out_d, out_v, 8)
+    assert np.array_equal(rd, d) and np.array_equal(ri, i) and np.array_equal(rv, v)
+    assert np.array_equal(out_d, d)  # filled in place
+    arrs = [np.zeros(N * K, t) for t in (np.int32, complex, complex, complex, np.int32, np.int32)]
+    cj = (np.repeat(np.arange(N), np.diff(H.indptr)) > H.indices).astype(np.int32)
+    res = jcl.data_indices_c(K, N, H.indptr, H.data, H.indices, Dx.data.astype(complex),
+                             Dy.data.astype(complex), cj, *arrs, 8)
+    A = jcl.ell_matrix(H, Dx, Dy)
+    assert np.array_equal(res[0], A.data) and np.array_equal(res[1], A.indices)
+    assert np.array_equal(res[2], A.dx_vec) and np.array_equal(res[3], A.dy_vec)
+    assert np.array_equal(res[5], A.valid_row_values)
+    rows = np.arange(N * K) // K
+    assert np.array_equal(res[4][res[5] == 1], (rows > A.indices)[res[5] == 1])
+    diag = np.random.default_rng(4).normal(size=N) + 0j
+    new = np.zeros_like(A.data)
+    jcl.diagonal_modifier(A.data, A.indices, A.valid_row_values, new, diag, K, A.data.size, 8)
+    assert np.array_equal(new, O.diagonal_modifier(A.data, A.indices, A.valid_row_values, diag, K))
+    for M, alpha, B in ((9, 0.97, 2.5), (12, 1.0, 3.1)):
+        bm, bn = jcl.table_G_delta(M, alpha, B, 8)
+        E = np.linspace(-alpha, alpha, 2 * M)
+        om, on = O.table_G_delta(M, E, B)
+        assert bm.shape == (M, M, 2 * M)
+        assert np.allclose(bm, om, rtol=1e-12, atol=1e-300)
+        assert np.allclose(bn, on, rtol=1e-12, atol=1e-300)
