@@ -192,6 +192,25 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
                  int R, int M, double *mu, void *workspace, size_t workspace_bytes,
                  void *stream);
 
+/* jkpm_moments without a workspace argument; it allocates nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its scratch (the two rotating vector batches and the
+ * moment-partial ring) is a library-owned device array of
+ * JKPM_NOALLOC_SCRATCH_BYTES (128 MiB) that every GPU the library runs on
+ * holds until the process ends; a matrix and R whose layout needs more
+ * (jkpm_moments_workspace) get JKPM_ERR_WORKSPACE.  Nothing in the scratch
+ * is read before it is written in the same call, so it keeps the last
+ * call's contents harmlessly; calls that overlap on one GPU (two streams or
+ * threads) share it and race.  The step-time profiler
+ * (jkpm_profile_enable) does not record these calls.
+ * jkpm_moments_noalloc_setup looks up the scratch's address on the current
+ * device ahead of the first call (the first call does it otherwise). */
+#define JKPM_NOALLOC_SCRATCH_BYTES (128u << 20)
+int jkpm_moments_noalloc_setup(void);
+int jkpm_moments_noalloc(const jkpm_ell *h, const void *v_r, const void *v_l0, const void *v_l1,
+                         int R, int M, double *mu, void *stream);
+
 /* ---------------------------------------------------------------------
  * Launch shape of the Chebyshev step kernel and its autotuner (no reference
  * counterpart: the reference launches <<<ceil(N/1024), 1024>>> always,

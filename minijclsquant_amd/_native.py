@@ -29,6 +29,8 @@ JKPM_C64 = 1
 
 JKPM_HERMITIAN_RTOL = HERMITIAN_RTOL = 1e-10
 
+JKPM_NOALLOC_SCRATCH_BYTES = 128 << 20
+
 JKPM_VOP_AXPBY = 0
 JKPM_VOP_MUL = 1
 JKPM_VOP_MUL2 = 2
@@ -58,6 +60,8 @@ SIGNATURES = {
     "jkpm_ell_get_values": (_int, [_vp, _vp, _int, _vp]),
     "jkpm_ell_hermiticity": (_int, [_vp, _dp, _dp, _vp]),
     "jkpm_zvec": (_int, [_int, _c.c_longlong, _dp, _dp, _vp, _vp, _vp, _vp, _vp]),
+    "jkpm_moments_noalloc_setup": (_int, []),
+    "jkpm_moments_noalloc": (_int, [_vp, _vp, _vp, _vp, _int, _int, _vp, _vp]),
     "jkpm_zdot_workspace": (_int, [_c.POINTER(_size)]),
     "jkpm_zdot": (_int, [_c.c_longlong, _dp, _vp, _vp, _int, _vp, _vp, _size, _vp]),
     "jkpm_ell_create_csr": (_int, [_vp, _vp, _vp, _int, _int, _dbl, _int, _int, _vp,

@@ -166,7 +166,7 @@ int device_info(DeviceInfo *out) {
 static int run_moments(const Ell &H, const void *vr, const void *vl0, const void *vl1, int R,
                        int M, double2 *mu, void *ws, size_t ws_bytes, hipStream_t s,
                        const Groups *grp = nullptr, bool doubling = false,
-                       void *stack = nullptr) {
+                       void *stack = nullptr, bool no_profile = false) {
     DeviceInfo dev;
     int rc = device_info(&dev);
     if (rc) return rc;
@@ -197,7 +197,8 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
     Prof &P = prof();
     std::unique_lock<std::mutex> plk(P.mu);
     const int every = P.every;
-    const bool profiling = every > 0 && cap == hipStreamCaptureStatusNone;
+    // (no_profile: the caller may not allocate, and recording takes events)
+    const bool profiling = every > 0 && cap == hipStreamCaptureStatusNone && !no_profile;
     if (!profiling) plk.unlock();
 
     auto step = [&](int n) -> int {
@@ -428,6 +429,42 @@ static int upload_values(Ell *H, const double *data, double scale, const int *id
     return rc;
 }
 
+}  // namespace jkpm
+
+// Scratch of jkpm_moments_noalloc: the two rotating vector batches and the
+// moment-partial ring of run_moments, in device memory the library owns
+// instead of a caller workspace.  Each GPU holds its own copy from module
+// load until the process ends.  In host code `scratch` only names the
+// variable; its device address on the current GPU comes from
+// hipGetSymbolAddress (noalloc_scratch), taken once per device.
+static __device__ __align__(256) unsigned char scratch[JKPM_NOALLOC_SCRATCH_BYTES];
+
+namespace jkpm {
+namespace {
+
+// Device address of `scratch` on the current device, looked up on the first
+// call per device and kept.
+int noalloc_scratch(void **out) {
+    static std::mutex m;
+    static std::vector<void *> ptrs;
+    int dev = 0;
+    JKPM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(m);
+    if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+    if (!ptrs[dev]) {
+        void *p = nullptr;
+        JKPM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("hipGetSymbolAddress returned no address for the no-alloc scratch");
+            return JKPM_ERR_HIP;
+        }
+        ptrs[dev] = p;
+    }
+    *out = ptrs[dev];
+    return JKPM_OK;
+}
+
+}  // namespace
 }  // namespace jkpm
 
 using namespace jkpm;
@@ -848,6 +885,40 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
         return arg_error("device pointers must be 16-byte aligned");
     return run_moments(h->e, v_r, v_l0, v_l1, R, M, (double2 *)mu, workspace, workspace_bytes,
                        (hipStream_t)stream);
+}
+
+int jkpm_moments_noalloc_setup(void) {
+    clear_error();
+    void *p = nullptr;
+    return noalloc_scratch(&p);
+}
+
+int jkpm_moments_noalloc(const jkpm_ell *h, const void *v_r, const void *v_l0, const void *v_l1,
+                         int R, int M, double *mu, void *stream) {
+    clear_error();
+    if (!h) return arg_error("handle is NULL");
+    if (!v_r || !v_l0 || !mu) return arg_error("NULL vector/mu");
+    if (R <= 0) return arg_error("R must be positive");
+    if (M <= 0) return arg_error("M must be positive");
+    if (!aligned16(v_r) || !aligned16(v_l0) || (v_l1 && !aligned16(v_l1)) || !aligned16(mu))
+        return arg_error("device pointers must be 16-byte aligned");
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const MomentsLayout L = moments_layout(h->e, R, v_l1 ? 2 : 1, dev);
+    if (L.total > JKPM_NOALLOC_SCRATCH_BYTES) {
+        set_error("jkpm_moments_noalloc: this matrix and R need " + std::to_string(L.total) +
+                  " bytes of scratch, more than the " +
+                  std::to_string((size_t)JKPM_NOALLOC_SCRATCH_BYTES) +
+                  " the library holds per GPU; use jkpm_moments with a workspace");
+        return JKPM_ERR_WORKSPACE;
+    }
+    void *scr = nullptr;
+    rc = noalloc_scratch(&scr);
+    if (rc) return rc;
+    return run_moments(h->e, v_r, v_l0, v_l1, R, M, (double2 *)mu, scr,
+                       (size_t)JKPM_NOALLOC_SCRATCH_BYTES, (hipStream_t)stream, nullptr, false,
+                       nullptr, true);
 }
 
 int jkpm_moments_keep(const jkpm_ell *h, const void *v_r, int R, int M, double *mu,
