@@ -25,7 +25,16 @@ N*M*R units per second.  The JSON line also carries
                 (Hermitian H), with its own roofline and DOS error;
   plain_c128    the reference's structure on the GPU (complex128, one vector
                 per recursion, one moment per step) and the headline's and
-                doubling form's ratios over it.
+                doubling form's ratios over it;
+  mu_err_vs_c128  max_n,r |mu_c64 - mu_c128| / mu_0;
+  config1 / batch / grad / kubo  the other four workloads (plain complex128
+                DOS with the oracle timed on the full workload; 64 disorder
+                realisations in one batch; the adjoint gradient; the Kubo
+                double moments), each with its own metric, the event-timed
+                roofline of its dominant kernel and its accuracy field
+                (parity with the oracle, error against complex128 / FP64, or
+                the finite-difference check), over a few timed steps each
+                (--sub-steps; --no-subs leaves them out).
 """
 from __future__ import annotations
 
@@ -94,12 +103,29 @@ def copy_ceiling(torch, nat, nbytes=2 << 30, reps=5):
     return 2 * nbytes / (ms / 1e3) / 1e9
 
 
+def host_cpu():
+    """(threads to use, logical CPUs visible, CPU model).  The threads are
+    OMP_NUM_THREADS when set (the GPU box sets it to its per-GPU CPU share),
+    else min(nproc, 16)."""
+    nproc = os.cpu_count() or 1
+    thr = int(os.environ.get("OMP_NUM_THREADS", min(nproc, 16)))
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return thr, nproc, model
+
+
 def cpu_baseline(A, r, target_s=15.0):
     """complex128 oracle (reference algorithm) on host cores: one vector, a
     bounded number of moments sized to ~target_s seconds."""
     from oracle import oracle as O
 
-    thr = int(os.environ.get("OMP_NUM_THREADS", min(os.cpu_count() or 1, 16)))
+    thr, nproc, model = host_cpu()
     Ht = A.data * (1 / A.bounds[1])
     N = int(A.shape[0])
     m_probe = 16
@@ -116,6 +142,10 @@ def cpu_baseline(A, r, target_s=15.0):
         "cores": thr,
         "kind": "port",
         "sample": f"same H, R=1 vector, M={m} moments, complex128, {dt:.1f} s",
+        "nproc": nproc,
+        "cpu_model": model,
+        "threads_note": "OpenMP threads = OMP_NUM_THREADS (the host's CPU share per GPU); "
+                        "nproc is every logical CPU the process sees",
     }
 
 
@@ -217,6 +247,30 @@ def event_ms(torch, fn, reps):
     return t0.elapsed_time(t1) / reps
 
 
+def profiled(nat, fn, every=4, kind=0):
+    """(mean ms per launch, launches) of one kind of kernel, from the
+    library's HIP-event runs around its launches during one untimed call of
+    fn() (kind 0: Chebyshev step launches; see jkpm_profile_read_kind)."""
+    nat.profile_read(reset=True, kind=kind)
+    nat.profile_enable(every)
+    try:
+        fn()
+    finally:
+        nat.profile_enable(0)
+    ms, n = nat.profile_read(reset=True, kind=kind)
+    return ms / max(n, 1), n
+
+
+def hbm_roofline(bytes_launch, launch_ms, n, kernel, **extra):
+    ach = bytes_launch / (launch_ms / 1e3) / 1e9
+    out = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+           "algorithmic_bytes_per_launch": int(bytes_launch),
+           "mean_launch_us": round(launch_ms * 1e3, 2), "launches_timed": n}
+    out.update(extra)
+    return out
+
+
 def base_line(metric, value, unit, args, ms, dtype, config, higher=True):
     return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": higher,
@@ -290,7 +344,6 @@ def bench_batch(args, torch, jcl, nat, _kpm):
                               for s in range(R)]))
     ingest = ingest_leg(blocks, torch, nat)
     bk = BatchKPM(blocks, vecs, M, "c64", tune=not args.no_tune)
-    del blocks, vecs
     N = int(bk.offsets[-1])
     log(f"launch shapes (variant, work-groups per CU): plain {bk.H.get_launch()}, "
         f"doubling {bk.H.get_launch(doubling=True)}")
@@ -315,6 +368,28 @@ def bench_batch(args, torch, jcl, nat, _kpm):
     bytes_d = N * K * 12 + R * N * 3 * 8
     ach = bytes_launch * (M - 1) / (ms / 1e3) / 1e9
     ach_d = bytes_d * max(1, M // 2) / (ms_d / 1e3) / 1e9
+    # the step kernel alone: event-timed runs of launches of one eager pass
+    # (a graph replay records no events)
+    launch_ms, n_prof = profiled(nat, bk.run, every=16)
+    torch.cuda.synchronize()
+    # every realisation against its own complex128 recursion (the per-matrix
+    # path the tests pin to the oracle): moments and the Jackson DOS on 1024
+    # rescaled energies
+    mu64 = bk.replay().cpu().numpy()  # [G, M, R]
+    es = torch.as_tensor(np.linspace(-0.99, 0.99, 1024), device="cuda")
+    mu_err = dos_err = 0.0
+    for g, (d, ix, n_g, k_g, sc) in enumerate(blocks):
+        hg = nat.DeviceELL(d, ix, n_g, k_g, sc, "c128")
+        vg = _kpm.as_batch(vecs[g], n_g, "c128")
+        m128 = hg.moments(vg, M)[0]
+        mu_err = max(mu_err, float(np.max(np.abs(mu64[g] - m128.cpu().numpy())) / n_g))
+        d128 = nat.dos(m128, es, nat.JKPM_KERNEL_JACKSON).cpu().numpy()
+        d64 = nat.dos(torch.as_tensor(mu64[g], device="cuda"), es,
+                      nat.JKPM_KERNEL_JACKSON).cpu().numpy()
+        dos_err = max(dos_err, float(np.max(np.abs(d64 - d128)) / np.max(np.abs(d128))))
+        del hg, vg, m128
+    del blocks, vecs
+    log(f"batch vs complex128: max |dmu|/mu0 {mu_err:.2e}, DOS err {dos_err:.2e}")
     out = base_line("batched realisations: Chebyshev SpMV-steps/s (sum N * M * R / s)",
                     units / (ms / 1e3), "N*M*R/s", args, ms, "c64",
                     {"workload": f"{G} square-lattice realisations, L ~ U{{128..1024}} "
@@ -322,16 +397,18 @@ def bench_batch(args, torch, jcl, nat, _kpm):
                                  "M = 1024, complex64 recursion + FP64 moments, one "
                                  "block-diagonal batch replayed as one HIP graph",
                      "G": G, "sum_N": N, "K": K, "M": M, "R": R})
-    out["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                       "note": "whole replay: (M-1) step launches' algorithmic bytes over the "
-                               "replay time (reductions and gaps included)",
-                       "algorithmic_bytes_per_launch": bytes_launch}
+    out["roofline"] = hbm_roofline(bytes_launch, launch_ms, n_prof, "cheb_step_kernel",
+                                   whole_replay_GBs=round(ach, 1),
+                                   whole_replay_frac=round(ach / HBM_PEAK_GBS, 4))
+    out["mu_err_vs_c128"] = mu_err
+    out["dos_err_vs_c128"] = dos_err
+    out["accuracy_note"] = ("max over the 64 realisations and all n of |mu_c64 - mu_c128| / mu_0, "
+                            "and of the Jackson DOS error on 1024 rescaled energies")
     out["doubling"] = {"value": units / (ms_d / 1e3), "ms_per_step": ms_d,
                        "frac": round(ach_d / HBM_PEAK_GBS, 4),
                        "max_rel_moment_diff_vs_plain": diff}
     out["ingest"] = ingest
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_config1(args, torch, jcl, nat, _kpm):
@@ -374,6 +451,7 @@ def bench_config1(args, torch, jcl, nat, _kpm):
         step()
     torch.cuda.synchronize()
     ms = event_ms(torch, step, args.steps)
+    launch_ms, n_prof = profiled(nat, step, every=4)
     mu_gpu = mu[0].cpu().numpy()
     # Side figure: the same moments from M/2 steps (Hermitian H, v_l = v_r).
     if not args.no_tune:
@@ -396,7 +474,7 @@ def bench_config1(args, torch, jcl, nat, _kpm):
     # the package's DOS table (GPU) for the parity check against the oracle's
     dos_gpu = jcl.kpm_dos(A, M, r, energies=E)[:, 1]
     # the oracle on the same inputs, every vector, timed on the host cores
-    thr = int(os.environ.get("OMP_NUM_THREADS", min(os.cpu_count() or 1, 16)))
+    thr, nproc, model = host_cpu()
     Ht = A.data * (1 / B)
     t0 = time.perf_counter()
     mu_cpu = np.stack([O.rec_A_tab(M, Ht, A.indices, K, r[k], n_threads=thr) for k in range(R)], 1)
@@ -413,13 +491,13 @@ def bench_config1(args, torch, jcl, nat, _kpm):
                      "N": N, "K": K, "M": M, "R": R})
     bytes_launch = N * K * 20 + R * N * 4 * 16
     ach = bytes_launch * (M - 1) / (ms / 1e3) / 1e9
-    out["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                       "note": "whole step over (M-1) step launches' algorithmic bytes; the "
-                               "23 MB working set is cache-resident, so this is not an HBM "
-                               "figure"}
+    out["roofline"] = hbm_roofline(
+        bytes_launch, launch_ms, n_prof, "cheb_step_kernel",
+        whole_step_GBs=round(ach, 1),
+        note="the 23 MB working set is cache-resident (256 MB Infinity Cache) and each launch "
+             "is latency-bound, so this is not an HBM-bound figure")
     out["cpu_baseline"] = {"value": units / cpu_s, "unit": "N*M*R/s", "cores": thr,
-                           "kind": "port",
+                           "kind": "port", "nproc": nproc, "cpu_model": model,
                            "sample": f"the full workload (R = {R} vectors, M = {M}, DOS), "
                                      f"complex128 oracle, {cpu_s:.2f} s"}
     out["gpu_over_cpu"] = (units / (ms / 1e3)) / (units / cpu_s)
@@ -433,7 +511,7 @@ def bench_config1(args, torch, jcl, nat, _kpm):
                        "note": "the same M moments from M/2 Chebyshev steps (mu_2k = 2<T_k|T_k> - "
                                "mu_0, mu_2k+1 = 2<T_k|T_k+1> - mu_1; Hermitian H), checked "
                                "against the oracle's moments"}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_grad(args, torch, jcl, nat, _kpm):
@@ -478,7 +556,30 @@ def bench_grad(args, torch, jcl, nat, _kpm):
     torch.cuda.synchronize()
     with torch.no_grad():
         ms_f = event_ms(torch, forward, args.steps)
+        launch_ms, n_prof = profiled(nat, forward, every=16)
     ms_fb = event_ms(torch, both, args.steps)
+    # adjoint gradient against central finite differences (h = 1e-6,
+    # complex128 forward passes) on n_fd seeded coordinates
+    both()
+    g_all = torch.cat([eps.grad, hop.grad]).cpu().numpy()
+    coords = np.random.default_rng(2026).choice(g_all.size, args.fd_coords, replace=False)
+    fd = []
+    with torch.no_grad():
+        base = torch.cat([eps, hop]).detach().clone()
+        for c in coords:
+            vals = []
+            for sgn in (1.0, -1.0):
+                p = base.clone()
+                p[c] += sgn * 1e-6
+                eps.data.copy_(p[:N])
+                hop.data.copy_(p[N:])
+                vals.append(float(forward()))
+            fd.append((vals[0] - vals[1]) / 2e-6)
+        eps.data.copy_(base[:N])
+        hop.data.copy_(base[N:])
+    fd = np.array(fd)
+    fd_err = float(np.linalg.norm(g_all[coords] - fd) / np.linalg.norm(fd))
+    log(f"gradient vs finite differences ({len(coords)} coordinates): {fd_err:.2e}")
     units = N * M * R
     out = base_line("differentiation: forward + adjoint backward, N * M * R / s",
                     units / (ms_fb / 1e3), "N*M*R/s", args, ms_fb, "c128",
@@ -491,9 +592,12 @@ def bench_grad(args, torch, jcl, nat, _kpm):
     out["forward_ms"] = ms_f
     out["backward_ms"] = ms_fb - ms_f
     out["backward_over_forward"] = (ms_fb - ms_f) / ms_f
-    out["forward_roofline"] = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "achieved": round(bytes_launch * (M - 1) / (ms_f / 1e3) / 1e9, 1),
-                               "note": "forward step bytes x (M-1) over the forward time"}
+    out["roofline"] = hbm_roofline(
+        bytes_launch, launch_ms, n_prof, "cheb_step_kernel (forward recursion)",
+        forward_whole_GBs=round(bytes_launch * (M - 1) / (ms_f / 1e3) / 1e9, 1),
+        note="the 160 MB step working set stays partly in the 256 MB Infinity Cache")
+    out["grad_vs_fd"] = {"rel_err": fd_err, "coordinates": int(len(coords)), "h": 1e-6,
+                         "bar": 1e-4}
     # the same with sqrt(M) checkpoints and recomputation (small memory)
     seg[0] = 0
     both()
@@ -503,13 +607,47 @@ def bench_grad(args, torch, jcl, nat, _kpm):
                             "runs the adjoint steps and folds only")
     out["sqrtM_checkpoints"] = {"ms_per_step": ms_fb_ck, "backward_ms": ms_fb_ck - ms_f,
                                 "backward_over_forward": (ms_fb_ck - ms_f) / ms_f}
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def kubo_mu_nm_fp64(torch, nat, _kpm, A, B, v_data, r, M, chunk=1 << 17):
+    """mu_mn = (1/R) sum_r <r| v T_m(H~) v T_n(H~) |r> in complex128: both
+    Chebyshev stacks from the complex128 step path (jkpm_moments_keep, the
+    recursion the tests pin to the oracle), Rk_n = v T_n r by jkpm_aAxby, and
+    the contraction sum_k conj(L[m, k]) Rk[n, k] as a complex128 GEMM in
+    chunks of k.  Returns a complex128 [M, M] device tensor."""
+    N, K = int(A.shape[0]), int(A.len_row)
+    R = r.shape[0]
+    H = nat.DeviceELL(A.data, A.indices, N, K, 1.0 / B, "c128")
+    V = nat.DeviceELL.like(H, v_data, 1.0, "c128")
+    rr = _kpm.as_batch(r, N, "c128")
+    _, T = H.moments_keep(rr, M)  # [M, N, R]: T_n(H~) r
+    Rk = torch.zeros((M, N, R), dtype=torch.complex128, device="cuda")
+    for n in range(M):
+        V.aAxby(1.0, 0.0, T[n], Rk[n])
+    del T
+    vr = Rk[0].clone()
+    _, L = H.moments_keep(vr, M)  # L_m = T_m(H~) (v r)
+    L2 = L[:M].reshape(M, N * R)
+    R2 = Rk.reshape(M, N * R)
+    mu = torch.zeros((M, M), dtype=torch.complex128, device="cuda")
+    for k0 in range(0, N * R, chunk):
+        k1 = min(N * R, k0 + chunk)
+        a = L2[:, k0:k1].conj().contiguous()
+        b = R2[:, k0:k1].transpose(0, 1).contiguous()
+        mu += torch.mm(a, b)
+        del a, b
+    del L, Rk, L2, R2
+    torch.cuda.synchronize()
+    return mu / R
 
 
 def bench_kubo(args, torch, jcl, nat, _kpm):
     """Honeycomb 512 x 512 cells (N = 524,288), t = -2.7, W = 1 eV, v_x on
     H's slots, M = 512, R = 4: mu_nm (complex64 stacks, FP32 MFMA GEMM,
-    FP64 combine) and sigma on 512 energies."""
+    FP64 combine) and sigma on 512 energies; the GEMM and the stack steps
+    timed alone by the library's event runs; mu_nm and sigma against the
+    complex128 stacks contracted in FP64."""
     from minijclsquant_amd.kubo import velocity_handle
 
     Lc, M, R, n_e = 512, 512, 4, 512
@@ -533,19 +671,46 @@ def bench_kubo(args, torch, jcl, nat, _kpm):
         step()
     torch.cuda.synchronize()
     ms = event_ms(torch, step, args.steps)
+    gemm_ms, n_gemm = profiled(nat, step, every=16, kind=nat.JKPM_PROF_KUBO_GEMM)
+    kstep_ms, n_kstep = profiled(nat, step, every=16, kind=nat.JKPM_PROF_KUBO_STEP)
+    sigma = step().cpu().numpy()
+    mu64 = mu.cpu().numpy()
+    ws = None
+    torch.cuda.empty_cache()
+    log("Kubo: complex128 stacks contracted in FP64 for the error")
+    ref = kubo_mu_nm_fp64(torch, nat, _kpm, A, B, jcl.modifier_velocity(A, "x"), r, M)
+    sig_ref = nat.kubo_sigma(ref, es, 0, B).cpu().numpy()
+    ref = ref.cpu().numpy()
+    torch.cuda.empty_cache()
+    mu_err = float(np.max(np.abs(mu64 - ref)) / np.max(np.abs(ref)))
+    sig_err = float(np.max(np.abs(sigma.real - sig_ref.real)) / np.max(np.abs(sig_ref.real)))
+    log(f"Kubo error vs FP64: mu_nm {mu_err:.2e}, sigma {sig_err:.2e}")
     flops = 8.0 * N * M * M * R
     tf = flops / (ms / 1e3) / 1e12
+    tf_gemm = flops / (gemm_ms / 1e3) / 1e12
     out = base_line("Kubo double moments: mu_nm GEMM TFLOP/s (8 N M^2 R per step)",
                     tf, "TFLOP/s", args, ms, "c64 stacks, fp32 MFMA",
                     {"workload": "honeycomb 512x512 cells periodic, t = -2.7, W = 1 eV, "
                                  "v_x on H's slots, M = 512, R = 4, complex64 stacks, FP32 "
                                  "MFMA accumulate, FP64 combine, sigma on 512 energies",
                      "N": N, "M": M, "R": R})
-    out["roofline"] = {"bound": "mfma", "achieved": round(tf, 2), "peak": 157.3,
-                       "unit": "TFLOP/s", "frac": round(tf / 157.3, 4), "traffic": None,
-                       "note": "whole step (stack recursion + GEMM + sigma) against the GEMM's "
-                               "flops; the GEMM kernel alone is in the committed profile"}
-    print(json.dumps(out), flush=True)
+    kstep_bytes = N * 4 * (8 + 8 + 4) + R * N * 8 * 7
+    out["roofline"] = {"bound": "mfma", "achieved": round(tf_gemm, 2), "peak": 157.3,
+                       "unit": "TFLOP/s", "frac": round(tf_gemm / 157.3, 4), "traffic": None,
+                       "kernel": "kubo_gemm3_kernel", "mean_launch_us": round(gemm_ms * 1e3, 1),
+                       "launches_timed": n_gemm,
+                       "note": "complex flops 8 N M^2 R over the GEMM launch (three real "
+                               "products per complex one execute 3/4 of them)",
+                       "whole_step_TFLOPs": round(tf, 2),
+                       "whole_step_frac": round(tf / 157.3, 4)}
+    out["stack_step"] = hbm_roofline(kstep_bytes, kstep_ms, n_kstep, "kubo_step_kernel",
+                                     steps_per_call=M - 1)
+    out["mu_nm_err_vs_fp64"] = mu_err
+    out["sigma_err_vs_fp64"] = sig_err
+    out["accuracy_note"] = ("max |mu_c64 - mu_fp64| / max |mu_fp64| and the same for the "
+                            "Kubo-Greenwood sigma on 512 rescaled energies; the FP64 reference "
+                            "contracts complex128 stacks of the same H, v_x and vectors")
+    return out
 
 
 def main():
@@ -573,6 +738,13 @@ def main():
                     help="skip the plain complex128 one-vector GPU recursion")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shape instead of the tuned one")
+    ap.add_argument("--no-subs", action="store_true",
+                    help="default workload only: skip the config1 / batch / grad / kubo "
+                         "sub-objects")
+    ap.add_argument("--sub-steps", type=int, default=3,
+                    help="timed steps of each sub-workload (at most --steps)")
+    ap.add_argument("--fd-coords", type=int, default=16,
+                    help="grad workload: coordinates of the finite-difference check")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware counters: 3 copies of 1 GiB with the "
                          "library's copy kernel, then 16 moments of the workload (15 "
@@ -592,10 +764,10 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
+    subs = {"config1": bench_config1, "batch": bench_batch, "grad": bench_grad,
+            "kubo": bench_kubo}
     if args.workload != "dos":
-        {"config1": bench_config1, "batch": bench_batch, "grad": bench_grad,
-         "kubo": bench_kubo}[args.workload](
-            args, torch, jcl, nat, _kpm)
+        print(json.dumps(subs[args.workload](args, torch, jcl, nat, _kpm)), flush=True)
         return
     L, M, R = args.cells, args.moments, args.vectors
     log(f"building workload: honeycomb {L}x{L} cells, M={M}, R={R}, {args.dtype}")
@@ -691,10 +863,15 @@ def main():
 
     d128 = None
     dos_err = None
+    mu_err = None
     if not args.no_dos_check and args.dtype == "c64":
-        log("complex128 reference run for the DOS error")
-        d128 = jcl.kpm_dos(A, M, r, energies=E, dtype="c128")[:, 1]
+        log("complex128 reference run for the moment and DOS errors")
+        mu128 = jcl.kpm_moments(A, M, r, dtype="c128")  # [R, M]
+        d128 = nat.dos(mu128.transpose(0, 1).contiguous(), es,
+                       nat.JKPM_KERNEL_JACKSON).cpu().numpy()
         dos_err = float(np.max(np.abs(dos.cpu().numpy() - d128)) / np.max(np.abs(d128)))
+        mu_err = float((mu[0].transpose(0, 1) - mu128).abs().max() / N)
+        del mu128
 
     dbl = None
     if not args.no_doubling:
@@ -760,12 +937,29 @@ def main():
             "N": N, "K": K, "M": M, "R": R, "bound": B,
         },
         "dos_err_vs_c128": dos_err,
+        "mu_err_vs_c128": mu_err,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "doubling": dbl,
         "plain_c128": plain,
         "two_step_tiles": tiles,
     }
+    if not args.no_subs:
+        # the other workloads, each with its own metric, dominant-kernel
+        # roofline and accuracy field; a few timed steps each
+        sub_args = argparse.Namespace(**vars(args))
+        sub_args.steps = max(1, min(args.steps, args.sub_steps))
+        sub_args.warmup = min(args.warmup, 1)
+        del h, vr, ws, ws_dbl, mu
+        for name, fn in subs.items():
+            torch.cuda.empty_cache()
+            log(f"sub-workload {name}")
+            t0 = time.perf_counter()
+            sub = fn(sub_args, torch, jcl, nat, _kpm)
+            for k in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):
+                sub.pop(k, None)
+            sub["wall_s"] = round(time.perf_counter() - t0, 1)
+            out[name] = sub
     print(json.dumps(out), flush=True)
 
 

@@ -482,6 +482,17 @@ int jkpm_kubo_sigma(const double *mu, int M, const double *e, int n_e, int kind,
 int jkpm_profile_enable(int every);
 int jkpm_profile_read(double *step_ms, long long *step_launches, int reset);
 
+/* The same totals per kind of timed kernel: JKPM_PROF_STEP (the Chebyshev
+ * step launches above; jkpm_profile_read reads this kind),
+ * JKPM_PROF_KUBO_STEP (runs of up to 8 fused Kubo stack steps, one run
+ * started every `every` steps) and JKPM_PROF_KUBO_GEMM (each mu_nm GEMM
+ * launch alone).  Nothing is recorded under stream capture. */
+#define JKPM_PROF_STEP 0
+#define JKPM_PROF_KUBO_STEP 1
+#define JKPM_PROF_KUBO_GEMM 2
+#define JKPM_PROF_KINDS 3
+int jkpm_profile_read_kind(int kind, double *ms, long long *launches, int reset);
+
 /* Plain device copy dst <- src of `bytes` bytes (a multiple of 16), used to
  * measure the achievable HBM rate and to calibrate traffic counters. */
 int jkpm_copy(void *dst, const void *src, size_t bytes, void *stream);

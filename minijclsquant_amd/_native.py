@@ -61,6 +61,7 @@ SIGNATURES = {
     "jkpm_ell_hermiticity": (_int, [_vp, _dp, _dp, _vp]),
     "jkpm_zvec": (_int, [_int, _c.c_longlong, _dp, _dp, _vp, _vp, _vp, _vp, _vp]),
     "jkpm_moments_noalloc_setup": (_int, []),
+    "jkpm_profile_read_kind": (_int, [_int, _dp, _c.POINTER(_c.c_longlong), _int]),
     "jkpm_moments_noalloc": (_int, [_vp, _vp, _vp, _vp, _int, _int, _vp, _vp]),
     "jkpm_zdot_workspace": (_int, [_c.POINTER(_size)]),
     "jkpm_zdot": (_int, [_c.c_longlong, _dp, _vp, _vp, _int, _vp, _vp, _size, _vp]),
@@ -861,10 +862,17 @@ def profile_enable(every: int) -> None:
     check(lib().jkpm_profile_enable(int(every)))
 
 
-def profile_read(reset: bool = True):
+JKPM_PROF_STEP = 0
+JKPM_PROF_KUBO_STEP = 1
+JKPM_PROF_KUBO_GEMM = 2
+
+
+def profile_read(reset: bool = True, kind: int = JKPM_PROF_STEP):
+    """(summed ms, launches) recorded for one kind of kernel since the last
+    reset of that kind."""
     ms = _c.c_double()
     n = _c.c_longlong()
-    check(lib().jkpm_profile_read(_c.byref(ms), _c.byref(n), 1 if reset else 0))
+    check(lib().jkpm_profile_read_kind(int(kind), _c.byref(ms), _c.byref(n), 1 if reset else 0))
     return float(ms.value), int(n.value)
 
 

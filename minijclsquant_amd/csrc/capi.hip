@@ -30,6 +30,7 @@ constexpr int kRing = 64;  // steps of partials held before one reduction launch
 struct ProfRun {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int steps = 0;
+    int kind = JKPM_PROF_STEP;  // which kernel the run times
 };
 constexpr int kProfRun = 8;  // steps per recorded run (at most)
 
@@ -39,8 +40,8 @@ struct Prof {
     long long counter = 0;
     std::vector<ProfRun> pending;
     std::vector<hipEvent_t> free_events;
-    double ms = 0.0;
-    long long launches = 0;
+    double ms[JKPM_PROF_KINDS] = {};
+    long long launches[JKPM_PROF_KINDS] = {};
 };
 Prof &prof() {
     static Prof p;
@@ -64,8 +65,8 @@ int drain_profile(Prof &p) {
         JKPM_HIP(hipEventSynchronize(pr.e1));
         float ms = 0.f;
         JKPM_HIP(hipEventElapsedTime(&ms, pr.e0, pr.e1));
-        p.ms += ms;
-        p.launches += pr.steps;
+        p.ms[pr.kind] += ms;
+        p.launches[pr.kind] += pr.steps;
         p.free_events.push_back(pr.e0);
         p.free_events.push_back(pr.e1);
     }
@@ -383,7 +384,7 @@ int run_tiles(const jkpm_tiles &T, const void *v_r, int R, int M, bool doubling,
         }
         if (rec) {
             JKPM_HIP(hipEventRecord(e1, s));
-            P.pending.push_back({e0, e1, 1});
+            P.pending.push_back({e0, e1, 1, JKPM_PROF_STEP});
             if (P.pending.size() >= 8192) {
                 rc = drain_profile(P);
                 if (rc) return rc;
@@ -1719,10 +1720,43 @@ int jkpm_kubo_mu_nm(const jkpm_ell *H, const jkpm_ell *V, const void *r, int R, 
     // Rk_0, out of place into its stack.  Step n >= 1 advances both on one
     // pass over the slots: the right one to T_{n+1}, Rk_n and the left one to
     // L_n.
+    // Profiling (jkpm_profile_enable): event pairs around runs of up to
+    // kProfRun stack steps (kind JKPM_PROF_KUBO_STEP, one run every `every`
+    // steps, never the first) and around the GEMM (JKPM_PROF_KUBO_GEMM).
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    JKPM_HIP(hipStreamIsCapturing(s, &cap));
+    Prof &P = prof();
+    std::unique_lock<std::mutex> plk(P.mu);
+    const int every = P.every;
+    const bool profiling = every > 0 && cap == hipStreamCaptureStatusNone;
+    if (!profiling) plk.unlock();
+    ProfRun run;
+    run.kind = JKPM_PROF_KUBO_STEP;
+    bool run_open = false;
+    auto open_run = [&](ProfRun &r) -> int {
+        r.e0 = take_event(P);
+        r.e1 = take_event(P);
+        if (!r.e0 || !r.e1) return hip_fail(hipErrorOutOfMemory, "hipEventCreate", __FILE__, __LINE__);
+        JKPM_HIP(hipEventRecord(r.e0, s));
+        r.steps = 0;
+        return JKPM_OK;
+    };
+    auto close_run = [&](ProfRun &r) -> int {
+        JKPM_HIP(hipEventRecord(r.e1, s));
+        P.pending.push_back(r);
+        if (P.pending.size() >= 8192) return drain_profile(P);
+        return JKPM_OK;
+    };
     rc = launch_kubo_step(H->e, &V->e, r, nullptr, Tb[1], Rs, R, 1.f, 0.f, K.nchunks, s);
     if (rc) return rc;
     JKPM_HIP(hipMemcpyAsync(Ls, Rs, vec, hipMemcpyDeviceToDevice, s));
     for (int n = 1; n < M; ++n) {
+        if (profiling && !run_open && P.counter % every == 0) {
+            rc = open_run(run);
+            if (rc) return rc;
+            run_open = true;
+        }
+        if (profiling) ++P.counter;
         const int m = n - 1;
         KuboLeft left;
         left.x = Ls + (size_t)m * vec;
@@ -1734,10 +1768,26 @@ int jkpm_kubo_mu_nm(const jkpm_ell *H, const jkpm_ell *V, const void *r, int R, 
                               Tb[(n + 1) & 1], Rs + (size_t)n * vec, R, 2.f, -1.f, K.nchunks, s,
                               &left);
         if (rc) return rc;
+        if (run_open && (++run.steps == kProfRun || n == M - 1)) {
+            rc = close_run(run);
+            if (rc) return rc;
+            run_open = false;
+        }
+    }
+    ProfRun gemm;
+    gemm.kind = JKPM_PROF_KUBO_GEMM;
+    if (profiling) {
+        rc = open_run(gemm);
+        if (rc) return rc;
     }
     rc = launch_kubo_gemm((const float2 *)Ls, (const float2 *)Rs, (long)H->e.n * R, M, K.nslices,
                           (float2 *)(w + K.part), s);
     if (rc) return rc;
+    if (profiling) {
+        gemm.steps = 1;
+        rc = close_run(gemm);
+        if (rc) return rc;
+    }
     return launch_kubo_reduce((const float2 *)(w + K.part), K.nslices, M, 1.0 / R, (double2 *)mu, s);
 }
 
@@ -1763,19 +1813,24 @@ int jkpm_profile_enable(int every) {
     return JKPM_OK;
 }
 
-int jkpm_profile_read(double *step_ms, long long *step_launches, int reset) {
+int jkpm_profile_read_kind(int kind, double *ms, long long *launches, int reset) {
     clear_error();
+    if (kind < 0 || kind >= JKPM_PROF_KINDS) return arg_error("unknown profile kind");
     Prof &P = prof();
     std::lock_guard<std::mutex> lk(P.mu);
     int rc = drain_profile(P);
     if (rc) return rc;
-    if (step_ms) *step_ms = P.ms;
-    if (step_launches) *step_launches = P.launches;
+    if (ms) *ms = P.ms[kind];
+    if (launches) *launches = P.launches[kind];
     if (reset) {
-        P.ms = 0.0;
-        P.launches = 0;
+        P.ms[kind] = 0.0;
+        P.launches[kind] = 0;
     }
     return JKPM_OK;
+}
+
+int jkpm_profile_read(double *step_ms, long long *step_launches, int reset) {
+    return jkpm_profile_read_kind(JKPM_PROF_STEP, step_ms, step_launches, reset);
 }
 
 int jkpm_copy(void *dst, const void *src, size_t bytes, void *stream) {

@@ -208,3 +208,63 @@ def test_mu_nm_independent_of_launch_widths(torch_gpu, monkeypatch):
         monkeypatch.setenv("JKPM_KUBO_SLICES", slices)
         got = jcl.kpm_kubo_moments(A, 128, r, v_data=v).cpu().numpy()
         assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) <= 2e-6, slices
+
+
+def _mu_nm_fp64(torch, A, B, v_data, r, M, chunk=1 << 17):
+    """(1/R) sum_r <r| v T_m v T_n |r> from complex128 stacks of the complex128
+    step path (jkpm_moments_keep; pinned to the oracle by the moment tests),
+    Rk_n = v T_n r by jkpm_aAxby, contracted as a complex128 GEMM."""
+    from minijclsquant_amd import _kpm
+
+    N, K, R = A.shape[0], A.len_row, r.shape[0]
+    H = nat.DeviceELL(A.data, A.indices, N, K, 1 / B, "c128")
+    V = nat.DeviceELL.like(H, v_data, 1.0, "c128")
+    _, T = H.moments_keep(_kpm.as_batch(r, N, "c128"), M)
+    Rk = torch.zeros((M, N, R), dtype=torch.complex128, device="cuda")
+    for n in range(M):
+        V.aAxby(1.0, 0.0, T[n], Rk[n])
+    del T
+    _, L = H.moments_keep(Rk[0].clone(), M)
+    L2, R2 = L[:M].reshape(M, N * R), Rk.reshape(M, N * R)
+    mu = torch.zeros((M, M), dtype=torch.complex128, device="cuda")
+    for k0 in range(0, N * R, chunk):
+        k1 = min(N * R, k0 + chunk)
+        mu += torch.mm(L2[:, k0:k1].conj().contiguous(),
+                       R2[:, k0:k1].transpose(0, 1).contiguous())
+    return mu / R
+
+
+def test_fp64_reference_vs_oracle(torch_gpu):
+    """The FP64 reference used at full size equals the oracle's mu_nm on a
+    small lattice (so it is pinned to the same brute-force checks)."""
+    A, B = lattice(8, seed=8)
+    r = vectors(A.shape[0], 3, 61)
+    v = jcl.modifier_velocity(A, "x")
+    ref = _mu_nm_fp64(torch_gpu, A, B, v, r, 40).cpu().numpy()
+    want = oracle_mu(A, B, v, r, 40)
+    assert np.max(np.abs(ref - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+def test_kubo_workload_size_real_velocity_vs_fp64(torch_gpu):
+    """The Kubo workload at full size with the real v_x (honeycomb 512 x 512
+    cells, N = 524,288, M = 512, R = 4): complex64 stacks + FP32 MFMA mu_nm
+    against complex128 stacks contracted in FP64, <= 2e-5 of max |mu|; the
+    Kubo-Greenwood sigma on 512 energies <= 1e-4 of its peak."""
+    torch = torch_gpu
+    A, B = lattice(512, seed=2024)
+    r = vectors(A.shape[0], 4, 700)
+    M = 512
+    mu = jcl.kpm_kubo_moments(A, M, r, "x")
+    es = np.linspace(-0.99, 0.99, 512)
+    sig = nat.kubo_sigma(mu, es, 0, B).cpu().numpy().real
+    mu = mu.cpu().numpy()
+    torch.cuda.empty_cache()
+    ref = _mu_nm_fp64(torch, A, B, jcl.modifier_velocity(A, "x"), r, M)
+    sig_ref = nat.kubo_sigma(ref, es, 0, B).cpu().numpy().real
+    ref = ref.cpu().numpy()
+    torch.cuda.empty_cache()
+    err = np.max(np.abs(mu - ref)) / np.max(np.abs(ref))
+    serr = np.max(np.abs(sig - sig_ref)) / np.max(np.abs(sig_ref))
+    print(f"mu_nm err {err:.3e}, sigma err {serr:.3e}")
+    assert err <= TOL_MU, err
+    assert serr <= 1e-4, serr
