@@ -753,6 +753,39 @@ def bench_kubo(args, torch, jcl, nat, _kpm):
     return out
 
 
+def probe_lattices(torch, jcl, nat, _kpm, steps=9):
+    """Counter probe (see --probe-lattices): the step kernel on four shapes,
+    each with row chunks of exactly one lattice stride (2048 rows), in this
+    order: P16, D16, H8, S8."""
+    H, _, _ = jcl.honeycomb_lattice(1024, t=-2.7, W=1.0, seed=2024)
+    A = jcl.ell_matrix(H)
+    B = (3 * 2.7 + 0.5) * 1.01
+    N = int(A.shape[0])
+    h = A.device("c64", 1 / B)
+    for R, dbl, tag in ((16, False, "P16"), (16, True, "D16"), (8, False, "H8")):
+        r = np.stack([jcl.random_vector_generator(N, 1000 + s) for s in range(R)])
+        vr = _kpm.as_batch(r, N, "c64")
+        h.set_launch(38, 4, dbl)
+        if dbl:
+            nat.moments_ex(h, None, vr, 2 * steps, True)
+        else:
+            h.moments(vr, steps + 1)
+        torch.cuda.synchronize()
+        log(f"{tag}: N={N} R={R} {'doubling' if dbl else 'plain'} done")
+    H, _ = jcl.square_lattice(2048, t=-1.0, W=1.0, seed=7)
+    A = jcl.ell_matrix(H)
+    N = int(A.shape[0])
+    h = A.device("c64", 1 / (4.5 * 1.01))
+    r = np.stack([jcl.random_vector_generator(N, 2000 + s) for s in range(8)])
+    vr = _kpm.as_batch(r, N, "c64")
+    h.set_launch(38, 8, False)
+    h.moments(vr, steps + 1)
+    torch.cuda.synchronize()
+    log(f"S8: N={N} R=8 plain done")
+    print(json.dumps({"probe": "lattices", "shapes": ["P16", "D16", "H8", "S8"],
+                      "step_launches_each": steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -785,6 +818,12 @@ def main():
                     help="timed steps of each sub-workload (at most --steps)")
     ap.add_argument("--fd-coords", type=int, default=16,
                     help="grad workload: coordinates of the finite-difference check")
+    ap.add_argument("--probe-lattices", action="store_true",
+                    help="short run for hardware counters: 8 step launches (after a first "
+                         "step) of four shapes whose row chunks are one lattice stride: "
+                         "honeycomb 1024x1024 cells R=16 plain and doubling and R=8 "
+                         "(variant 38, 4 work-groups per CU), square 2048x2048 R=8 "
+                         "(variant 38, 8 work-groups per CU)")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware counters: 3 copies of 1 GiB with the "
                          "library's copy kernel, then 16 moments of the workload (15 "
@@ -804,6 +843,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
+    if args.probe_lattices:
+        probe_lattices(torch, jcl, nat, _kpm)
+        return
     subs = {"config1": bench_config1, "batch": bench_batch, "grad": bench_grad,
             "kubo": bench_kubo}
     if args.workload != "dos":
