@@ -53,10 +53,11 @@ def _compile(src: Path) -> Path:
 
 
 def build(verbose: bool = True) -> Path:
-    """Compiles every csrc/*.hip for gfx950 and links the shared library."""
+    """Compiles every csrc/*.hip (and the host-only csrc/*.cpp) for gfx950 and
+    links the shared library."""
     OBJ.mkdir(exist_ok=True)
     LIB_DIR.mkdir(exist_ok=True)
-    sources = sorted(CSRC.glob("*.hip"))
+    sources = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     if not sources:
         raise RuntimeError(f"no HIP sources under {CSRC}")
     jobs = min(len(sources), int(os.environ.get("MAX_JOBS", "8")))

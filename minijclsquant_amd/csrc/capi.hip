@@ -23,6 +23,7 @@ namespace {
 thread_local std::string g_err;
 
 constexpr int kRing = 64;  // steps of partials held before one reduction launch
+constexpr int kXcdCount = 8;  // XCDs of the MI355X: tables are padded to a multiple
 
 // ---- profiling of the step kernel (events on the launch stream) ----
 // A recorded run: events before its first and after its last step launch,
@@ -1105,34 +1106,12 @@ void free_table(ChunkTable &c) {
 // neighbour rows at the same time (one HBM fetch shared through L2).
 int build_table(const jkpm_ell *h, int G, const long long *row_offsets, const int *chunk_rows,
                 int mult, const DeviceInfo &dev, ChunkTable *out) {
-    const long target = std::max<long>(G, (long)dev.cu_count * mult);
-    std::vector<long> cnt(G, 1);
-    auto rows_of = [&](int g) { return (long)(row_offsets[g + 1] - row_offsets[g]); };
-    auto size_of = [&](int g) { return (double)rows_of(g) / (double)cnt[g]; };
-    auto cmp = [&](int a, int b) { return size_of(a) < size_of(b); };
-    std::priority_queue<int, std::vector<int>, decltype(cmp)> q(cmp);
-    for (int g = 0; g < G; ++g) q.push(g);
-    long total = G;
-    while (total < target && !q.empty()) {
-        const int g = q.top();
-        q.pop();
-        if (rows_of(g) / 64 <= cnt[g]) continue;  // chunks of at least 64 rows: g is done
-        ++cnt[g];
-        ++total;
-        q.push(g);
-    }
-    std::vector<long> rows{0};
-    std::vector<int> gch{0};
-    for (int g = 0; g < G; ++g) {
-        const long n0 = row_offsets[g], ng = rows_of(g);
-        long per = (ng + cnt[g] - 1) / cnt[g];
-        per = (per + 63) / 64 * 64;
-        if (chunk_rows && chunk_rows[g] > 0) per = chunk_rows[g];
-        for (long r0 = 0; r0 < ng; r0 += per) rows.push_back(n0 + std::min(ng, r0 + per));
-        gch.push_back((int)rows.size() - 1);
-    }
-    // Pad with empty chunks to a multiple of the XCD count; they belong to no group.
-    while ((rows.size() - 1) % 8 != 0 && rows.size() - 1 >= 8) rows.push_back(h->e.n);
+    std::vector<long> rows;
+    std::vector<int> gch;
+    std::string perr;
+    if (plan_chunks(h->e.n, G, row_offsets, chunk_rows, (long)dev.cu_count * mult, kXcdCount,
+                    &rows, &gch, &perr))
+        return arg_error(perr);
     ChunkTable c;
     c.nchunks = (int)rows.size() - 1;
     hipError_t e;
@@ -1905,9 +1884,10 @@ int jkpm_tiles_create(const jkpm_ell *h, int R, int max_rows, jkpm_tiles **out) 
     std::vector<int> colh((size_t)H.n * H.k);
     JKPM_HIP(hipMemcpy(colh.data(), H.col, colh.size() * sizeof(int), hipMemcpyDeviceToHost));
     TilePlanHost plan;
-    int rc = build_tile_plan(H.n, H.k, colh, cap, &plan);
-    if (!rc) rc = validate_tile_plan(H.n, H.k, colh, cap, plan);
-    if (rc) return rc;
+    std::string perr;
+    if (build_tile_plan(H.n, H.k, colh, cap, kXcdCount, &plan, &perr) ||
+        validate_tile_plan(H.n, H.k, colh, cap, plan, &perr))
+        return arg_error(perr);
     auto *t = new (std::nothrow) jkpm_tiles;
     if (!t) return arg_error("host allocation failed");
     t->h = h;
@@ -1953,7 +1933,7 @@ int jkpm_tiles_create(const jkpm_ell *h, int R, int max_rows, jkpm_tiles **out) 
          (e = hipMemcpy(t->hrows, plan.hrows.data(), plan.hrows.size() * sizeof(int),
                         hipMemcpyHostToDevice)) != hipSuccess))
         return fail(e, "hipMemcpy(tile plan)");
-    rc = launch_tile_permute(H, t->perm, t->inv, t->val, t->col, true, nullptr);
+    int rc = launch_tile_permute(H, t->perm, t->inv, t->val, t->col, true, nullptr);
     if (!rc) rc = launch_halo_gather(*t, true, nullptr);
     if (!rc && (e = hipStreamSynchronize(nullptr)) != hipSuccess) return fail(e, "tile permute");
     if (rc) {
@@ -1974,9 +1954,10 @@ int jkpm_tiles_plan_check(int n_rows, int len_row, const int *col_slot_major, in
     for (int j : colh)
         if (j < 0 || j >= n_rows) return arg_error("column index out of range");
     TilePlanHost plan;
-    int rc = build_tile_plan(n_rows, len_row, colh, cap, &plan);
-    if (!rc) rc = validate_tile_plan(n_rows, len_row, colh, cap, plan);
-    if (rc) return rc;
+    std::string perr;
+    if (build_tile_plan(n_rows, len_row, colh, cap, kXcdCount, &plan, &perr) ||
+        validate_tile_plan(n_rows, len_row, colh, cap, plan, &perr))
+        return arg_error(perr);
     if (npatch) *npatch = plan.npatch;
     if (halo_rows) *halo_rows = (long long)plan.hrows.size();
     if (max_ext) *max_ext = plan.max_ext;

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/jclkpm.h"
+#include "plan.h"
 
 struct jkpm_tiles;
 
@@ -158,19 +159,10 @@ int launch_csr_to_ell(const int *indptr, const int *indices, const double *data,
 int launch_ell_cols_unpack(const Ell &H, int *out, hipStream_t s);
 
 // ---- two steps per launch over LDS patches (tiles.hip) ----
-struct TilePlanHost {
-    int npatch = 0, max_ext = 0;
-    std::vector<int> perm, inv, pstart, hoff, hrows;  // new -> old, old -> new, tables
-    std::vector<short> lcol;                          // [k][n] patch-local columns
-};
 // Rows of T_{n+1} the LDS tile holds for a vector group of rb (value type vtype).
 int tile_cap_rows(int vtype, int rb);
 int tile_max_group();
-// Patches of at most `cap` patch + halo rows, grown breadth-first; colh is the
-// slot-major [k][n] column array.
-int build_tile_plan(int n, int k, const std::vector<int> &colh, int cap, TilePlanHost *out);
-int validate_tile_plan(int n, int k, const std::vector<int> &colh, int cap,
-                       const TilePlanHost &plan);
+// (the patch plan itself, TilePlanHost and build_tile_plan, is in plan.h)
 // Values (and columns, renumbered) of H in the plan's row order.
 int launch_tile_permute(const Ell &H, const int *perm, const int *inv, void *val_t, int *col_t,
                         bool with_cols, hipStream_t s);
