@@ -33,6 +33,9 @@ extern "C" {
 #define JKPM_ERR_HIP 2         /* a HIP runtime call failed */
 #define JKPM_ERR_ALLOC 3       /* device allocation failed */
 #define JKPM_ERR_WORKSPACE 4   /* workspace smaller than required */
+/* Test hook: with JKPM_FAULT=alloc in the environment, handle creation
+ * (jkpm_ell_create and the entry points built on it) returns JKPM_ERR_ALLOC
+ * before allocating, so callers can exercise their error paths. */
 
 #define JKPM_C128 0
 #define JKPM_C64 1
@@ -120,6 +123,15 @@ int jkpm_ell_set_values(jkpm_ell *h, const double *data, double scale,
  * plain recursion tolerates and these forms do not. */
 #define JKPM_HERMITIAN_RTOL 1e-10
 int jkpm_ell_hermiticity(const jkpm_ell *h, double *abs_err, double *max_abs, void *stream);
+
+/* Non-finite entries (NaN, +-Inf) among the n doubles of device array x
+ * (e.g. the 2*M*R doubles of a moment array): *first_bad = the smallest
+ * such index or -1, *count = how many.  Synchronises `stream`; refused under
+ * stream capture.  No reference counterpart (the reference returns NaN
+ * moments silently); growing moments usually mean a spectral bound below
+ * the largest |eigenvalue|. */
+int jkpm_check_finite(const double *x, long long n, long long *first_bad, long long *count,
+                      void *stream);
 
 /* Row-major complex128 copy of the values in the units they were uploaded in
  * (stored values / scale), to host or device memory (dst_on_device).  The

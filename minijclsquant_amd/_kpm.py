@@ -45,14 +45,21 @@ def moments(A, M: int, v_r, v_l=None, v_l2=None, dtype="c128", scale: float = 1.
     if doubling:
         if v_l is not None or v_l2 is not None:
             raise ValueError("the doubling form needs v_l = v_r")
-        return nat.moments_ex(h, None, vr, M, True).transpose(0, 1)[None]
+        mu = nat.moments_ex(h, None, vr, M, True)
+        nat.check_finite(mu, "moments [m, r]", _NONFINITE_HINT)
+        return mu.transpose(0, 1)[None]
     vl = vr if v_l is None else as_batch(v_l, n, dtype)
     vl2 = None if v_l2 is None else as_batch(v_l2, n, dtype)
     for w in (vl, vl2):
         if w is not None and w.shape != vr.shape:
             raise ValueError("left and right vectors must have the same shape")
     mu = h.moments(vr, M, vl, vl2)  # [n_left, M, R]
+    nat.check_finite(mu, "moments [l, m, r]", _NONFINITE_HINT)
     return mu.transpose(1, 2)
+
+
+_NONFINITE_HINT = ("; the inputs hold NaN/Inf, or the spectral bound is below the largest "
+                   "|eigenvalue| so that T_n(H/B) grows without bound (set H.bounds wider)")
 
 
 def to_host(mu, single: bool):

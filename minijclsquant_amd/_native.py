@@ -11,6 +11,7 @@ over ``extern "C"`` symbols (jclsquant/cuda_cython/obv.pyx:9-49).
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from pathlib import Path
 
@@ -59,6 +60,8 @@ SIGNATURES = {
     "jkpm_ell_set_values": (_int, [_vp, _vp, _dbl, _int, _vp]),
     "jkpm_ell_get_values": (_int, [_vp, _vp, _int, _vp]),
     "jkpm_ell_hermiticity": (_int, [_vp, _dp, _dp, _vp]),
+    "jkpm_check_finite": (_int, [_vp, _c.c_longlong, _c.POINTER(_c.c_longlong),
+                                 _c.POINTER(_c.c_longlong), _vp]),
     "jkpm_zvec": (_int, [_int, _c.c_longlong, _dp, _dp, _vp, _vp, _vp, _vp, _vp]),
     "jkpm_moments_noalloc_setup": (_int, []),
     "jkpm_profile_read_kind": (_int, [_int, _dp, _c.POINTER(_c.c_longlong), _int]),
@@ -860,6 +863,26 @@ def kubo_sigma(mu, e_scaled, kind: int, bound: float, stream=None):
 def profile_enable(every: int) -> None:
     """Record HIP events around every `every`-th step launch (0 = off)."""
     check(lib().jkpm_profile_enable(int(every)))
+
+
+def check_finite(t, what="result", hint="", stream=None) -> None:
+    """Raises FloatingPointError when the complex128/float64 device tensor t
+    holds NaN or Inf (jkpm_check_finite); JKPM_CHECK_FINITE=0 skips the check."""
+    if os.environ.get("JKPM_CHECK_FINITE", "1") == "0":
+        return
+    torch = _torch()
+    if t.dtype not in (torch.complex128, torch.float64) or not t.is_cuda:
+        raise ValueError("check_finite takes complex128 or float64 device tensors")
+    t = t.contiguous()
+    n = t.numel() * (2 if t.dtype == torch.complex128 else 1)
+    first, count = _c.c_longlong(), _c.c_longlong()
+    check(lib().jkpm_check_finite(t.data_ptr(), n, _c.byref(first), _c.byref(count),
+                                  stream_ptr(stream)))
+    if count.value:
+        idx = int(first.value) // (2 if t.dtype == torch.complex128 else 1)
+        pos = tuple(int(i) for i in np.unravel_index(idx, tuple(t.shape)))
+        raise FloatingPointError(f"{count.value} non-finite values in the {what}, the first at "
+                                 f"index {pos}{hint}")
 
 
 JKPM_PROF_STEP = 0

@@ -499,6 +499,10 @@ int jkpm_ell_create(const double *data, const int *indices, int n_rows, int len_
                                  std::to_string(t) + " is outside [0, n_rows)");
     }
 
+    if (inject_alloc_failure()) {
+        set_error("device allocation failed (injected by JKPM_FAULT=alloc)");
+        return JKPM_ERR_ALLOC;
+    }
     auto *h = new (std::nothrow) jkpm_ell;
     if (!h) return arg_error("host allocation failed");
     Ell &H = h->e;
@@ -568,6 +572,10 @@ int jkpm_ell_create_csr(const int *indptr, const int *indices, const double *dat
     if (len_row < 0) return arg_error("len_row must be >= 0 (0: the longest row)");
     int rc = validate_vtype(value_type);
     if (rc) return rc;
+    if (inject_alloc_failure()) {
+        set_error("device allocation failed (injected by JKPM_FAULT=alloc)");
+        return JKPM_ERR_ALLOC;
+    }
     hipStream_t s = (hipStream_t)stream;
     hipError_t e;
     // nnz = indptr[n_rows]
@@ -706,6 +714,10 @@ int jkpm_ell_create_like(const jkpm_ell *pattern, const double *data, double sca
     if (!pattern || !data) return arg_error("pattern/data is NULL");
     int rc = validate_vtype(value_type);
     if (rc) return rc;
+    if (inject_alloc_failure()) {
+        set_error("device allocation failed (injected by JKPM_FAULT=alloc)");
+        return JKPM_ERR_ALLOC;
+    }
     auto *h = new (std::nothrow) jkpm_ell;
     if (!h) return arg_error("host allocation failed");
     Ell &H = h->e;
@@ -762,6 +774,14 @@ int jkpm_ell_set_values(jkpm_ell *h, const double *data, double scale, int src_o
     if (rc) return rc;
     JKPM_HIP(hipStreamSynchronize((hipStream_t)stream));
     return JKPM_OK;
+}
+
+int jkpm_check_finite(const double *x, long long n, long long *first_bad, long long *count,
+                      void *stream) {
+    clear_error();
+    if (!first_bad || !count || (n > 0 && !x)) return arg_error("NULL argument");
+    if (n < 0) return arg_error("n must be >= 0");
+    return count_nonfinite(x, (long)n, first_bad, count, (hipStream_t)stream);
 }
 
 int jkpm_ell_hermiticity(const jkpm_ell *h, double *abs_err, double *max_abs, void *stream) {

@@ -1,4 +1,5 @@
-// Input checks on device matrices: the Hermiticity of a handle's values.
+// Input and output checks: the Hermiticity of a handle's values, non-finite
+// entries of a result array, and the allocation-failure test hook.
 //
 // The doubling form of the moments, the Kubo double moments and the adjoint
 // recursion all rely on A^H = A (they use <T_k|T_k> for mu_2k, <r|v T_m v T_n|r>
@@ -59,7 +60,70 @@ __global__ void __launch_bounds__(kBlock) herm_kernel(const Real *__restrict__ v
     }
 }
 
+// out[0] = smallest index of a non-finite entry (initialised to n by the
+// caller), out[1] = count of non-finite entries.
+__global__ void __launch_bounds__(kBlock) nonfinite_kernel(const double *__restrict__ x, long n,
+                                                           unsigned long long *__restrict__ out) {
+    unsigned long long first = (unsigned long long)n, count = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        if (!isfinite(x[i])) {
+            if ((unsigned long long)i < first) first = (unsigned long long)i;
+            ++count;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        first = min(first, (unsigned long long)__shfl_xor(first, off));
+        count += __shfl_xor(count, off);
+    }
+    if ((threadIdx.x & 63) == 0 && count) {
+        atomicMin(&out[0], first);
+        atomicAdd(&out[1], count);
+    }
+}
+
 }  // namespace
+
+int count_nonfinite(const double *x, long n, long long *first_bad, long long *count,
+                    hipStream_t s) {
+    *first_bad = -1;
+    *count = 0;
+    if (n <= 0) return JKPM_OK;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    JKPM_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone)
+        return arg_error("jkpm_check_finite synchronises its stream: not under capture");
+    unsigned long long *d = nullptr;
+    JKPM_HIP(hipMalloc(&d, 2 * sizeof(unsigned long long)));
+    unsigned long long host[2] = {(unsigned long long)n, 0};
+    hipError_t e = hipMemcpyAsync(d, host, sizeof(host), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        DeviceInfo dev;
+        int rc = device_info(&dev);
+        if (rc) {
+            hipFree(d);
+            return rc;
+        }
+        const long want = (n + kBlock - 1) / kBlock;
+        const int grid = (int)std::max(1L, std::min(want, (long)dev.cu_count * 4));
+        hipLaunchKernelGGL(nonfinite_kernel, dim3(grid), dim3(kBlock), 0, s, x, n, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(host, d, sizeof(host), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "non-finite check", __FILE__, __LINE__);
+    if (host[1]) {
+        *first_bad = (long long)host[0];
+        *count = (long long)host[1];
+    }
+    return JKPM_OK;
+}
+
+bool inject_alloc_failure() {
+    const char *v = getenv("JKPM_FAULT");
+    return v && strcmp(v, "alloc") == 0;
+}
 
 int hermiticity(const jkpm_ell *h, hipStream_t s, double *abs_err, double *max_abs) {
     if (h->herm_err >= 0.0) {

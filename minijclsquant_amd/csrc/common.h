@@ -231,6 +231,13 @@ int hermiticity(const jkpm_ell *h, hipStream_t s, double *abs_err, double *max_a
 // JKPM_ERR_ARG naming `what` unless max |a_ij - conj(a_ji)| <=
 // JKPM_HERMITIAN_RTOL * max |a_ij|.
 int require_hermitian(const jkpm_ell *h, hipStream_t s, const char *what);
+// Index of the first non-finite double of x[0..n) and the count (-1, 0 when
+// all are finite).  Synchronises s.
+int count_nonfinite(const double *x, long n, long long *first_bad, long long *count,
+                    hipStream_t s);
+// JKPM_FAULT=alloc in the environment: handle creation reports an
+// allocation failure (a test hook for the callers' error paths).
+bool inject_alloc_failure();
 
 }  // namespace jkpm
 

@@ -54,7 +54,9 @@ def kpm_kubo_moments(H, M, random_vector, direction="x", v_data=None):
     values (row-major, on H's slots)."""
     Hd, Vd = velocity_handle(H, direction, v_data)
     r = _kpm.as_batch(random_vector, int(H.shape[0]), "c64")
-    return nat.kubo_mu_nm(Hd, Vd, r, int(M))
+    mu = nat.kubo_mu_nm(Hd, Vd, r, int(M))
+    nat.check_finite(mu, "double moments [m, n]", _kpm._NONFINITE_HINT)
+    return mu
 
 
 def kpm_sigma(H, M=None, random_vector=None, direction="x", energies=None, kind="greenwood",
