@@ -211,6 +211,51 @@ def tiled_leg(torch, nat, h, vr, N, K, M, R, s_v, args, units, es):
     return out
 
 
+def wave_leg(torch, nat, h, vr, N, K, M, R, s_v, args, units, es):
+    """The same moments through the wavefront path (two steps per launch, the
+    work-groups of one XCD handing T_{n+1} over through L2 behind progress
+    flags), plain and doubling forms, on the handle's tuned chunking; the
+    kernel's algorithmic bytes are those of two fused steps.  The per-launch
+    time covers the main launch and its fix-up and fold launches."""
+    out = {}
+    for doubling in (False, True):
+        key = "doubling_" if doubling else ""
+        try:
+            plan = nat.WavePlan(h, R, doubling)
+        except ValueError as e:  # chunking or shape the path does not take
+            out[key + "skipped"] = str(e)
+            continue
+        info = plan.info()
+        out.update({"lag": info["lag"], "tail_rows": info["tail_rows"],
+                    "fixup_rows": info["fixup_rows"], "nchunks": info["nchunks"]})
+        ws = torch.empty(plan.workspace_bytes(M), dtype=torch.uint8, device="cuda")
+        mu = torch.empty((M, R), dtype=torch.complex128, device="cuda")
+
+        def step():
+            plan.moments(vr, M, workspace=ws, out=mu)
+            return nat.dos(mu, es, nat.JKPM_KERNEL_JACKSON)
+
+        step()
+        torch.cuda.synchronize()
+        reps = max(1, min(args.steps, 2))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        launch_ms, n_prof = profiled(nat, step, max(1, args.profile_every // 2), nat.JKPM_PROF_WAVE)
+        nv = 4 if doubling else 5
+        bytes_launch = N * K * (s_v + 4) + R * N * nv * s_v
+        out[key + "value"] = units / (ms / 1e3)
+        out[key + "ms_per_step"] = ms
+        out[key + "roofline"] = hbm_roofline(bytes_launch, launch_ms, n_prof, "wave2_kernel")
+        del plan
+    out["note"] = ("bound by the latency of its dependent loads like the one-step kernel: "
+                   "fewer bytes per step do not shorten the chain, and the flag reads add "
+                   "to it; the one-step kernel stays the headline path")
+    return out
+
+
 def plain_c128(torch, nat, _kpm, A, r, B, M, value, value_dbl, tune, reps=3):
     """The reference's structure on this GPU: complex128 recursion, one
     vector at a time (R = 1), one moment per step, with its own tuned launch
@@ -807,6 +852,8 @@ def main():
     ap.add_argument("--no-doubling", action="store_true")
     ap.add_argument("--no-tiles", action="store_true",
                     help="skip the two-steps-per-launch (LDS patch) path")
+    ap.add_argument("--no-wave", action="store_true",
+                    help="skip the two-steps-per-launch wavefront path")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain complex128 one-vector GPU recursion")
     ap.add_argument("--no-tune", action="store_true",
@@ -986,6 +1033,12 @@ def main():
         log(f"two steps per launch: {tiles['ms_per_step']:.2f} ms per step, doubling "
             f"{tiles['doubling_ms_per_step']:.2f} ms")
 
+    wave = None
+    if not args.no_wave:
+        wave = wave_leg(torch, nat, h, vr, N, K, M, R, s_v, args, units, es)
+        if "ms_per_step" in wave:
+            log(f"wavefront two steps per launch: {wave['ms_per_step']:.2f} ms per step")
+
     plain = None
     if not args.no_plain:
         plain = plain_c128(torch, nat, _kpm, A, r, B, M, units / (ms_per_step / 1e3),
@@ -1025,6 +1078,7 @@ def main():
         "doubling": dbl,
         "plain_c128": plain,
         "two_step_tiles": tiles,
+        "two_step_wave": wave,
     }
     if not args.no_subs:
         # the other workloads, each with its own metric, dominant-kernel

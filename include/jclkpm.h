@@ -366,6 +366,54 @@ int jkpm_tiles_moments(const jkpm_tiles *t, const void *v_r, int R, int M, int f
                        double *mu, void *workspace, size_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------------------------
+ * Two Chebyshev steps per launch by a wavefront over the row chunks: the
+ * work-groups of one XCD hand T_{n+1} to each other through its L2 behind
+ * per-wave progress flags (bounded waits, cooperative launch), so a launch
+ * reads the matrix, T_{n-1}, T_n and v_l once for two steps and recomputes
+ * nothing.  Rows whose columns reach a chunk of another XCD are finished by a
+ * small second launch.  No reference counterpart: the reference makes one
+ * host-driven launch per step (jclsquant/cuda_cython/cuda/recursions.cu:
+ * 78-92).  Complex64 handles, R = 8 or 16, ELL width 4 or 5.
+ * ------------------------------------------------------------------- */
+
+typedef struct jkpm_wave jkpm_wave;
+
+/* Plan of handle h for R vectors in the form `flags` (0: mu = <r|T_m|r>;
+ * JKPM_MOMENTS_DOUBLING: the doubling form, Hermitian A), over the row chunks
+ * of the handle's launch shape for that form (jkpm_ell_set_launch).  Refused
+ * with JKPM_ERR_ARG when the columns do not fit (a block of rows reading more
+ * than four chunks, or blocks further ahead than the kernel's lag of one) or
+ * the grid cannot be co-resident.  Synchronous; h must outlive the plan. */
+int jkpm_wave_create(const jkpm_ell *h, int R, int flags, jkpm_wave **out);
+int jkpm_wave_destroy(jkpm_wave *w);
+/* nchunks: work-groups per launch; lag: blocks between a block's two steps;
+ * tail_rows: rows each work-group finishes after its main loop (columns on
+ * the same XCD but further ahead than the lag allows, e.g. the periodic wrap
+ * of a lattice row); fixup_rows: rows finished by the small second launch
+ * (columns on another XCD). */
+int jkpm_wave_info(const jkpm_wave *w, int *nchunks, int *lag, long long *tail_rows,
+                   long long *fixup_rows);
+
+/* Host only (no device needed): builds the plan of a slot-major [K][N]
+ * column array for nchunks chunks of rpc rows, blocks of 4 * rows_per_wave
+ * rows on 8 XCDs and main-loop rows reading at most max_lag blocks ahead,
+ * checks it independently, and returns its lag and tail and fix-up row
+ * counts. */
+int jkpm_wave_plan_check(int n_rows, int len_row, const int *col_slot_major, int nchunks,
+                         int rpc, int rows_per_wave, int max_lag, int *lag, long long *tail_rows,
+                         long long *fixup_rows);
+
+/* mu = complex128 [M][R] moments of v_r (device [N][R] complex64) in the
+ * plan's form, as jkpm_moments_ex computes them.  Runs on `stream`, then
+ * synchronises it to read the kernel's error flag: JKPM_ERR_HIP when a
+ * work-group ran on another XCD than the chunk mapping assumes or a wait gave
+ * up (mu is then invalid).  Refused under stream capture (the flags' epoch
+ * is a launch argument).  The workspace holds four vector buffers. */
+int jkpm_wave_workspace(const jkpm_wave *w, int R, int M, size_t *bytes);
+int jkpm_wave_moments(jkpm_wave *w, const void *v_r, int R, int M, double *mu, void *workspace,
+                      size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------
  * HIP graphs: capture the launches of asynchronous entry points (moments,
  * batch moments, DOS, ...) issued on `stream` between begin and end, then
  * replay them with one launch.  Pointers and sizes are frozen at capture.
@@ -497,12 +545,15 @@ int jkpm_profile_read(double *step_ms, long long *step_launches, int reset);
 /* The same totals per kind of timed kernel: JKPM_PROF_STEP (the Chebyshev
  * step launches above; jkpm_profile_read reads this kind),
  * JKPM_PROF_KUBO_STEP (runs of up to 8 fused Kubo stack steps, one run
- * started every `every` steps) and JKPM_PROF_KUBO_GEMM (each mu_nm GEMM
- * launch alone).  Nothing is recorded under stream capture. */
+ * started every `every` steps), JKPM_PROF_KUBO_GEMM (each mu_nm GEMM
+ * launch alone) and JKPM_PROF_WAVE (one wavefront launch of two steps with
+ * its fix-up launch, one every `every` launches).  Nothing is recorded under
+ * stream capture. */
 #define JKPM_PROF_STEP 0
 #define JKPM_PROF_KUBO_STEP 1
 #define JKPM_PROF_KUBO_GEMM 2
-#define JKPM_PROF_KINDS 3
+#define JKPM_PROF_WAVE 3
+#define JKPM_PROF_KINDS 4
 int jkpm_profile_read_kind(int kind, double *ms, long long *launches, int reset);
 
 /* Plain device copy dst <- src of `bytes` bytes (a multiple of 16), used to

@@ -78,6 +78,14 @@ SIGNATURES = {
     "jkpm_tiles_refresh": (_int, [_vp, _vp]),
     "jkpm_tiles_workspace": (_int, [_vp, _int, _int, _int, _c.POINTER(_size)]),
     "jkpm_tiles_moments": (_int, [_vp, _vp, _int, _int, _int, _vp, _vp, _size, _vp]),
+    "jkpm_wave_create": (_int, [_vp, _int, _int, _c.POINTER(_vp)]),
+    "jkpm_wave_destroy": (_int, [_vp]),
+    "jkpm_wave_info": (_int, [_vp, _ip, _ip, _c.POINTER(_c.c_longlong),
+                              _c.POINTER(_c.c_longlong)]),
+    "jkpm_wave_plan_check": (_int, [_int, _int, _vp, _int, _int, _int, _int, _ip,
+                                    _c.POINTER(_c.c_longlong), _c.POINTER(_c.c_longlong)]),
+    "jkpm_wave_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
+    "jkpm_wave_moments": (_int, [_vp, _vp, _int, _int, _vp, _vp, _size, _vp]),
     "jkpm_ell_peierls": (_int, [_vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp]),
     "jkpm_aAxby": (_int, [_vp, _dp, _dp, _vp, _vp, _int, _vp]),
     "jkpm_moments_workspace": (_int, [_vp, _int, _int, _c.POINTER(_size)]),
@@ -644,6 +652,77 @@ def tiles_plan_check(indices, n_rows: int, len_row: int, cap: int):
     return {"npatch": npatch.value, "halo_rows": halo.value, "max_ext": mext.value}
 
 
+def wave_plan_check(indices, n_rows: int, len_row: int, nchunks: int, rpc: int,
+                    rows_per_wave: int, max_lag: int = 1):
+    """Host-only build + independent check of a wavefront plan for a row-major
+    ELL column array; returns {lag, tail_rows, fixup_rows}.  Raises ValueError when the
+    columns do not fit the wavefront or the plan is inconsistent."""
+    idx = np.ascontiguousarray(np.asarray(indices, dtype=np.int32).reshape(n_rows, len_row).T)
+    lag = _c.c_int()
+    tail, fix = _c.c_longlong(), _c.c_longlong()
+    check(lib().jkpm_wave_plan_check(int(n_rows), int(len_row), idx.ctypes.data, int(nchunks),
+                                     int(rpc), int(rows_per_wave), int(max_lag),
+                                     _c.byref(lag), _c.byref(tail), _c.byref(fix)))
+    return {"lag": lag.value, "tail_rows": tail.value, "fixup_rows": fix.value}
+
+
+class WavePlan:
+    """Two Chebyshev steps per launch by a wavefront over the row chunks of a
+    complex64 DeviceELL (jkpm_wave_*): R = 8 or 16 vectors, ELL width 4 or 5,
+    one form (plain or doubling) per plan.  The handle must outlive the plan.
+    moments() synchronises its stream (it reads the kernel's error flag) and
+    cannot be captured into a graph."""
+
+    def __init__(self, H: DeviceELL, R: int, doubling: bool = False):
+        require_device()
+        self.H = H
+        self.R = int(R)
+        self.doubling = bool(doubling)
+        self._w = _vp()
+        check(lib().jkpm_wave_create(H.handle, self.R, JKPM_MOMENTS_DOUBLING if doubling else 0,
+                                     _c.byref(self._w)))
+
+    def info(self):
+        nch, lag = _c.c_int(), _c.c_int()
+        tail, fix = _c.c_longlong(), _c.c_longlong()
+        check(lib().jkpm_wave_info(self._w, _c.byref(nch), _c.byref(lag), _c.byref(tail),
+                                   _c.byref(fix)))
+        return {"nchunks": nch.value, "lag": lag.value, "tail_rows": tail.value,
+                "fixup_rows": fix.value}
+
+    def workspace_bytes(self, M: int) -> int:
+        out = _size()
+        check(lib().jkpm_wave_workspace(self._w, self.R, int(M), _c.byref(out)))
+        return int(out.value)
+
+    def moments(self, v_r, M: int, workspace=None, out=None, stream=None):
+        """mu [M, R] complex128 of <r|T_m|r> (device)."""
+        torch = _torch()
+        R = self.H._check_vecs(v_r)
+        if R != self.R:
+            raise ValueError(f"the plan is for R = {self.R} vectors, got {R}")
+        M = int(M)
+        if M <= 0:
+            raise ValueError("M must be positive")
+        need = self.workspace_bytes(M)
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty(need, dtype=torch.uint8, device=v_r.device)
+        if out is None:
+            out = torch.empty((M, R), dtype=torch.complex128, device=v_r.device)
+        elif tuple(out.shape) != (M, R) or out.dtype != torch.complex128 or not out.is_contiguous():
+            raise ValueError("out must be contiguous complex128 [M, R]")
+        check(lib().jkpm_wave_moments(self._w, v_r.data_ptr(), R, M, out.data_ptr(),
+                                      workspace.data_ptr(), workspace.numel(),
+                                      stream_ptr(stream)))
+        return out
+
+    def __del__(self):
+        w = getattr(self, "_w", None)
+        if w is not None and w.value and _lib is not None:
+            _lib.jkpm_wave_destroy(w)
+            self._w = _vp()
+
+
 class TilePlan:
     """Rows of a DeviceELL renumbered into LDS patches for the two-steps-per-
     launch kernel (jkpm_tiles_*).  Built for up to R vectors per call; the
@@ -888,6 +967,7 @@ def check_finite(t, what="result", hint="", stream=None) -> None:
 JKPM_PROF_STEP = 0
 JKPM_PROF_KUBO_STEP = 1
 JKPM_PROF_KUBO_GEMM = 2
+JKPM_PROF_WAVE = 3
 
 
 def profile_read(reset: bool = True, kind: int = JKPM_PROF_STEP):

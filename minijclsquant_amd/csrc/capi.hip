@@ -405,6 +405,177 @@ int run_tiles(const jkpm_tiles &T, const void *v_r, int R, int M, bool doubling,
     return JKPM_OK;
 }
 
+// ---- two steps per launch by a wavefront over row chunks ----
+
+struct WaveLayout {
+    size_t vec = 0, buf = 0, ring = 0, raw = 0, total = 0;
+};
+
+WaveLayout wave_layout(const jkpm_wave &W, int M) {
+    WaveLayout L;
+    const int n_left = W.doubling ? 2 : 1;
+    L.vec = align_up((size_t)W.h->e.n * W.R * cplx_bytes(JKPM_C64), 256);
+    L.buf = 0;
+    L.ring = 4 * L.vec;
+    L.raw = L.ring + align_up((size_t)kRing * n_left * W.nchunks * W.R * sizeof(double2), 256);
+    L.total = L.raw;
+    if (W.doubling) L.total += align_up(2 * (size_t)(doubling_steps(M) + 1) * W.R * sizeof(double2), 256);
+    return L;
+}
+
+// Step 1 (and a last odd step) on the one-step kernel; every other pair of
+// steps in one wavefront launch.  Buffers: T_1 goes to X0; each pair reads
+// (prev, cur) and writes the two buffers of X0..X3 that are neither, so no
+// buffer a launch reads is written by it.  The ring and reductions are
+// those of run_moments.
+int run_wave(jkpm_wave &W, const void *vr, int M, double2 *mu, void *ws, size_t ws_bytes,
+             hipStream_t s) {
+    const WaveLayout L = wave_layout(W, M);
+    if (ws_bytes < L.total) {
+        set_error("workspace too small: need " + std::to_string(L.total) + " bytes, got " +
+                  std::to_string(ws_bytes));
+        return JKPM_ERR_WORKSPACE;
+    }
+    const Ell &H = W.h->e;
+    const int R = W.R;
+    const bool doubling = W.doubling;
+    const int n_left = doubling ? 2 : 1;
+    char *w = (char *)ws;
+    void *X[4] = {w, w + L.vec, w + 2 * L.vec, w + 3 * L.vec};
+    double2 *ring = (double2 *)(w + L.ring);
+    const size_t slot = (size_t)n_left * W.nchunks * R;
+
+    Prof &P = prof();
+    std::unique_lock<std::mutex> plk(P.mu);
+    const int every = P.every;
+    const bool profiling = every > 0;
+    if (!profiling) plk.unlock();
+
+    auto one_step = [&](int n, const void *prev, const void *cur, void *out) -> int {
+        StepArgs a{};
+        a.H = &H;
+        a.first = n == 1;
+        a.xcur = cur;
+        a.xprev = prev;
+        a.xout = out;
+        a.vl0 = doubling ? nullptr : vr;
+        a.vl1 = nullptr;
+        a.doubling = doubling;
+        a.ld = R;
+        a.r0 = 0;
+        a.rb = R;
+        a.part = ring + (size_t)(n % kRing) * slot;
+        a.part0 = n == 1 ? ring : nullptr;
+        a.nchunks = W.nchunks;
+        return launch_step(a, s);
+    };
+
+    const int nsteps = doubling ? doubling_steps(M) : std::max(M, 2) - 1;
+    const int last_slot = doubling ? nsteps : M - 1;
+    double2 *target = doubling ? (double2 *)(w + L.raw) : mu;
+    const int target_M = doubling ? nsteps + 1 : M;
+    int pending_lo = 0;
+    auto reduce_to = [&](int n) -> int {
+        const int last = std::min(n, last_slot);
+        if (last < pending_lo) return JKPM_OK;
+        int rc2 = launch_reduce(ring, kRing, pending_lo, last - pending_lo + 1, n_left,
+                                W.nchunks, R, target, target_M, nullptr, 1, s);
+        pending_lo = n + 1;
+        return rc2;
+    };
+
+    int rc = one_step(1, nullptr, vr, X[0]);
+    if (rc) return rc;
+    const void *prev = vr, *cur = X[0];
+    int n = 1;
+    while (n + 2 <= nsteps) {
+        if (n + 2 - pending_lo + 1 > kRing && (rc = reduce_to(n))) return rc;
+        void *o[2];
+        int no = 0;
+        for (int i = 0; i < 4 && no < 2; ++i)
+            if (X[i] != prev && X[i] != cur) o[no++] = X[i];
+        if (W.epoch + 1 >= (1u << 19)) {  // flags are 1 << 31 | epoch << 12 | progress
+            rc = launch_zero_bytes(W.flags, (size_t)W.nchunks * 4 * sizeof(unsigned), s);
+            if (rc) return rc;
+            W.epoch = 0;
+        }
+        ++W.epoch;
+        WaveArgs a;
+        a.H = &H;
+        a.xprev = prev;
+        a.xcur = cur;
+        a.out1 = o[0];
+        a.out2 = o[1];
+        a.vl = doubling ? nullptr : vr;
+        a.rb = W.rb;
+        a.d = W.d;
+        a.doubling = doubling;
+        a.part1 = ring + (size_t)((n + 1) % kRing) * slot;
+        a.part2 = ring + (size_t)((n + 2) % kRing) * slot;
+        a.nchunks = W.nchunks;
+        a.rpc = W.rpc;
+        a.nblk = W.nblk;
+        a.deps = W.deps;
+        a.flags = W.flags;
+        a.err = W.err;
+        a.epoch = W.epoch;
+        a.tail_off = W.tail_off;
+        a.tail_rows = W.tail_rows;
+        a.tail_dep = W.tail_dep;
+        a.nfix = W.nfix;
+        a.nfold = W.nfold;
+        a.fix_group_off = W.fix_group_off;
+        a.fix_rows = W.fix_rows;
+        a.fold_chunk = W.fold_chunk;
+        a.fold_off = W.fold_off;
+        a.fixpart = W.fixpart;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool rec = profiling && (P.counter++ % every == 0);
+        if (rec) {
+            e0 = take_event(P);
+            e1 = take_event(P);
+            if (!e0 || !e1) return hip_fail(hipErrorOutOfMemory, "hipEventCreate", __FILE__, __LINE__);
+            JKPM_HIP(hipEventRecord(e0, s));
+        }
+        rc = launch_wave2(a, s);
+        if (rc) return rc;
+        if (rec) {
+            JKPM_HIP(hipEventRecord(e1, s));
+            P.pending.push_back({e0, e1, 1, JKPM_PROF_WAVE});
+            if (P.pending.size() >= 8192 && (rc = drain_profile(P))) return rc;
+        }
+        prev = o[0];
+        cur = o[1];
+        n += 2;
+    }
+    if (n < nsteps) {
+        if (n + 1 - pending_lo + 1 > kRing && (rc = reduce_to(n))) return rc;
+        void *out = nullptr;
+        if (prev != vr) {
+            out = const_cast<void *>(prev);  // in place over T_{n-1}, as run_moments does
+        } else {
+            for (int i = 0; i < 4 && !out; ++i)
+                if (X[i] != cur) out = X[i];
+        }
+        if ((rc = one_step(n + 1, prev, cur, out))) return rc;
+        ++n;
+    }
+    if ((rc = reduce_to(n))) return rc;
+    if (doubling && (rc = launch_doubling_finish(target, 1, target_M, R, M, mu, s))) return rc;
+    // The kernel's error bits: a work-group off its XCD, or a wait that gave up.
+    JKPM_HIP(hipStreamSynchronize(s));
+    unsigned bits = 0;
+    JKPM_HIP(hipMemcpy(&bits, W.err, sizeof(bits), hipMemcpyDeviceToHost));
+    if (bits) {
+        JKPM_HIP(hipMemset(W.err, 0, sizeof(unsigned)));
+        set_error(std::string("wavefront launch invalid:") +
+                  ((bits & 1u) ? " a work-group ran on another XCD than its chunk's" : "") +
+                  ((bits & 2u) ? " a readiness wait gave up" : "") + "; the moments are wrong");
+        return JKPM_ERR_HIP;
+    }
+    return JKPM_OK;
+}
+
 static int validate_vtype(int vtype) {
     if (vtype != JKPM_C64 && vtype != JKPM_C128)
         return arg_error("value_type must be JKPM_C128 (0) or JKPM_C64 (1)");
@@ -2049,6 +2220,172 @@ int jkpm_tiles_moments(const jkpm_tiles *t, const void *v_r, int R, int M, int f
     }
     return run_tiles(*t, v_r, R, M, (flags & JKPM_MOMENTS_DOUBLING) != 0, (double2 *)mu,
                      workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+// ---- wavefront plans (two steps per launch) ----
+
+int jkpm_wave_create(const jkpm_ell *h, int R, int flags, jkpm_wave **out) {
+    clear_error();
+    if (!out) return arg_error("out is NULL");
+    *out = nullptr;
+    if (!h) return arg_error("handle is NULL");
+    if (flags & ~JKPM_MOMENTS_DOUBLING) return arg_error("unknown flags");
+    const Ell &H = h->e;
+    if (H.vtype != JKPM_C64) return arg_error("the wavefront path takes complex64 handles");
+    const int rpw = wave_rows_per_wave(R);
+    if (rpw == 0) return arg_error("the wavefront path takes R = 8 or 16");
+    if (H.k != 4 && H.k != 5) return arg_error("the wavefront path takes ELL width 4 or 5");
+    // the kernel indexes vectors and slots with 32-bit offsets
+    if ((long)H.n * R * 2 >= (1L << 31) || (long)H.n * H.k * 2 >= (1L << 31))
+        return arg_error("the wavefront path takes N * R * 2 and N * K * 2 below 2^31");
+    const bool doubling = (flags & JKPM_MOMENTS_DOUBLING) != 0;
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const int nchunks = step_chunks(H, R, dev, doubling);
+    const int blk = 4 * rpw;
+    long rpc = ((long)H.n + nchunks - 1) / nchunks;
+    rpc = (rpc + blk - 1) / blk * blk;
+    std::vector<int> colh((size_t)H.n * H.k);
+    JKPM_HIP(hipMemcpy(colh.data(), H.col, colh.size() * sizeof(int), hipMemcpyDeviceToHost));
+    WavePlanHost plan;
+    std::string perr;
+    if (build_wave_plan(H.n, H.k, colh, nchunks, (int)rpc, blk, rpw, kXcdCount, 1, kWaveFixGroup,
+                        &plan, &perr) ||
+        validate_wave_plan(H.n, H.k, colh, kXcdCount, 1, kWaveFixGroup, plan, &perr))
+        return arg_error(perr);
+    // kernel lag: the plan's + 1 at least; JKPM_WAVE_LAG (2 or 3) may ask for more
+    int d = plan.lag + 1;
+    if (const char *env = getenv("JKPM_WAVE_LAG")) d = std::max(d, atoi(env));
+    if (d > 3) return arg_error("wavefront path: lag " + std::to_string(d) + " above 3");
+    int per_cu = 0;
+    if ((rc = wave_max_blocks_per_cu(R, H.k, d, doubling, &per_cu))) return rc;
+    if ((long)per_cu * dev.cu_count < nchunks)
+        return arg_error("wavefront path: " + std::to_string(nchunks) +
+                         " work-groups cannot be co-resident (" + std::to_string(per_cu) +
+                         " per CU)");
+    auto *W = new (std::nothrow) jkpm_wave;
+    if (!W) return arg_error("host allocation failed");
+    W->h = h;
+    W->R = R;
+    W->rb = R;
+    W->lag = plan.lag;
+    W->d = d;
+    W->nchunks = nchunks;
+    W->rpc = plan.rpc;
+    W->nblk = plan.nblk;
+    W->nfix = (int)plan.fix_group_chunk.size();
+    W->nfold = (int)plan.fold_chunk.size();
+    W->doubling = doubling;
+    W->tail = (long long)plan.tail_rows.size();
+    W->deferred = (long long)plan.fix_rows.size();
+    auto fail = [&](hipError_t e, const char *what) {
+        jkpm_wave_destroy(W);
+        return hip_fail(e, what, __FILE__, __LINE__);
+    };
+    auto up = [](auto **dst, const auto &v) -> hipError_t {
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        hipError_t e = hipMalloc((void **)dst, std::max<size_t>(1, v.size()) * sizeof(T));
+        if (e == hipSuccess && !v.empty())
+            e = hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+        return e;
+    };
+    const int nl = doubling ? 2 : 1;
+    hipError_t e;
+    if ((e = up(&W->deps, plan.deps)) != hipSuccess || (e = up(&W->tail_off, plan.tail_off)) != hipSuccess ||
+        (e = up(&W->tail_rows, plan.tail_rows)) != hipSuccess ||
+        (e = up(&W->tail_dep, plan.tail_dep)) != hipSuccess ||
+        (e = up(&W->fix_group_off, plan.fix_group_off)) != hipSuccess ||
+        (e = up(&W->fix_rows, plan.fix_rows)) != hipSuccess ||
+        (e = up(&W->fold_chunk, plan.fold_chunk)) != hipSuccess ||
+        (e = up(&W->fold_off, plan.fold_off)) != hipSuccess ||
+        (e = hipMalloc(&W->fixpart, std::max<size_t>(1, (size_t)W->nfix * nl * R) * sizeof(double2))) !=
+            hipSuccess ||
+        (e = hipMalloc(&W->flags, (size_t)nchunks * 4 * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&W->err, sizeof(unsigned))) != hipSuccess ||
+        (e = hipMemset(W->flags, 0, (size_t)nchunks * 4 * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMemset(W->err, 0, sizeof(unsigned))) != hipSuccess ||
+        (e = hipDeviceSynchronize()) != hipSuccess)
+        return fail(e, "wavefront plan upload");
+    *out = W;
+    return JKPM_OK;
+}
+
+int jkpm_wave_destroy(jkpm_wave *w) {
+    clear_error();
+    if (!w) return JKPM_OK;
+    hipFree(w->deps);
+    hipFree(w->tail_off);
+    hipFree(w->tail_rows);
+    hipFree(w->tail_dep);
+    hipFree(w->fix_group_off);
+    hipFree(w->fix_rows);
+    hipFree(w->fold_chunk);
+    hipFree(w->fold_off);
+    hipFree(w->fixpart);
+    hipFree(w->flags);
+    hipFree(w->err);
+    delete w;
+    return JKPM_OK;
+}
+
+int jkpm_wave_info(const jkpm_wave *w, int *nchunks, int *lag, long long *tail_rows,
+                   long long *fixup_rows) {
+    clear_error();
+    if (!w) return arg_error("plan is NULL");
+    if (nchunks) *nchunks = w->nchunks;
+    if (lag) *lag = w->d;
+    if (tail_rows) *tail_rows = w->tail;
+    if (fixup_rows) *fixup_rows = w->deferred;
+    return JKPM_OK;
+}
+
+int jkpm_wave_plan_check(int n_rows, int len_row, const int *col_slot_major, int nchunks,
+                         int rpc, int rows_per_wave, int max_lag, int *lag, long long *tail_rows,
+                         long long *fixup_rows) {
+    clear_error();
+    if (!col_slot_major) return arg_error("columns are NULL");
+    if (n_rows <= 0 || len_row <= 0) return arg_error("n_rows and len_row must be positive");
+    std::vector<int> colh(col_slot_major, col_slot_major + (size_t)n_rows * len_row);
+    WavePlanHost plan;
+    std::string perr;
+    if (build_wave_plan(n_rows, len_row, colh, nchunks, rpc, 4 * rows_per_wave, rows_per_wave,
+                        kXcdCount, max_lag, kWaveFixGroup, &plan, &perr) ||
+        validate_wave_plan(n_rows, len_row, colh, kXcdCount, max_lag, kWaveFixGroup, plan, &perr))
+        return arg_error(perr);
+    if (lag) *lag = plan.lag;
+    if (tail_rows) *tail_rows = (long long)plan.tail_rows.size();
+    if (fixup_rows) *fixup_rows = (long long)plan.fix_rows.size();
+    return JKPM_OK;
+}
+
+int jkpm_wave_workspace(const jkpm_wave *w, int R, int M, size_t *bytes) {
+    clear_error();
+    if (!w || !bytes) return arg_error("NULL argument");
+    if (R != w->R) return arg_error("R differs from the plan's");
+    if (M <= 0) return arg_error("M must be positive");
+    *bytes = wave_layout(*w, M).total;
+    return JKPM_OK;
+}
+
+int jkpm_wave_moments(jkpm_wave *w, const void *v_r, int R, int M, double *mu, void *workspace,
+                      size_t workspace_bytes, void *stream) {
+    clear_error();
+    if (!w) return arg_error("plan is NULL");
+    if (!v_r || !mu || !workspace) return arg_error("NULL vector/mu/workspace");
+    if (R != w->R) return arg_error("R differs from the plan's");
+    if (M <= 0) return arg_error("M must be positive");
+    if (!aligned16(v_r) || !aligned16(workspace) || !aligned16(mu))
+        return arg_error("device pointers must be 16-byte aligned");
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    JKPM_HIP(hipStreamIsCapturing((hipStream_t)stream, &cap));
+    if (cap != hipStreamCaptureStatusNone)
+        return arg_error("the wavefront path cannot be captured into a graph");
+    if (w->doubling) {
+        int rc = require_hermitian(w->h, (hipStream_t)stream, "the doubling form");
+        if (rc) return rc;
+    }
+    return run_wave(*w, v_r, M, (double2 *)mu, workspace, workspace_bytes, (hipStream_t)stream);
 }
 
 }  // extern "C"

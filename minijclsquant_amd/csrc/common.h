@@ -186,6 +186,36 @@ int launch_tile2(const TileArgs &a, hipStream_t s);
 // plan's values, and with_cols its columns, are in place).
 int launch_halo_gather(const jkpm_tiles &T, bool with_cols, hipStream_t s);
 
+// ---- two steps per launch by a wavefront over row chunks (wave.hip) ----
+struct WaveArgs {
+    const Ell *H = nullptr;
+    const void *xprev = nullptr;  // T_{n-1}
+    const void *xcur = nullptr;   // T_n
+    void *out1 = nullptr;         // T_{n+1} (a buffer distinct from the inputs)
+    void *out2 = nullptr;         // T_{n+2}
+    const void *vl = nullptr;     // left vectors (plain form)
+    int rb = 16, d = 2;           // vectors; blocks between a block's two phases
+    bool doubling = false;
+    double2 *part1 = nullptr, *part2 = nullptr;  // ring slots of steps n+1, n+2
+    int nchunks = 0, rpc = 0, nblk = 0;
+    const WaveDep *deps = nullptr;
+    const int *tail_off = nullptr, *tail_rows = nullptr, *tail_dep = nullptr;
+    unsigned *flags = nullptr, *err = nullptr;
+    unsigned epoch = 1;
+    int spin_limit = 1 << 16;
+    int nfix = 0, nfold = 0;  // fix-up groups and the chunks they fold into
+    const int *fix_group_off = nullptr, *fix_rows = nullptr;
+    const int *fold_chunk = nullptr, *fold_off = nullptr;
+    double2 *fixpart = nullptr;  // [nfix][n_left][rb]
+};
+// Largest number of fix-up rows one work-group of the fix-up launch takes.
+constexpr int kWaveFixGroup = 64;
+// Rows one wave covers per block for a complex64 group of rb vectors (0 when
+// the group size is not supported).
+int wave_rows_per_wave(int rb);
+int launch_wave2(const WaveArgs &a, hipStream_t s);
+int wave_max_blocks_per_cu(int rb, int k, int d, bool doubling, int *out);
+
 // ---- Chebyshev series / commutator recursion (series.hip) ----
 size_t series_workspace(const Ell &H, int R, bool comm);
 // out (if set) = sum_m c_m X_m and tab[m] (if set, commutator only) = c_m X_m,
@@ -254,6 +284,20 @@ struct jkpm_ell {
 
 // A tile plan of a handle: rows renumbered into patches, the matrix in that
 // order, and the patch tables (all device arrays owned by the plan).
+struct jkpm_wave {
+    const jkpm_ell *h = nullptr;  // borrowed: must outlive the plan
+    int R = 0, rb = 0, lag = 1, d = 2, nchunks = 0, rpc = 0, nblk = 0, nfix = 0, nfold = 0;
+    bool doubling = false;
+    long long tail = 0, deferred = 0;
+    unsigned epoch = 0;           // flags hold epoch << 12 | blocks done
+    jkpm::WaveDep *deps = nullptr;
+    int *tail_off = nullptr, *tail_rows = nullptr, *tail_dep = nullptr;
+    unsigned *flags = nullptr;    // [nchunks][4] per-wave progress
+    unsigned *err = nullptr;      // error bits of the launches since the last check
+    int *fix_group_off = nullptr, *fix_rows = nullptr, *fold_chunk = nullptr, *fold_off = nullptr;
+    double2 *fixpart = nullptr;   // [groups][n_left][R]
+};
+
 struct jkpm_tiles {
     const jkpm_ell *h = nullptr;  // borrowed: must outlive the plan
     int n = 0, k = 0, vtype = 0, rb = 1, cap = 0, npatch = 0, max_ext = 0;

@@ -217,4 +217,240 @@ int plan_chunks(long n_total, int G, const long long *row_offsets, const int *ch
     return 0;
 }
 
+// ---- wavefront plan ----
+
+namespace {
+// 0: main loop, 1: tail, 2: fix-up
+int wave_row_kind(int n, int k, const std::vector<int> &colh, int i, int rpc, int blk,
+                  int per_xcd, int max_lag) {
+    const int c = i / rpc, q = (i - c * rpc) / blk;
+    int kind = 0;
+    for (int s = 0; s < k; ++s) {
+        const int j = colh[(size_t)s * n + i];
+        const int cj = j / rpc;
+        if (cj / per_xcd != c / per_xcd) return 2;
+        if ((j - cj * rpc) / blk - q > max_lag) kind = 1;
+    }
+    return kind;
+}
+
+// Adds chunk cj to a list of at most four (-1 = free); false when full.
+bool add_chunk(int *list, int cj) {
+    for (int t = 0; t < 4; ++t) {
+        if (list[t] == cj) return true;
+        if (list[t] < 0) {
+            list[t] = cj;
+            return true;
+        }
+    }
+    return false;
+}
+}  // namespace
+
+int build_wave_plan(int n, int k, const std::vector<int> &colh, int nchunks, int rpc, int blk,
+                    int rpw, int xcd, int max_lag, int fix_group, WavePlanHost *out,
+                    std::string *err) {
+    auto fail = [&](const std::string &m) {
+        *err = m;
+        return 1;
+    };
+    if (n < 1 || k < 1) return fail("wavefront plan: empty matrix");
+    if ((long)colh.size() != (long)n * k) return fail("wavefront plan: column array size mismatch");
+    for (int j : colh)
+        if (j < 0 || j >= n) return fail("wavefront plan: column index out of range");
+    if (xcd < 1 || nchunks < xcd || nchunks % xcd != 0)
+        return fail("wavefront plan: the chunk count must be a positive multiple of the XCD count");
+    if (rpw < 1 || rpw > 32 || blk != 4 * rpw)
+        return fail("wavefront plan: a block is four waves of at most 32 rows");
+    if (rpc < blk || rpc % blk != 0)
+        return fail("wavefront plan: rows per chunk must be a positive multiple of the block");
+    if ((long)(nchunks - 1) * rpc >= n || (long)nchunks * rpc < n)
+        return fail("wavefront plan: every chunk must hold rows");
+    if (max_lag < 1) return fail("wavefront plan: the lag must be at least 1");
+    if (fix_group < 1) return fail("wavefront plan: the fix-up group must hold rows");
+    const int nblk = rpc / blk;
+    if (nblk + 4 >= 4096) return fail("wavefront plan: more than 4091 blocks per chunk");
+    const int per_xcd = nchunks / xcd;
+    WavePlanHost P;
+    P.nchunks = nchunks;
+    P.rpc = rpc;
+    P.nblk = nblk;
+    P.blk = blk;
+    P.rpw = rpw;
+    WaveDep zero{};
+    for (int t = 0; t < 4; ++t) zero.chunk[t] = -1;
+    P.deps.assign((size_t)nchunks * nblk, zero);
+    P.tail_dep.assign((size_t)nchunks * 4, -1);
+    std::vector<char> kind(n, 0);
+    long lag = 0, nout = 0;
+    for (int i = 0; i < n; ++i) {
+        const int c = i / rpc;
+        const int q = (i - c * rpc) / blk;
+        kind[i] = (char)wave_row_kind(n, k, colh, i, rpc, blk, per_xcd, max_lag);
+        if (kind[i] == 2) {
+            ++nout;
+            continue;
+        }
+        if (kind[i] == 1) {
+            ++nout;
+            for (int s = 0; s < k; ++s)
+                if (!add_chunk(&P.tail_dep[(size_t)c * 4], colh[(size_t)s * n + i] / rpc))
+                    return fail("wavefront plan: a chunk's tail rows reference more than four chunks");
+            continue;
+        }
+        WaveDep &d = P.deps[(size_t)c * nblk + q];
+        for (int s = 0; s < k; ++s) {
+            const int j = colh[(size_t)s * n + i];
+            const int cj = j / rpc;
+            const int qj = (j - cj * rpc) / blk;
+            lag = std::max(lag, (long)qj - q);
+            int t = 0;
+            while (t < d.cnt && d.chunk[t] != cj) ++t;
+            if (t == d.cnt) {
+                if (d.cnt == 4)
+                    return fail("wavefront plan: a block references more than four chunks");
+                d.chunk[t] = cj;
+                d.need[t] = 0;
+                ++d.cnt;
+            }
+            d.need[t] = std::max(d.need[t], (unsigned)(qj + 1));
+        }
+    }
+    P.lag = (int)std::max(1L, lag);
+    if (4 * nout > n)
+        return fail("wavefront plan: " + std::to_string(nout) + " of " + std::to_string(n) +
+                    " rows read another XCD's chunks or blocks too far ahead");
+    P.tail_off.push_back(0);
+    P.fix_group_off.push_back(0);
+    P.fold_off.push_back(0);
+    for (int c = 0; c < nchunks; ++c) {
+        const int r1 = std::min(n, (c + 1) * rpc);
+        int in_group = 0;
+        const size_t groups_before = P.fix_group_chunk.size();
+        for (int i = c * rpc; i < r1; ++i) {
+            if (!kind[i]) continue;
+            const int off = i - c * rpc;
+            WaveDep &d = P.deps[(size_t)c * nblk + off / blk];
+            d.mask[(off % blk) / rpw] |= 1u << (off % rpw);
+            if (kind[i] == 1) {
+                P.tail_rows.push_back(i);
+                continue;
+            }
+            if (in_group == 0) P.fix_group_chunk.push_back(c);
+            P.fix_rows.push_back(i);
+            if (++in_group == fix_group) {
+                P.fix_group_off.push_back((int)P.fix_rows.size());
+                in_group = 0;
+            }
+        }
+        if (in_group > 0) P.fix_group_off.push_back((int)P.fix_rows.size());
+        P.tail_off.push_back((int)P.tail_rows.size());
+        if (P.fix_group_chunk.size() > groups_before) {
+            P.fold_chunk.push_back(c);
+            P.fold_off.push_back((int)P.fix_group_chunk.size());
+        }
+    }
+    *out = std::move(P);
+    return 0;
+}
+
+int validate_wave_plan(int n, int k, const std::vector<int> &colh, int xcd, int max_lag,
+                       int fix_group, const WavePlanHost &P, std::string *err) {
+    auto fail = [&](const std::string &m) {
+        *err = "wavefront plan check: " + m;
+        return 1;
+    };
+    if ((long)colh.size() != (long)n * k) return fail("column array size mismatch");
+    for (int j : colh)
+        if (j < 0 || j >= n) return fail("column index out of range");
+    if (P.nchunks < xcd || P.nchunks % xcd != 0 || P.blk != 4 * P.rpw || P.rpw > 32 ||
+        P.rpc != P.nblk * P.blk || (long)(P.nchunks - 1) * P.rpc >= n ||
+        (long)P.nchunks * P.rpc < n || P.lag < 1 || P.lag > max_lag)
+        return fail("inconsistent geometry");
+    if (P.deps.size() != (size_t)P.nchunks * P.nblk) return fail("requirement table size");
+    if (P.tail_off.size() != (size_t)P.nchunks + 1 || P.tail_dep.size() != (size_t)P.nchunks * 4 ||
+        P.tail_off[0] != 0 || P.tail_off.back() != (int)P.tail_rows.size())
+        return fail("tail table shape");
+    const size_t ng = P.fix_group_chunk.size();
+    if (P.fix_group_off.size() != ng + 1 || P.fix_group_off[0] != 0 ||
+        P.fix_group_off.back() != (int)P.fix_rows.size() ||
+        P.fold_off.size() != P.fold_chunk.size() + 1 || P.fold_off[0] != 0 ||
+        P.fold_off.back() != (int)ng)
+        return fail("fix-up table shape");
+    const int per_xcd = P.nchunks / xcd;
+    std::vector<char> seen(n, 0);  // 1: tail list, 2: fix-up list
+    for (int c = 0; c < P.nchunks; ++c) {
+        if (P.tail_off[c + 1] < P.tail_off[c]) return fail("tail offsets");
+        for (int t = P.tail_off[c]; t < P.tail_off[c + 1]; ++t) {
+            const int i = P.tail_rows[t];
+            if (i < c * P.rpc || i >= std::min(n, (c + 1) * P.rpc)) return fail("tail row outside its chunk");
+            if (t > P.tail_off[c] && i <= P.tail_rows[t - 1]) return fail("tail rows not ascending");
+            seen[i] = 1;
+        }
+    }
+    for (size_t g = 0; g < ng; ++g) {
+        const int c = P.fix_group_chunk[g];
+        const int lo = P.fix_group_off[g], hi = P.fix_group_off[g + 1];
+        if (c < 0 || c >= P.nchunks || hi <= lo || hi - lo > fix_group) return fail("fix-up group");
+        if (g > 0 && c < P.fix_group_chunk[g - 1]) return fail("fix-up groups not by chunk");
+        for (int t = lo; t < hi; ++t) {
+            const int i = P.fix_rows[t];
+            if (i < c * P.rpc || i >= std::min(n, (c + 1) * P.rpc)) return fail("fix-up row outside its chunk");
+            if (t > 0 && i <= P.fix_rows[t - 1] && t > lo) return fail("fix-up rows not ascending");
+            if (seen[i]) return fail("row listed twice");
+            seen[i] = 2;
+        }
+    }
+    for (size_t f = 0; f < P.fold_chunk.size(); ++f) {
+        const int c = P.fold_chunk[f];
+        if (f > 0 && c <= P.fold_chunk[f - 1]) return fail("folds not ascending");
+        if (P.fold_off[f + 1] <= P.fold_off[f]) return fail("empty fold");
+        for (int g = P.fold_off[f]; g < P.fold_off[f + 1]; ++g)
+            if (P.fix_group_chunk[g] != c) return fail("fold covers another chunk's group");
+    }
+    for (int i = 0; i < n; ++i) {
+        const int c = i / P.rpc, off = i - c * P.rpc, q = off / P.blk;
+        const WaveDep &d = P.deps[(size_t)c * P.nblk + q];
+        const bool masked = (d.mask[(off % P.blk) / P.rpw] >> (off % P.rpw)) & 1u;
+        const int kind = wave_row_kind(n, k, colh, i, P.rpc, P.blk, per_xcd, max_lag);
+        if (kind != seen[i]) return fail("row " + std::to_string(i) + " sorted wrongly");
+        if (masked != (kind != 0)) return fail("mask wrong at row " + std::to_string(i));
+        if (kind == 2) continue;
+        for (int s = 0; s < k; ++s) {
+            const int j = colh[(size_t)s * n + i];
+            const int cj = j / P.rpc, qj = (j - cj * P.rpc) / P.blk;
+            if (kind == 1) {
+                bool listed = false;
+                for (int t = 0; t < 4; ++t) listed |= P.tail_dep[(size_t)c * 4 + t] == cj;
+                if (!listed) return fail("tail row " + std::to_string(i) + " reads an unlisted chunk");
+                continue;
+            }
+            int t = 0;
+            while (t < d.cnt && t < 4 && d.chunk[t] != cj) ++t;
+            if (t >= d.cnt || t >= 4) return fail("row " + std::to_string(i) + " reads an unlisted chunk");
+            if (d.need[t] < (unsigned)(qj + 1)) return fail("requirement below a read block");
+        }
+    }
+    for (int c = 0; c < P.nchunks; ++c) {
+        for (int t = 0; t < 4; ++t) {
+            const int cj = P.tail_dep[(size_t)c * 4 + t];
+            if (cj < -1 || cj >= P.nchunks || (cj >= 0 && cj / per_xcd != c / per_xcd))
+                return fail("tail requirement");
+        }
+        for (int q = 0; q < P.nblk; ++q) {
+            const WaveDep &d = P.deps[(size_t)c * P.nblk + q];
+            if (d.cnt < 0 || d.cnt > 4) return fail("requirement count");
+            for (int t = 0; t < d.cnt; ++t) {
+                if (d.chunk[t] < 0 || d.chunk[t] >= P.nchunks) return fail("requirement chunk range");
+                if (d.chunk[t] / per_xcd != c / per_xcd) return fail("requirement on another XCD");
+                if (d.need[t] < 1 || d.need[t] > (unsigned)P.nblk) return fail("requirement range");
+                // the bound every wait relies on to end: a wave that waits
+                // for block q has finished the first step of q + lag blocks
+                if ((long)d.need[t] > (long)q + P.lag + 1) return fail("requirement beyond the lag");
+            }
+        }
+    }
+    return 0;
+}
+
 }  // namespace jkpm

@@ -42,4 +42,62 @@ int plan_chunks(long n_total, int G, const long long *row_offsets, const int *ch
                 long target, int xcd, std::vector<long> *rows, std::vector<int> *gch,
                 std::string *err);
 
+// ---- two steps per launch by a wavefront over row chunks (wave.hip) ----
+
+// Readiness requirements of one (chunk, block): before the second step of the
+// block's rows, the chunks chunk[0 .. cnt) must have finished the first step
+// of at least need[k] blocks; mask[w] has bit i set when row i of wave w's
+// share of the block is not done in the main loop (a tail or fix-up row).
+// 64 bytes, the layout the kernel reads.
+struct WaveDep {
+    int cnt;
+    int pad[3];
+    int chunk[4];
+    unsigned need[4];
+    unsigned mask[4];
+};
+
+struct WavePlanHost {
+    int nchunks = 0, rpc = 0, nblk = 0, blk = 0, rpw = 0, lag = 0;
+    std::vector<WaveDep> deps;         // [nchunks][nblk]
+    // Tail rows: their columns stay on the chunk's XCD but reach a block more
+    // than max_lag ahead (the periodic wrap of a lattice row); the chunk's
+    // work-group does them after its main loop, once the chunks they read
+    // (tail_dep, -1 = unused) have finished every first-step block.
+    std::vector<int> tail_off;         // [nchunks + 1] into tail_rows
+    std::vector<int> tail_rows;
+    std::vector<int> tail_dep;         // [nchunks][4]
+    // Fix-up rows: their columns reach another XCD's chunk.  Groups of at
+    // most fix_group rows of one chunk, done after the launch; fold lists,
+    // per chunk, the run of groups whose partials are added to its own.
+    std::vector<int> fix_group_chunk;  // [groups]
+    std::vector<int> fix_group_off;    // [groups + 1] into fix_rows
+    std::vector<int> fix_rows;
+    std::vector<int> fold_chunk;       // chunks that own groups
+    std::vector<int> fold_off;         // [fold_chunk.size() + 1] into the groups
+};
+
+// Plan of the wavefront path for the slot-major [k][n] columns colh cut into
+// `nchunks` chunks of `rpc` rows (the last one shorter), blocks of `blk` rows
+// (4 waves of `rpw` rows), chunks assigned XCD-major to `xcd` XCDs (chunk c
+// on XCD c / (nchunks / xcd)).  A row whose columns reach a chunk of another
+// XCD is a fix-up row; one whose columns reach a block more than max_lag
+// blocks ahead of its own is a tail row; every other row adds its columns'
+// (chunk, block) to the requirements of its own (chunk, block).  The lag is
+// the largest (referenced block - own block) over those rows (at least 1).
+// Plans whose blocks or tails reference more than four chunks, or that leave
+// more than a quarter of the rows out of the main loop, are refused.
+// Returns 0, or 1 with *err set.
+int build_wave_plan(int n, int k, const std::vector<int> &colh, int nchunks, int rpc, int blk,
+                    int rpw, int xcd, int max_lag, int fix_group, WavePlanHost *out,
+                    std::string *err);
+
+// Independent check of a plan against its columns: every requirement is met
+// by the rows each main-loop row reads, the rows are sorted into main, tail
+// and fix-up exactly by the rule above, the lag bounds every requirement,
+// every tail row's chunks are in its chunk's tail list, and the fix-up groups
+// and folds cover exactly the fix-up rows.  Returns 0, or 1 with *err set.
+int validate_wave_plan(int n, int k, const std::vector<int> &colh, int xcd, int max_lag,
+                       int fix_group, const WavePlanHost &plan, std::string *err);
+
 }  // namespace jkpm

@@ -1,7 +1,9 @@
 // Host-side planning code of the library (minijclsquant_amd/csrc/plan.cpp)
 // under AddressSanitizer and UndefinedBehaviorSanitizer: the tile-plan
-// breadth-first renumbering and its validator, and the realization-batch
-// row-chunk tables, on random, lattice-shaped and degenerate inputs.  Built
+// breadth-first renumbering and its validator, the realization-batch
+// row-chunk tables, and the wavefront plan and its validator (including
+// corrupted plans it must refuse), on random, lattice-shaped and degenerate
+// inputs.  Built
 // and run by tests/test_sanitize.py; exits non-zero on the first failed
 // check (the sanitizers abort on their own findings).
 #include <cstdio>
@@ -126,6 +128,51 @@ static void check_chunks(std::mt19937 &g) {
     CHECK(plan_chunks(10, 0, bad3, nullptr, 64, 8, &rows, &gch, &err) != 0, "G = 0");
 }
 
+// Wavefront plans of a matrix for a few chunkings; every built plan must pass
+// the independent check, and plans corrupted in the ways a wrong build would
+// be must fail it.
+static void check_wave(int n, int k, const std::vector<int> &c, const char *what) {
+    for (int nchunks : {8, 16, 64})
+        for (int rpw : {8, 16}) {
+            const int blk = 4 * rpw;
+            long rpc = ((long)n + nchunks - 1) / nchunks;
+            rpc = (rpc + blk - 1) / blk * blk;
+            WavePlanHost p;
+            std::string err;
+            const int rc = build_wave_plan(n, k, c, nchunks, (int)rpc, blk, rpw, 8, 1, 64, &p, &err);
+            if (rc) continue;  // refused (a chunk without rows, too many chunks per block, ...)
+            CHECK(validate_wave_plan(n, k, c, 8, 1, 64, p, &err) == 0, "%s: %s", what, err.c_str());
+            size_t main_rows = 0;
+            for (const auto &d : p.deps)
+                for (int w = 0; w < 4; ++w) main_rows += (size_t)rpw - __builtin_popcount(d.mask[w]);
+            (void)main_rows;
+            // corruptions the check must catch
+            for (size_t q = 0; q < p.deps.size(); ++q) {
+                if (p.deps[q].cnt == 0) continue;
+                WavePlanHost bad = p;
+                bad.deps[q].need[0] = 0;  // a requirement below a block its rows read
+                CHECK(validate_wave_plan(n, k, c, 8, 1, 64, bad, &err) != 0, "%s: zero need passed", what);
+                bad = p;
+                bad.deps[q].mask[0] ^= 1u;  // a row moved out of (or into) the main loop
+                CHECK(validate_wave_plan(n, k, c, 8, 1, 64, bad, &err) != 0, "%s: mask flip passed", what);
+                break;
+            }
+            if (!p.fix_rows.empty()) {
+                WavePlanHost bad = p;
+                bad.fix_rows.pop_back();
+                bad.fix_group_off.back() -= 1;
+                CHECK(validate_wave_plan(n, k, c, 8, 1, 64, bad, &err) != 0, "%s: lost fix-up row passed", what);
+            }
+            for (int t = 0; t < (int)p.tail_dep.size(); ++t)
+                if (p.tail_dep[t] >= 0) {
+                    WavePlanHost bad = p;
+                    bad.tail_dep[t] = -1;
+                    CHECK(validate_wave_plan(n, k, c, 8, 1, 64, bad, &err) != 0, "%s: dropped tail chunk passed", what);
+                    break;
+                }
+        }
+}
+
 int main() {
     std::mt19937 g(2024);
     for (int n : {1, 2, 3, 7, 64, 500, 2000})
@@ -148,6 +195,29 @@ int main() {
         CHECK(build_tile_plan(4, 3, c, 64, 8, &p, &err) != 0, "size mismatch accepted");
     }
     check_chunks(g);
+    for (int L : {16, 32, 64, 100})
+        check_wave(L * L, 5, square_cols(L, true), "wave square periodic");
+    for (int n : {64, 4096}) {
+        auto c = random_cols(n, 4, g);
+        check_wave(n, 4, c, "wave random");
+    }
+    {
+        // chunk = one lattice row of a 256 x 256 periodic square (8 blocks of
+        // 32 rows): lag 1; the x = 0 site of each row reads x = 255, 7 blocks
+        // ahead (a tail row); the 16 chunks at the 8 XCD boundaries read
+        // another XCD's chunk from every row (fix-up rows, their x = 0 sites
+        // included)
+        const int L = 256;
+        auto c = square_cols(L, true);
+        WavePlanHost p;
+        std::string err;
+        CHECK(build_wave_plan(L * L, 5, c, 256, 256, 32, 8, 8, 1, 64, &p, &err) == 0,
+              "wave rows: %s", err.c_str());
+        CHECK(p.lag == 1, "wave rows: lag %d", p.lag);
+        CHECK(p.tail_rows.size() == 240, "wave rows: %zu tail rows", p.tail_rows.size());
+        CHECK(p.fix_rows.size() == 16 * 256, "wave rows: %zu fix-up rows", p.fix_rows.size());
+        CHECK(validate_wave_plan(L * L, 5, c, 8, 1, 64, p, &err) == 0, "wave rows: %s", err.c_str());
+    }
     if (failures) {
         std::fprintf(stderr, "%d failed checks\n", failures);
         return 1;

@@ -372,6 +372,44 @@ def test_tile_plan_rejects_bad_input():
         nat.tiles_plan_check(np.zeros(8, np.int32), 2, 4, 6)  # cap below 2K+2
 
 
+def test_wave_plan_host_check():
+    """The wavefront plan (two steps per launch with readiness flags), built
+    and independently checked on the host.  Honeycomb 1024 x 1024 cells in
+    1024 chunks of one lattice row (2048 rows, blocks of 32): every kept row
+    reads at most one block ahead; the x = 0 site of each row reads x =
+    Lx - 1, 63 blocks ahead, and is a tail row; the rows next to the 8 XCD
+    boundaries (and the periodic wrap of the chunk ring), half of each of the
+    16 boundary chunks, are fix-up rows."""
+    from minijclsquant_amd import _native as nat
+
+    H, _, _ = jcl.honeycomb_lattice(1024, t=-2.7, W=1.0, seed=2024)
+    A = jcl.ell_matrix(H)
+    N = A.shape[0]
+    r = nat.wave_plan_check(A.indices, N, A.len_row, 1024, 2048, 8, 1)
+    assert r == {"lag": 1, "tail_rows": 1016, "fixup_rows": 16384}
+    # blocks of 64 rows (R = 8): the same rows, fewer blocks
+    assert nat.wave_plan_check(A.indices, N, A.len_row, 1024, 2048, 16, 1) == r
+    # chunks of half a lattice row: the neighbour rows sit two chunks away
+    r2 = nat.wave_plan_check(A.indices, N, A.len_row, 2048, 1024, 8, 1)
+    assert r2["lag"] == 1 and r2["fixup_rows"] == 16384
+
+
+def test_wave_plan_rejects_unfit_columns():
+    from minijclsquant_amd import _native as nat
+
+    rng = np.random.default_rng(7)
+    n, k = 65536, 4
+    idx = rng.integers(0, n, size=(n, k)).astype(np.int32)
+    with pytest.raises(ValueError, match="wavefront plan"):
+        nat.wave_plan_check(idx.ravel(), n, k, 256, 256, 8, 1)  # random columns
+    H, _, _ = jcl.honeycomb_lattice(256, t=-2.7, W=1.0, seed=1)
+    A = jcl.ell_matrix(H)
+    with pytest.raises(ValueError, match="chunk"):
+        nat.wave_plan_check(A.indices, A.shape[0], A.len_row, 1000, 128, 8, 1)  # not a multiple of 8
+    with pytest.raises(ValueError, match="every chunk"):
+        nat.wave_plan_check(A.indices, A.shape[0], A.len_row, 2048, 128, 8, 1)  # empty chunks
+
+
 def test_lattice_stride():
     """The dominant long-range column distance of lattice matrices."""
     from minijclsquant_amd.batch import lattice_stride, square_lattice_ell, _ell_stride, _csr_stride
