@@ -338,7 +338,7 @@ def hbm_roofline(bytes_launch, launch_ms, n, kernel, **extra):
 
 def base_line(metric, value, unit, args, ms, dtype, config, higher=True):
     return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": higher,
+            "warmup": max(1, args.warmup), "ms_per_step": ms, "higher_is_better": higher,
             "scaling": "weak", "vs_baseline": None, "dtype": dtype,
             "data": "synthetic (seeded lattice disorder and random-phase vectors)",
             "config": config}
@@ -413,7 +413,7 @@ def bench_batch(args, torch, jcl, nat, _kpm):
     log(f"launch shapes (variant, work-groups per CU): plain {bk.H.get_launch()}, "
         f"doubling {bk.H.get_launch(doubling=True)}")
     bk.capture()
-    for _ in range(args.warmup):
+    for _ in range(max(1, args.warmup)):  # tuning and allocation stay untimed
         bk.replay()
     torch.cuda.synchronize()
     ms = event_ms(torch, bk.replay, args.steps)
@@ -512,7 +512,7 @@ def bench_config1(args, torch, jcl, nat, _kpm):
         nat.dos(mu[0], es, nat.JKPM_KERNEL_JACKSON, stream=stream, out=dos_out, workspace=dws,
                 check_range=False)
 
-    for _ in range(args.warmup):
+    for _ in range(max(1, args.warmup)):  # tuning and allocation stay untimed
         step()
     torch.cuda.synchronize()
     ms = event_ms(torch, step, args.steps)
@@ -531,7 +531,7 @@ def bench_config1(args, torch, jcl, nat, _kpm):
         nat.dos(mu_d, es, nat.JKPM_KERNEL_JACKSON, stream=stream, out=dos_d, workspace=dws,
                 check_range=False)
 
-    for _ in range(args.warmup):
+    for _ in range(max(1, args.warmup)):  # tuning and allocation stay untimed
         step_d()
     torch.cuda.synchronize()
     ms_d = event_ms(torch, step_d, args.steps)
@@ -636,7 +636,7 @@ def bench_grad(args, torch, jcl, nat, _kpm):
         hop.grad = None
         forward().backward()
 
-    for _ in range(args.warmup):
+    for _ in range(max(1, args.warmup)):  # tuning and allocation stay untimed
         both()
     torch.cuda.synchronize()
     with torch.no_grad():
@@ -752,7 +752,7 @@ def bench_kubo(args, torch, jcl, nat, _kpm):
         nat.kubo_mu_nm(Hd, Vd, rr, M, workspace=ws, out=mu)
         return nat.kubo_sigma(mu, es, 0, B)
 
-    for _ in range(args.warmup):
+    for _ in range(max(1, args.warmup)):  # tuning and allocation stay untimed
         step()
     torch.cuda.synchronize()
     ms = event_ms(torch, step, args.steps)
@@ -1085,7 +1085,7 @@ def main():
         # roofline and accuracy field; a few timed steps each
         sub_args = argparse.Namespace(**vars(args))
         sub_args.steps = max(1, min(args.steps, args.sub_steps))
-        sub_args.warmup = min(args.warmup, 1)
+        sub_args.warmup = 1
         del h, vr, ws, ws_dbl, mu
         for name, fn in subs.items():
             torch.cuda.empty_cache()
