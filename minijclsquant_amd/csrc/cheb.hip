@@ -254,8 +254,10 @@ int launch_step(const StepArgs &a, hipStream_t s) {
     double2 *p_part0 = a.part0 ? a.part0 + a.r0 : nullptr;
     int p_first = a.first ? 1 : 0, p_rpc = (int)rpc, p_nch = a.nchunks;
     const long *p_rows = a.chunk_rows;
+    int p_rev = a.reverse ? 1 : 0;
     void *args[] = {&p_val,  &p_col,  &p_n,    &p_k,     &p_xcur,  &p_xprev, &p_xout, &p_vl0,
-                    &p_vl1,  &p_ld,   &p_part, &p_part0, &p_first, &p_rpc,   &p_nch,  &p_rows};
+                    &p_vl1,  &p_ld,   &p_part, &p_part0, &p_first, &p_rpc,   &p_nch,  &p_rows,
+                    &p_rev};
     JKPM_HIP(hipLaunchKernel(e.fn, dim3(a.nchunks), dim3(kBlock), args, 0, s));
     JKPM_CHECK_LAUNCH("cheb_step_kernel");
     return JKPM_OK;

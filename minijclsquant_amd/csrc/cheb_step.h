@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const Real *__restrict__ xcur, const Real *xprev, Real *xout,
     const Real *__restrict__ vl0, const Real *__restrict__ vl1, int ld,
     double2 *__restrict__ part, double2 *__restrict__ part0, int first_arg,
-    int rows_per_chunk, int nchunks, const long *__restrict__ chunk_rows) {
+    int rows_per_chunk, int nchunks, const long *__restrict__ chunk_rows, int reverse) {
     const bool first = F0 && first_arg;
     using V = vec_t<Real, 2 * VPL>;
     using C2 = vec_t<Real, 2>;
@@ -94,6 +94,8 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
             const long rw = base + (long)u * STRIDE;
             ok[u] = rw < row_end;
             rows[u] = ok[u] ? rw : base;
+            // reverse: the chunk is swept from its last row to its first
+            if (reverse) rows[u] = row_begin + row_end - 1 - rows[u];
         }
         Real sr[UNR][VPL], si[UNR][VPL];
 #pragma unroll

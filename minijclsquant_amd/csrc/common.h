@@ -91,6 +91,7 @@ struct StepArgs {
     int nchunks;
     const long *chunk_rows = nullptr;  // device [nchunks+1] row starts, or uniform chunks
     bool doubling = false;  // partials <T_{n+1}|T_{n+1}>, <T_n|T_{n+1}> (no left vectors)
+    bool reverse = false;   // sweep each chunk from its last row to its first
 };
 
 // mu[g][m][r] from the doubling form's raw dots raw[l][g][s][r] (s < S1):
