@@ -439,6 +439,8 @@ def test_tuned_times_short_steps_longer(tmp_path, monkeypatch):
     monkeypatch.setenv("JKPM_TUNE_TABLE", str(tmp_path / "table.json"))
     monkeypatch.delenv("JKPM_LAUNCH", raising=False)
     monkeypatch.setattr(autotune, "_key", lambda h, R, d: f"k|R={R}|{d}")
+    # the finalist round times real launches; the stand-in keeps the sweep's pick
+    monkeypatch.setattr(autotune, "_finalists", lambda h, v_r, v, m, us, d: (v, us))
 
     class Handle:
         def __init__(self, us):
