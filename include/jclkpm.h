@@ -408,7 +408,9 @@ int jkpm_wave_plan_check(int n_rows, int len_row, const int *col_slot_major, int
  * synchronises it to read the kernel's error flag: JKPM_ERR_HIP when a
  * work-group ran on another XCD than the chunk mapping assumes or a wait gave
  * up (mu is then invalid).  Refused under stream capture (the flags' epoch
- * is a launch argument).  The workspace holds four vector buffers. */
+ * is a launch argument).  The workspace holds four vector buffers.  The
+ * plan's progress flags are shared by its calls: one call at a time per
+ * plan (calls that overlap on one plan race). */
 int jkpm_wave_workspace(const jkpm_wave *w, int R, int M, size_t *bytes);
 int jkpm_wave_moments(jkpm_wave *w, const void *v_r, int R, int M, double *mu, void *workspace,
                       size_t workspace_bytes, void *stream);

@@ -671,7 +671,8 @@ class WavePlan:
     complex64 DeviceELL (jkpm_wave_*): R = 8 or 16 vectors, ELL width 4 or 5,
     one form (plain or doubling) per plan.  The handle must outlive the plan.
     moments() synchronises its stream (it reads the kernel's error flag) and
-    cannot be captured into a graph."""
+    cannot be captured into a graph; one call at a time per plan (the plan's
+    progress flags are shared by its calls)."""
 
     def __init__(self, H: DeviceELL, R: int, doubling: bool = False):
         require_device()
