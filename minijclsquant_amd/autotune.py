@@ -1,7 +1,8 @@
 """Launch-shape autotuner with a persistent table.
 
-The step kernel has 48 variants (rows per lane x non-temporal vector loads x
-vectors per lane x non-temporal stores x non-temporal matrix loads) and a
+The step kernel has 52 variants (rows per lane x non-temporal vector loads x
+vectors per lane x non-temporal stores x non-temporal matrix loads, then
+four neighbour-exchange twins: 48..51 twin 0, 2, 12, 14) and a
 work-group count per CU, set separately for the
 plain recursion and for the doubling form; jkpm_autotune times them on the
 caller's matrix and vectors.  Results are cached in a JSON table keyed by
@@ -70,6 +71,33 @@ _FINAL_US = 12000.0  # timed span per finalist pass
 _FINAL_PASSES = 3
 
 
+N_VARIANTS = 52
+_NX_BASE = (0, 2, 12, 14)  # variants 48..51 twin these
+
+
+def _neighbours(v):
+    """Variants one choice away from v: a flipped non-temporal matrix-load,
+    store or stream-load bit within its family (gathering 0..47 or
+    neighbour-exchange 48..51), and its twin in the other family."""
+    def flip_ntm(u):  # non-temporal matrix loads: +-24
+        return u - 24 if u >= 24 else u + 24
+
+    def flip_ns(u):  # non-temporal stores: +-12 within the 24-block
+        return u - 12 if u % 24 >= 12 else u + 12
+
+    def flip_nt(u):  # non-temporal stream loads: bit 1 (12 and 24 are multiples of 4)
+        return u ^ 2
+
+    if v >= 48:
+        base = _NX_BASE[v - 48]
+        out = [48 + _NX_BASE.index(f(base)) for f in (flip_ns, flip_nt)]
+        return out + [base]
+    out = [f(v) for f in (flip_ntm, flip_ns, flip_nt)]
+    if v in _NX_BASE:
+        out.append(48 + _NX_BASE.index(v))
+    return out
+
+
 def _finalists(h, v_r, v, m, us, doubling):
     """Second round over the winner of the sweep and its neighbours one
     variant bit away (non-temporal matrix loads, stores, stream loads), at
@@ -82,7 +110,7 @@ def _finalists(h, v_r, v, m, us, doubling):
     M = steps + 1 if not doubling else 2 * steps
     need = nat.moments_ex_workspace_bytes(h, None, int(v_r.shape[1]), M, doubling)
     ws = torch.empty(need, dtype=torch.uint8, device=v_r.device)
-    cands = [v] + [v ^ bit for bit in (24, 12, 2) if 0 <= v ^ bit < 48]
+    cands = [v] + _neighbours(v)
     best = (float("inf"), v)
     for c in cands:
         h.set_launch(c, m, doubling)
@@ -133,7 +161,7 @@ def tuned(h: "nat.DeviceELL", v_r, steps: int = 8, retune: bool = False,
         return v, m
 
 
-def tuned_batch(bk, steps: int = 16, retune: bool = False, variants=range(48)):
+def tuned_batch(bk, steps: int = 16, retune: bool = False, variants=range(N_VARIANTS)):
     """Step-kernel variant of both forms for a BatchKPM whose chunk tables are
     fixed (chunks of one lattice stride): times every variant on the batch
     itself, where the tables, not chunk_mult, set the work-group count.

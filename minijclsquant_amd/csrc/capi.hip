@@ -1147,13 +1147,15 @@ int jkpm_moments_keep(const jkpm_ell *h, const void *v_r, int R, int M, double *
 // ---- launch shape and autotuner ----
 
 namespace {
-constexpr int kVariants = 48;  // unroll x non-temporal x lane shape x store mode x matrix loads (cheb_step.h)
+// unroll x non-temporal x lane shape x store mode x matrix loads, then the
+// neighbour-exchange twins (cheb_step.h)
+constexpr int kVariants = JKPM_STEP_VARIANTS;
 }
 
 int jkpm_ell_set_launch(jkpm_ell *h, int variant, int chunk_mult, int flags) {
     clear_error();
     if (!h) return arg_error("handle is NULL");
-    if (variant < 0 || variant >= kVariants) return arg_error("variant must be 0..47");
+    if (variant < 0 || variant >= kVariants) return arg_error("variant must be 0.." + std::to_string(kVariants - 1));
     if (chunk_mult < 1 || chunk_mult > 64) return arg_error("chunk_mult must be 1..64");
     if (flags & ~JKPM_MOMENTS_DOUBLING) return arg_error("unknown flags");
     if (flags & JKPM_MOMENTS_DOUBLING) {

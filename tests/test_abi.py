@@ -35,6 +35,22 @@ def test_library_exports_every_declared_symbol():
     assert set(nat.SIGNATURES) == set(declared_functions())
 
 
+def test_header_constants_match_the_wrapper():
+    """Every #define of the header that the ctypes wrapper restates has the
+    same value there (status codes, value types, flags, variant count)."""
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    defines = dict(re.findall(r"^#define\s+(JKPM_[A-Z0-9_]+)\s+(.+?)\s*$", text, flags=re.M))
+    checked = 0
+    for name, expr in defines.items():
+        if not hasattr(nat, name):
+            continue
+        value = eval(expr.replace("u <<", " <<").replace("u)", ")"), {})
+        assert getattr(nat, name) == value, name
+        checked += 1
+    assert checked >= 15
+    assert nat.JKPM_STEP_VARIANTS == 52
+
+
 def test_symbols_are_c_linkage():
     # extern "C": the plain name resolves (no C++ mangling) via dlsym
     lib = ctypes.CDLL(str(nat.LIB_PATH))

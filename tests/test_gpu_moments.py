@@ -261,7 +261,7 @@ def test_launch_variants_and_autotune(torch_gpu, dtype, tmp_path, monkeypatch):
     h = nat.DeviceELL(Ht.data, Ht.indices, A.shape[0], A.len_row, 1.0, dtype)
     vr = _kpm.as_batch(r, A.shape[0], dtype)
     ref = h.moments(vr, 200)[0].cpu().numpy()
-    for v in range(48):
+    for v in range(nat.JKPM_STEP_VARIANTS):
         for m in (1, 4, 32):
             h.set_launch(v, m)
             got = h.moments(vr, 200)[0].cpu().numpy()
@@ -269,7 +269,7 @@ def test_launch_variants_and_autotune(torch_gpu, dtype, tmp_path, monkeypatch):
     # odd batch (narrow lanes and power-of-two groups) under every variant
     r5 = _kpm.as_batch(r[:5], A.shape[0], dtype)
     ref5 = h.moments(r5, 90)[0].cpu().numpy()
-    for v in range(48):
+    for v in range(nat.JKPM_STEP_VARIANTS):
         h.set_launch(v, 8)
         got = h.moments(r5, 90)[0].cpu().numpy()
         assert np.max(np.abs(got - ref5)) / abs(ref5[0, 0]) <= 1e-12, v
@@ -279,7 +279,7 @@ def test_launch_variants_and_autotune(torch_gpu, dtype, tmp_path, monkeypatch):
     # (the two forms round differently: complex64 vectors leave ~1e-7)
     tol_form = 1e-11 if dtype == "c128" else 1e-6
     assert np.max(np.abs(refd - ref)) / abs(ref[0, 0]) <= tol_form
-    for v in range(48):
+    for v in range(nat.JKPM_STEP_VARIANTS):
         for m in (1, 4, 32):
             h.set_launch(v, m, doubling=True)
             assert h.get_launch() == (0, 8)
@@ -287,17 +287,17 @@ def test_launch_variants_and_autotune(torch_gpu, dtype, tmp_path, monkeypatch):
             assert np.max(np.abs(got - refd)) / abs(ref[0, 0]) <= 1e-12, (v, m)
     monkeypatch.setenv("JKPM_TUNE_TABLE", str(tmp_path / "t.json"))
     v, m = autotune.tuned(h, vr, steps=4)
-    assert 0 <= v <= 47 and m in (2, 4, 8, 16, 32)
+    assert 0 <= v < nat.JKPM_STEP_VARIANTS and m in (2, 4, 8, 16, 32)
     assert (tmp_path / "t.json").exists()
     vd, md = autotune.tuned(h, vr, steps=4, doubling=True)
-    assert 0 <= vd <= 47 and md in (2, 4, 8, 16, 32)
+    assert 0 <= vd < nat.JKPM_STEP_VARIANTS and md in (2, 4, 8, 16, 32)
     assert h.get_launch(doubling=True) == (vd, md) and h.get_launch() == (v, m)
     h.set_launch(0, 8)
     h.set_launch(0, 8, doubling=True)
     assert autotune.tuned(h, vr) == (v, m)  # from the table
     assert autotune.tuned(h, vr, doubling=True) == (vd, md)
     with pytest.raises(ValueError):
-        h.set_launch(48, 8)
+        h.set_launch(nat.JKPM_STEP_VARIANTS, 8)
     assert nat.lib().jkpm_ell_set_launch(h.handle, 0, 8, 2) == nat.JKPM_ERR_ARG  # unknown flag
 
 
