@@ -234,16 +234,11 @@ int jkpm_moments_noalloc(const jkpm_ell *h, const void *v_r, const void *v_l0, c
  * the matrix values and columns (24..47): unroll = two rows per lane,
  * nontemporal = non-temporal loads of the read-once vector streams
  * (T_{n-1}, v_l), shape = complex64 vectors per lane 2 << shape (lanes per
- * row shrink to match; c64 groups of 4+ vectors).  48..51 are the
- * neighbour-exchange twins of 0, 2, 12 and 14: each row's own T_n is loaded
- * once, and the slots whose column is that row or the row handled by the
- * next lane group below or above take T_n through cross-lane permutes
- * instead of gathers (complex64 groups of 8 or 16 vectors, complex128 of 4
- * or 8; other shapes run the twin's base).
+ * row shrink to match; c64 groups of 4+ vectors).
  * Variants exist for ELL widths 4 and 5; other shapes use 0.  chunk_mult:
  * work-groups per CU (1..64, default 8).  flags = JKPM_MOMENTS_DOUBLING
  * addresses the doubling form's own setting; 0 the plain recursion's. */
-#define JKPM_STEP_VARIANTS 52
+#define JKPM_STEP_VARIANTS 48
 int jkpm_ell_set_launch(jkpm_ell *h, int variant, int chunk_mult, int flags);
 int jkpm_ell_get_launch(const jkpm_ell *h, int *variant, int *chunk_mult, int flags);
 

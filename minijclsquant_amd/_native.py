@@ -147,8 +147,8 @@ def series(h, v_op, v, M: int, c, want_sum: bool = True, want_table: bool = Fals
     return out, tab
 
 JKPM_MOMENTS_DOUBLING = 1
-# step-kernel launch variants: 0..47 gathering, 48..51 neighbour exchange
-JKPM_STEP_VARIANTS = 52
+# step-kernel launch variants (jkpm_ell_set_launch)
+JKPM_STEP_VARIANTS = 48
 
 
 def moments_ex_workspace_bytes(h, b, R: int, M: int, doubling: bool) -> int:

@@ -1,8 +1,7 @@
 """Launch-shape autotuner with a persistent table.
 
-The step kernel has 52 variants (rows per lane x non-temporal vector loads x
-vectors per lane x non-temporal stores x non-temporal matrix loads, then
-four neighbour-exchange twins: 48..51 twin 0, 2, 12, 14) and a
+The step kernel has 48 variants (rows per lane x non-temporal vector loads x
+vectors per lane x non-temporal stores x non-temporal matrix loads) and a
 work-group count per CU, set separately for the
 plain recursion and for the doubling form; jkpm_autotune times them on the
 caller's matrix and vectors.  Results are cached in a JSON table keyed by
@@ -71,14 +70,12 @@ _FINAL_US = 12000.0  # timed span per finalist pass
 _FINAL_PASSES = 3
 
 
-N_VARIANTS = 52
-_NX_BASE = (0, 2, 12, 14)  # variants 48..51 twin these
+N_VARIANTS = nat.JKPM_STEP_VARIANTS
 
 
 def _neighbours(v):
-    """Variants one choice away from v: a flipped non-temporal matrix-load,
-    store or stream-load bit within its family (gathering 0..47 or
-    neighbour-exchange 48..51), and its twin in the other family."""
+    """Variants one choice away from v: non-temporal matrix loads, stores or
+    stream loads flipped."""
     def flip_ntm(u):  # non-temporal matrix loads: +-24
         return u - 24 if u >= 24 else u + 24
 
@@ -88,14 +85,7 @@ def _neighbours(v):
     def flip_nt(u):  # non-temporal stream loads: bit 1 (12 and 24 are multiples of 4)
         return u ^ 2
 
-    if v >= 48:
-        base = _NX_BASE[v - 48]
-        out = [48 + _NX_BASE.index(f(base)) for f in (flip_ns, flip_nt)]
-        return out + [base]
-    out = [f(v) for f in (flip_ntm, flip_ns, flip_nt)]
-    if v in _NX_BASE:
-        out.append(48 + _NX_BASE.index(v))
-    return out
+    return [f(v) for f in (flip_ntm, flip_ns, flip_nt)]
 
 
 def _finalists(h, v_r, v, m, us, doubling):

@@ -1147,8 +1147,7 @@ int jkpm_moments_keep(const jkpm_ell *h, const void *v_r, int R, int M, double *
 // ---- launch shape and autotuner ----
 
 namespace {
-// unroll x non-temporal x lane shape x store mode x matrix loads, then the
-// neighbour-exchange twins (cheb_step.h)
+// unroll x non-temporal x lane shape x store mode x matrix loads (cheb_step.h)
 constexpr int kVariants = JKPM_STEP_VARIANTS;
 }
 

@@ -158,13 +158,6 @@ StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl, bool firs
     // The doubling kernels lack the first-step accumulators: its first launch
     // takes the base kernel of the shape (the single-form kernels all have them).
     if (first && dbl) return pick_c128(rb, nl, k, 0, dbl, true);
-    if (variant >= kVariantsGather) {
-        // neighbour-exchange twin where one exists, else its gathering base
-        const int base = kNxBase[(variant - kVariantsGather) % kNxVariants];
-        const StepEntry e = pick_step_nx_c128(rb, nl, k, base, dbl);
-        if (e.fn) return e;
-        variant = base;
-    }
     if (variant >= 24) {
         const StepEntry e = pick_step_c128_ntm(rb, nl, k, variant - 24, dbl);
         if (e.fn) return e;

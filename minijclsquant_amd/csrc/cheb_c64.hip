@@ -184,13 +184,6 @@ StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl,
     // The doubling kernels lack the first-step accumulators: its first launch
     // takes the base kernel of the shape (the single-form kernels all have them).
     if (first && dbl) return pick_c64(rb, wide, nl, k, 0, dbl, true);
-    if (variant >= kVariantsGather) {
-        // neighbour-exchange twin where one exists, else its gathering base
-        const int base = kNxBase[(variant - kVariantsGather) % kNxVariants];
-        const StepEntry e = pick_step_nx_c64(rb, wide, nl, k, base, dbl);
-        if (e.fn) return e;
-        variant = base;
-    }
     if (variant >= 24) {
         const StepEntry e = pick_step_c64_ntm(rb, wide, nl, k, variant - 24, dbl);
         if (e.fn) return e;

@@ -41,16 +41,8 @@ __device__ __forceinline__ V load_stream(const V *p) {
 // but of the doubling-form kernels only the base kernel of each shape has
 // them, and the first step of a doubling recursion runs that one.  A kernel
 // without F0 treats first as 0.
-// NX (neighbour exchange, fixed width only): every lane group loads T_n of
-// its own row (coalesced), and a slot whose column is the row itself or the
-// row of the lane group LPR lanes below or above takes its T_n from there
-// through a cross-lane permute instead of a gather; only the other slots
-// (the far neighbours of a lattice) are gathered, under an exec mask.  The
-// match is on the row index itself, so any matrix and either sweep
-// direction are correct, and the arithmetic is that of the gathering kernel
-// (bitwise equal results).
 template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool DBL = false,
-          bool NS = false, bool NTM = false, bool F0 = false, bool NX = false>
+          bool NS = false, bool NTM = false, bool F0 = false>
 __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const Real *__restrict__ val, const int *__restrict__ col, int n, int k_rt,
     const Real *__restrict__ xcur, const Real *xprev, Real *xout,
@@ -92,20 +84,14 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const Real *vls[2] = {vl0, vl1};
 
     constexpr int STRIDE = kWaves * RPW;  // rows between a lane's consecutive rows
-    // NX kernels permute registers between lanes, so every lane of a wave
-    // stays in the loop until the wave's last row (`base` is then the wave's
-    // first row of the pass, and the lane's row is base + lr); a lane that
-    // left early would hold registers that no longer describe its row.
-    const int lane_row = NX ? lr : 0;
-    for (long base = row_begin + wave * RPW + (NX ? 0 : lr); base < row_end;
-         base += (long)STRIDE * UNR) {
+    for (long base = row_begin + wave * RPW + lr; base < row_end; base += (long)STRIDE * UNR) {
         // Rows past the chunk end are clamped to `base` (a valid row) for the
         // loads and their results are dropped.
         long rows[UNR];
         bool ok[UNR];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            const long rw = base + lane_row + (long)u * STRIDE;
+            const long rw = base + (long)u * STRIDE;
             ok[u] = rw < row_end;
             rows[u] = ok[u] ? rw : base;
             // reverse: the chunk is swept from its last row to its first
@@ -135,59 +121,12 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
                     h[u][s] = load_stream<NTM>(reinterpret_cast<const C2 *>(val + 2 * e));
                 }
             V x[UNR][KT];
-            if constexpr (NX) {
-                // rows of this lane group and of the groups LPR lanes below
-                // and above (a wrapped lane is some other row of the wave:
-                // it only matches a column that is that row)
-                const int src_dn = (lane - LPR) & 63, src_up = (lane + LPR) & 63;
-                V mine[UNR];
-                int r_me[UNR], r_dn[UNR], r_up[UNR];
-                bool need[UNR][KT];
 #pragma unroll
-                for (int u = 0; u < UNR; ++u) {
-                    mine[u] = *reinterpret_cast<const V *>(xcur + rows[u] * ld * 2 + voff);
-                    // a clamped lane matches no column (its results are dropped)
-                    r_me[u] = ok[u] ? (int)rows[u] : -1;
-                    r_dn[u] = __shfl(r_me[u], src_dn);
-                    r_up[u] = __shfl(r_me[u], src_up);
-                }
-                // far slots: gathered, for the lanes that need them only
+            for (int u = 0; u < UNR; ++u)
 #pragma unroll
-                for (int u = 0; u < UNR; ++u)
-#pragma unroll
-                    for (int s = 0; s < KT; ++s) {
-                        const int jj = j[u][s];
-                        need[u][s] = jj != r_me[u] && jj != r_dn[u] && jj != r_up[u];
-                        if (need[u][s])
-                            x[u][s] = *reinterpret_cast<const V *>(xcur + (long)jj * ld * 2 + voff);
-                    }
-#pragma unroll
-                for (int u = 0; u < UNR; ++u) {
-                    V dn, up;
-#pragma unroll
-                    for (int c = 0; c < 2 * VPL; ++c) {
-                        dn[c] = __shfl(mine[u][c], src_dn);
-                        up[c] = __shfl(mine[u][c], src_up);
-                    }
-#pragma unroll
-                    for (int s = 0; s < KT; ++s)
-                        if (!need[u][s]) {
-                            const int jj = j[u][s];
-                            x[u][s] = jj == r_me[u] ? mine[u] : (jj == r_dn[u] ? dn : up);
-                        }
-                    if constexpr (DBL) {
-                        own[u] = mine[u];
-                        have_own[u] = true;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < UNR; ++u)
-#pragma unroll
-                    for (int s = 0; s < KT; ++s)
-                        x[u][s] = *reinterpret_cast<const V *>(xcur + (long)j[u][s] * ld * 2 + voff);
-            }
-            if constexpr (DBL && !NX) {
+                for (int s = 0; s < KT; ++s)
+                    x[u][s] = *reinterpret_cast<const V *>(xcur + (long)j[u][s] * ld * 2 + voff);
+            if constexpr (DBL) {
 #pragma unroll
                 for (int u = 0; u < UNR; ++u)
 #pragma unroll
@@ -363,14 +302,6 @@ struct StepEntry {
 // dbl selects the doubling form (two partial sets, see DBL above); first
 // selects the base kernel of the shape with the first-step accumulators (F0).
 // variant 0..47: 24..47 are 0..23 with non-temporal matrix loads (NTM).
-// 48..51: the neighbour-exchange (NX) twins of variants 0, 2, 12 and 14 (one
-// row per lane, the first lane shape; cached or non-temporal stream loads and
-// stores), for the group sizes the workloads run (complex64 groups of 8 and
-// 16 vectors, complex128 groups of 4 and 8); other shapes take the base.
-constexpr int kVariantsGather = 48;
-constexpr int kNxVariants = 4;
-constexpr int kNxBase[kNxVariants] = {0, 2, 12, 14};
-static_assert(kVariantsGather + kNxVariants == JKPM_STEP_VARIANTS, "variant count");
 StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl = false,
                         bool first = false);
 StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl = false,
@@ -379,10 +310,6 @@ StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl = false,
 // shapes that have none.  Defined in their own translation units.
 StepEntry pick_step_c64_ntm(int rb, bool wide, int nl, int k, int variant, bool dbl);
 StepEntry pick_step_c128_ntm(int rb, int nl, int k, int variant, bool dbl);
-// The NX twin of base variant 0, 2, 12 or 14 (fn is NULL for shapes that
-// have none).  Defined in cheb_nx_c64.hip and cheb_nx_c128.hip.
-StepEntry pick_step_nx_c64(int rb, bool wide, int nl, int k, int base, bool dbl);
-StepEntry pick_step_nx_c128(int rb, int nl, int k, int base, bool dbl);
 
 }  // namespace step
 }  // namespace jkpm

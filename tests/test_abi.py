@@ -48,7 +48,7 @@ def test_header_constants_match_the_wrapper():
         assert getattr(nat, name) == value, name
         checked += 1
     assert checked >= 15
-    assert nat.JKPM_STEP_VARIANTS == 52
+    assert nat.JKPM_STEP_VARIANTS == 48
 
 
 def test_symbols_are_c_linkage():

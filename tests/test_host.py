@@ -474,6 +474,31 @@ def test_tuned_times_short_steps_longer(tmp_path, monkeypatch):
     assert autotune.tuned(h, Vecs()) == (5, 4) and h.calls == []
 
 
+def test_finalist_neighbours_flip_one_choice():
+    """The tuner's finalist round times the sweep's winner against the
+    variants that differ from it in exactly one of: non-temporal matrix
+    loads (+-24), non-temporal stores (+-12 within a 24-block), non-temporal
+    stream loads (bit 1); the unroll bit and lane shape stay."""
+    from minijclsquant_amd import autotune
+
+    def fields(v):
+        return {"ntm": v >= 24, "ns": v % 24 >= 12, "nt": bool(v & 2), "unroll": v & 1,
+                "shape": (v % 12) >> 2}
+
+    assert autotune.N_VARIANTS == 48
+    for v in range(48):
+        nb = autotune._neighbours(v)
+        assert len(nb) == 3 and len(set(nb)) == 3
+        for u, flipped in zip(nb, ("ntm", "ns", "nt")):
+            assert 0 <= u < 48
+            a, b = fields(v), fields(u)
+            assert a[flipped] != b[flipped]
+            assert all(a[k] == b[k] for k in a if k != flipped), (v, u)
+    # XOR with 12 or 24 is not a one-choice flip everywhere
+    assert autotune._neighbours(20)[1] == 8 and (20 ^ 12) == 24
+    assert autotune._neighbours(38)[0] == 14 and (38 ^ 24) == 62
+
+
 def _min_image_bonds(S, ncx, ncy, thr):
     """Brute-force nearest neighbours of the 4-atom-cell flake on its torus:
     every pair, separation wrapped to the nearest periodic image (periods
