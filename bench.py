@@ -54,7 +54,8 @@ sys.path.insert(0, str(ROOT))
 METRIC = ("Chebyshev SpMV-steps/s (N·M·R/s) + % HBM roofline at DOS err "
           "≤1e-5 vs FP64")
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = ROOT / "profiles" / "cheb_step_c64_r16_honeycomb1024_pmc.md"
+PROFILES = ROOT / "profiles"
+TRAFFIC_FILE = PROFILES / "cheb_step_c64_r16_honeycomb1024_pmc.md"
 
 
 def log(msg):
@@ -73,17 +74,26 @@ def build_workload(L, M, R, seed):
     return A, r, B
 
 
-def traffic_from_profile(N, K, R, key="traffic_bytes_per_launch"):
-    """HBM bytes per launch from the committed counter summary, if it was
-    measured on this exact workload shape."""
-    if not TRAFFIC_FILE.exists():
+def profile_traffic(path, key, **shape):
+    """HBM bytes per launch (FETCH_SIZE and WRITE_SIZE, calibrated) from a
+    committed counter summary, if its `workload:` line names exactly this
+    shape (every given key=value matches), else None."""
+    if not path.exists():
         return None
-    txt = TRAFFIC_FILE.read_text()
+    txt = path.read_text()
     m = re.search(rf"^{key}:\s*([0-9.eE+]+)", txt, re.M)
-    w = re.search(r"workload:\s*N=(\d+)\s+K=(\d+)\s+R=(\d+)", txt)
-    if not m or not w or tuple(map(int, w.groups())) != (N, K, R):
+    w = re.search(r"^workload:(.*)$", txt, re.M)
+    if not m or not w:
+        return None
+    have = {k: int(v) for k, v in re.findall(r"(\w+)=(\d+)", w.group(1))}
+    if any(have.get(k) != int(v) for k, v in shape.items()):
         return None
     return float(m.group(1))
+
+
+def traffic_from_profile(N, K, R, key="traffic_bytes_per_launch"):
+    """The headline workload's traffic figure (see profile_traffic)."""
+    return profile_traffic(TRAFFIC_FILE, key, N=N, K=K, R=R)
 
 
 def copy_ceiling(torch, nat, nbytes=2 << 30, reps=5):
@@ -465,6 +475,9 @@ def bench_batch(args, torch, jcl, nat, _kpm):
     out["roofline"] = hbm_roofline(bytes_launch, launch_ms, n_prof, "cheb_step_kernel",
                                    whole_replay_GBs=round(ach, 1),
                                    whole_replay_frac=round(ach / HBM_PEAK_GBS, 4))
+    out["roofline"]["traffic"] = profile_traffic(
+        PROFILES / "cheb_step_c64_r8_batch64_square_trace.md", "traffic_bytes_per_launch",
+        N=N, K=K, R=R)
     out["mu_err_vs_c128"] = mu_err
     out["dos_err_vs_c128"] = dos_err
     out["accuracy_note"] = ("max over the 64 realisations and all n of |mu_c64 - mu_c128| / mu_0, "
@@ -571,7 +584,11 @@ def bench_config1(args, torch, jcl, nat, _kpm):
         bytes_launch, launch_ms, n_prof, "cheb_step_kernel",
         whole_step_GBs=round(ach, 1),
         note="the 23 MB working set is cache-resident (256 MB Infinity Cache) and each launch "
-             "is latency-bound, so this is not an HBM-bound figure")
+             "is latency-bound, so this is not an HBM-bound figure; traffic is what leaves "
+             "the L2, served by the Infinity Cache")
+    out["roofline"]["traffic"] = profile_traffic(
+        PROFILES / "cheb_step_c128_r4_square256_pmc.md", "traffic_bytes_per_launch",
+        N=N, K=K, R=R)
     if ref_s is not None:
         out["cpu_baseline"] = {"value": units / ref_s, "unit": "N*M*R/s", "cores": thr,
                                "kind": "reference", "nproc": nproc, "cpu_model": model,
@@ -681,6 +698,8 @@ def bench_grad(args, torch, jcl, nat, _kpm):
         bytes_launch, launch_ms, n_prof, "cheb_step_kernel (forward recursion)",
         forward_whole_GBs=round(bytes_launch * (M - 1) / (ms_f / 1e3) / 1e9, 1),
         note="the 160 MB step working set stays partly in the 256 MB Infinity Cache")
+    out["roofline"]["traffic"] = profile_traffic(
+        PROFILES / "grad_adjoint_square512_pmc.md", "traffic_bytes_per_launch", N=N, K=K, R=R)
     out["grad_vs_fd"] = {"rel_err": fd_err, "coordinates": int(len(coords)), "h": 1e-6,
                          "bar": 1e-4}
     # the same with sqrt(M) checkpoints and recomputation (small memory)
@@ -780,8 +799,11 @@ def bench_kubo(args, torch, jcl, nat, _kpm):
                                  "MFMA accumulate, FP64 combine, sigma on 512 energies",
                      "N": N, "M": M, "R": R})
     kstep_bytes = N * 4 * (8 + 8 + 4) + R * N * 8 * 7
+    kubo_pmc = PROFILES / "kubo_step_and_gemm_honeycomb512_pmc.md"
     out["roofline"] = {"bound": "mfma", "achieved": round(tf_gemm, 2), "peak": 157.3,
-                       "unit": "TFLOP/s", "frac": round(tf_gemm / 157.3, 4), "traffic": None,
+                       "unit": "TFLOP/s", "frac": round(tf_gemm / 157.3, 4),
+                       "traffic": profile_traffic(kubo_pmc, "gemm_traffic_bytes_per_launch",
+                                                  N=N, M=M, R=R),
                        "kernel": "kubo_gemm3_kernel", "mean_launch_us": round(gemm_ms * 1e3, 1),
                        "launches_timed": n_gemm,
                        "note": "complex flops 8 N M^2 R over the GEMM launch (three real "
@@ -790,6 +812,8 @@ def bench_kubo(args, torch, jcl, nat, _kpm):
                        "whole_step_frac": round(tf / 157.3, 4)}
     out["stack_step"] = hbm_roofline(kstep_bytes, kstep_ms, n_kstep, "kubo_step_kernel",
                                      steps_per_call=M - 1)
+    out["stack_step"]["traffic"] = profile_traffic(kubo_pmc, "step_traffic_bytes_per_launch",
+                                                   N=N, M=M, R=R)
     out["mu_nm_err_vs_fp64"] = mu_err
     out["sigma_err_vs_fp64"] = sig_err
     out["accuracy_note"] = ("max |mu_c64 - mu_fp64| / max |mu_fp64| and the same for the "

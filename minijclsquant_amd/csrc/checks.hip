@@ -19,8 +19,15 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// max that keeps a NaN from either side (fmax would drop it), so a matrix
+// with a NaN value reports a NaN asymmetry and is refused
+__device__ __forceinline__ double max_nan(double a, double b) {
+    return (b > a || b != b) ? b : a;
+}
+
 // out[0] = bits of max |a_ij - conj(a_ji)|, out[1] = bits of max |a_ij|.
-// Non-negative doubles order like their bit patterns as unsigned integers.
+// Non-negative doubles order like their bit patterns as unsigned integers,
+// and every NaN pattern orders above +inf, so the maxima keep a NaN.
 template <typename Real>
 __global__ void __launch_bounds__(kBlock) herm_kernel(const Real *__restrict__ val,
                                                       const int *__restrict__ col, int n, int k,
@@ -46,13 +53,13 @@ __global__ void __launch_bounds__(kBlock) herm_kernel(const Real *__restrict__ v
         }
         // a_ij - conj(a_ji)
         const double dr = ar - br, di = ai + bi;
-        err = fmax(err, sqrt(dr * dr + di * di));
+        err = max_nan(err, sqrt(dr * dr + di * di));
         const double vr = (double)val[2 * t], vi = (double)val[2 * t + 1];
-        mag = fmax(mag, sqrt(vr * vr + vi * vi));
+        mag = max_nan(mag, sqrt(vr * vr + vi * vi));
     }
     for (int off = 32; off > 0; off >>= 1) {
-        err = fmax(err, __shfl_xor(err, off));
-        mag = fmax(mag, __shfl_xor(mag, off));
+        err = max_nan(err, __shfl_xor(err, off));
+        mag = max_nan(mag, __shfl_xor(mag, off));
     }
     if ((threadIdx.x & 63) == 0) {
         atomicMax(&out[0], (unsigned long long)__double_as_longlong(err));

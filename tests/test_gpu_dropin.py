@@ -140,6 +140,21 @@ def test_hermiticity_exact_for_hermitian_inputs(torch_gpu, dtype):
         assert err == 0.0 and mag > 0 and h.is_hermitian()
 
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_hermiticity_reports_nan_values(torch_gpu, dtype):
+    """A NaN value makes the asymmetry NaN (not a maximum that skips it), so
+    the doubling form refuses the matrix instead of trusting it."""
+    H = jcl.square_lattice(7, W=2.0, seed=2)[0]
+    A = jcl.ell_matrix(H)
+    A.data[5] = np.nan
+    h = A.device(dtype, 1 / A.bounds[1])
+    err, mag = h.hermiticity()
+    assert np.isnan(err) and not h.is_hermitian()
+    r = jcl.random_vector_generator(A.shape[0], 1)
+    with pytest.raises(ValueError, match="Hermitian"):
+        jcl.rec_A_tab(16, A.modifier(jcl.modifier_bounds), r, doubling=True)
+
+
 def test_hermiticity_guard_on_reference_graphene(torch_gpu):
     """H_graphene 'ELL' (jclsquant/hams.py:160-162) carries H += 1e-6 (Dx + Dy),
     antisymmetric in the bond: |A_ij - conj(A_ji)| = 2e-6 |dx_ij + dy_ij| / B."""

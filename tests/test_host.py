@@ -1,11 +1,15 @@
 """Host logic of the package (CPU only): ELL conversion, bounds, lattice
 builders, modifiers, kernels -- checked against the oracle and closed forms."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
 
 import minijclsquant_amd as jcl
 from oracle import oracle as O
+
+ROOT = Path(__file__).resolve().parent.parent
 
 
 def ragged_csr(N=80, seed=0, kmax=7):
@@ -618,3 +622,42 @@ def test_extra_funs_match_oracle():
         assert bm.shape == (M, M, 2 * M)
         assert np.allclose(bm, om, rtol=1e-12, atol=1e-300)
         assert np.allclose(bn, on, rtol=1e-12, atol=1e-300)
+
+
+def test_bench_reads_traffic_from_committed_profiles():
+    """bench.py fills every roofline's `traffic` from the committed counter
+    summaries (FETCH_SIZE x 2 + WRITE_SIZE per dispatch), and only for the
+    workload shape each summary names."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("bench_mod", ROOT / "bench.py")
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    P = bench.PROFILES
+    got = {
+        "headline": bench.traffic_from_profile(2097152, 4, 16),
+        "headline_dbl": bench.traffic_from_profile(2097152, 4, 16,
+                                                   "doubling_traffic_bytes_per_launch"),
+        "batch": bench.profile_traffic(P / "cheb_step_c64_r8_batch64_square_trace.md",
+                                       "traffic_bytes_per_launch", N=24054938, K=5, R=8),
+        "config1": bench.profile_traffic(P / "cheb_step_c128_r4_square256_pmc.md",
+                                         "traffic_bytes_per_launch", N=65536, K=5, R=4),
+        "grad": bench.profile_traffic(P / "grad_adjoint_square512_pmc.md",
+                                      "traffic_bytes_per_launch", N=262144, K=5, R=8),
+        "kubo_gemm": bench.profile_traffic(P / "kubo_step_and_gemm_honeycomb512_pmc.md",
+                                           "gemm_traffic_bytes_per_launch", N=524288, M=512, R=4),
+        "kubo_step": bench.profile_traffic(P / "kubo_step_and_gemm_honeycomb512_pmc.md",
+                                           "step_traffic_bytes_per_launch", N=524288, M=512, R=4),
+    }
+    # algorithmic bytes per launch of each kernel; counters sit 0-7 % above
+    alg = {"headline": 2097152 * (4 * 12 + 16 * 32), "headline_dbl": 2097152 * (4 * 12 + 16 * 24),
+           "batch": 24054938 * (5 * 12 + 8 * 32), "config1": 65536 * (5 * 20 + 4 * 64),
+           "grad": 262144 * (5 * 20 + 8 * 64), "kubo_gemm": 2 * 512 * 2097152 * 8,
+           "kubo_step": 524288 * 4 * 20 + 4 * 524288 * 8 * 7}
+    for k, v in got.items():
+        assert v is not None, k
+        assert 1.0 <= v / alg[k] <= 1.07, (k, v / alg[k])
+    # another shape gets no figure
+    assert bench.traffic_from_profile(2097152, 4, 8) is None
+    assert bench.profile_traffic(P / "cheb_step_c128_r4_square256_pmc.md",
+                                 "traffic_bytes_per_launch", N=65536, K=5, R=8) is None
