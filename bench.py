@@ -267,29 +267,38 @@ def wave_leg(torch, nat, h, vr, N, K, M, R, s_v, args, units, es):
 
 
 def plain_c128(torch, nat, _kpm, A, r, B, M, value, value_dbl, tune, reps=3):
-    """The reference's structure on this GPU: complex128 recursion, one
-    vector at a time (R = 1), one moment per step, with its own tuned launch
-    shape.  Timed with HIP events on the current stream over `reps`
-    recursions."""
+    """The reference's structure on this GPU: complex128 recursion on
+    complex128 matrix values (as the reference stores them), one vector at a
+    time (R = 1), one moment per step, with its own tuned launch shape.
+    Timed with HIP events on the current stream over `reps` recursions; the
+    same path on the packed real copy of the (real) values is timed beside
+    it."""
     N = int(A.shape[0])
     h = A.device("c128", 1.0 / B)
     v1 = _kpm.as_batch(r[0], N, "c128")
-    launch = None
-    if tune:
-        from minijclsquant_amd import autotune
-
-        launch = autotune.tuned(h, v1)
     ws = torch.empty(h.workspace_bytes(1, 1), dtype=torch.uint8, device="cuda")
     mu = torch.empty((1, M, 1), dtype=torch.complex128, device="cuda")
-    h.moments(v1, M, workspace=ws, out=mu)
-    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0.record()
-    for _ in range(reps):
+
+    def timed(real):
+        h.use_real_values(real)
+        launch = None
+        if tune:
+            from minijclsquant_amd import autotune
+
+            launch = autotune.tuned(h, v1)
         h.moments(v1, M, workspace=ws, out=mu)
-    t1.record()
-    t1.synchronize()
-    ms = t0.elapsed_time(t1) / reps
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(reps):
+            h.moments(v1, M, workspace=ws, out=mu)
+        t1.record()
+        t1.synchronize()
+        return t0.elapsed_time(t1) / reps, launch
+
+    ms_r, launch_r = timed(True) if h.is_real() else (None, None)
+    ms, launch = timed(False)
     v = N * M / (ms / 1e3)
+    v_r = None if ms_r is None else N * M / (ms_r / 1e3)
     # the reference-compatible host-pointer entry (rec_A_tab): upload of the
     # matrix and vector over PCIe, the recursion, download of the moments
     Ht = np.ascontiguousarray(A.data * (1.0 / B))
@@ -303,10 +312,16 @@ def plain_c128(torch, nat, _kpm, A, r, B, M, value, value_dbl, tune, reps=3):
         "host_entry_ms": host_s * 1e3,
         "unit": "N*M*R/s",
         "ms_per_recursion": ms,
-        "sample": f"same H, R=1 vector, M={M} moments, complex128, {reps} recursions",
+        "sample": f"same H, R=1 vector, M={M} moments, complex128 vectors and matrix values, "
+                  f"{reps} recursions",
         "launch": launch,
         "headline_over_plain": value / v,
         "doubling_over_plain": None if value_dbl is None else value_dbl / v,
+        "real_values": None if v_r is None else {
+            "value": v_r, "ms_per_recursion": ms_r, "launch": launch_r,
+            "note": "the same path with the step kernels on the packed real copy of the values",
+            "headline_over_it": value / v_r,
+            "doubling_over_it": None if value_dbl is None else value_dbl / v_r},
     }
 
 
@@ -439,8 +454,9 @@ def bench_batch(args, torch, jcl, nat, _kpm):
     ms_d = event_ms(torch, dbl, args.steps)
     diff = float((mu_d - bk.mu).abs().max() / bk.mu[:, 0].abs().max())
     units = N * M * R
-    bytes_launch = N * K * 12 + R * N * 4 * 8
-    bytes_d = N * K * 12 + R * N * 3 * 8
+    mv = 4 if bk.H.is_real() else 8  # matrix value bytes per slot (packed real copy)
+    bytes_launch = N * K * (mv + 4) + R * N * 4 * 8
+    bytes_d = N * K * (mv + 4) + R * N * 3 * 8
     ach = bytes_launch * (M - 1) / (ms / 1e3) / 1e9
     ach_d = bytes_d * max(1, M // 2) / (ms_d / 1e3) / 1e9
     # the step kernel alone: event-timed runs of launches of one eager pass
@@ -578,7 +594,8 @@ def bench_config1(args, torch, jcl, nat, _kpm):
                                  "seed 11), K = 5, B = (4 + W/2) * 1.01, M = 512, R = 4 phase "
                                  "vectors (seeds 0..3), Jackson kernel, DOS on 1024 energies",
                      "N": N, "K": K, "M": M, "R": R})
-    bytes_launch = N * K * 20 + R * N * 4 * 16
+    mv = 8 if h.is_real() else 16  # matrix value bytes per slot (packed real copy)
+    bytes_launch = N * K * (mv + 4) + R * N * 4 * 16
     ach = bytes_launch * (M - 1) / (ms / 1e3) / 1e9
     out["roofline"] = hbm_roofline(
         bytes_launch, launch_ms, n_prof, "cheb_step_kernel",
@@ -690,7 +707,10 @@ def bench_grad(args, torch, jcl, nat, _kpm):
                                  "complex128, loss = sum_E w(E) DOS(E) on 256 energies; "
                                  "gradients for all onsite energies and bonds",
                      "N": N, "K": K, "M": M, "R": R, "params": int(N + pairs.shape[0])})
-    bytes_launch = N * K * 20 + R * N * 4 * 16
+    with torch.no_grad():  # real onsite energies and hoppings: the packed real copy
+        real = bool((lattice_values(eps, hop, slot_param).imag == 0).all())
+    mv = 8 if real else 16
+    bytes_launch = N * K * (mv + 4) + R * N * 4 * 16
     out["forward_ms"] = ms_f
     out["backward_ms"] = ms_fb - ms_f
     out["backward_over_forward"] = (ms_fb - ms_f) / ms_f
@@ -997,7 +1017,8 @@ def main():
     units = N * M * R
     value = units / (ms_per_step / 1e3)
     s_v = 8 if args.dtype == "c64" else 16
-    bytes_launch = N * K * (s_v + 4) + R * N * 4 * s_v
+    mv = s_v // 2 if h.is_real() else s_v  # matrix value bytes per slot (packed real copy)
+    bytes_launch = N * K * (mv + 4) + R * N * 4 * s_v
     achieved = bytes_launch / (launch_ms / 1e3) / 1e9
     roofline = {
         "bound": "hbm",
@@ -1008,6 +1029,7 @@ def main():
         "traffic": traffic_from_profile(N, K, R),
         "kernel": "cheb_step_kernel",
         "algorithmic_bytes_per_launch": bytes_launch,
+        "matrix_value_bytes_per_slot": mv,
         "mean_launch_us": round(launch_ms * 1e3, 2),
         "launches_timed": n_prof,
         "copy_kernel_GBs": round(copy_gbs, 1),
@@ -1032,7 +1054,7 @@ def main():
         torch.cuda.synchronize()
         ms_d, launch_d, n_d, dos_d = timed(True)
         log(f"doubling form: {ms_d:.2f} ms per step")
-        bytes_d = N * K * (s_v + 4) + R * N * 3 * s_v
+        bytes_d = N * K * (mv + 4) + R * N * 3 * s_v
         ach_d = bytes_d / (launch_d / 1e3) / 1e9
         dbl = {
             "value": units / (ms_d / 1e3),

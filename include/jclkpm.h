@@ -124,6 +124,22 @@ int jkpm_ell_set_values(jkpm_ell *h, const double *data, double scale,
 #define JKPM_HERMITIAN_RTOL 1e-10
 int jkpm_ell_hermiticity(const jkpm_ell *h, double *abs_err, double *max_abs, void *stream);
 
+/* *is_real = 1 when every stored imaginary part of the handle's values is
+ * exactly 0 (checked on the device at each value upload; a Peierls update
+ * makes the values complex).  The Chebyshev step kernels of such a matrix
+ * read a packed copy of the real parts, 4 (complex64) or 8 (complex128)
+ * bytes per slot instead of 8 or 16; results are those of the complex
+ * arithmetic with zero imaginary parts.  JKPM_REAL_VALUES=0 in the
+ * environment turns the check and the copy off (every handle then reports 0
+ * and runs on its complex values).  No reference
+ * counterpart (the reference stores complex128 values always,
+ * jclsquant/ell_matrix.py:187-200). */
+int jkpm_ell_is_real(const jkpm_ell *h, int *is_real);
+/* on = 0: the step kernels of this handle read its complex values even when
+ * they are all real (the complex128 storage of the reference, or an A/B
+ * comparison); on = 1 (the default) lets them read the packed real copy. */
+int jkpm_ell_use_real_values(jkpm_ell *h, int on);
+
 /* Non-finite entries (NaN, +-Inf) among the n doubles of device array x
  * (e.g. the 2*M*R doubles of a moment array): *first_bad = the smallest
  * such index or -1, *count = how many.  Synchronises `stream`; refused under

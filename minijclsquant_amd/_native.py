@@ -60,6 +60,8 @@ SIGNATURES = {
     "jkpm_ell_set_values": (_int, [_vp, _vp, _dbl, _int, _vp]),
     "jkpm_ell_get_values": (_int, [_vp, _vp, _int, _vp]),
     "jkpm_ell_hermiticity": (_int, [_vp, _dp, _dp, _vp]),
+    "jkpm_ell_is_real": (_int, [_vp, _ip]),
+    "jkpm_ell_use_real_values": (_int, [_vp, _int]),
     "jkpm_check_finite": (_int, [_vp, _c.c_longlong, _c.POINTER(_c.c_longlong),
                                  _c.POINTER(_c.c_longlong), _vp]),
     "jkpm_zvec": (_int, [_int, _c.c_longlong, _dp, _dp, _vp, _vp, _vp, _vp, _vp]),
@@ -475,6 +477,19 @@ class DeviceELL:
         check(lib().jkpm_ell_hermiticity(self._h, _c.byref(err), _c.byref(mag),
                                          stream_ptr(stream)))
         return float(err.value), float(mag.value)
+
+    def is_real(self) -> bool:
+        """True when every stored value has a zero imaginary part; the step
+        kernels then read a packed real copy (jkpm_ell_is_real)."""
+        v = _c.c_int()
+        check(lib().jkpm_ell_is_real(self._h, _c.byref(v)))
+        return bool(v.value)
+
+    def use_real_values(self, on: bool = True) -> None:
+        """on=False: the step kernels read the complex values even when they
+        are all real (jkpm_ell_use_real_values)."""
+        check(lib().jkpm_ell_use_real_values(self._h, 1 if on else 0))
+        self._use_real = bool(on)
 
     def is_hermitian(self, stream=None) -> bool:
         """True when max |a_ij - conj(a_ji)| <= HERMITIAN_RTOL * max |a_ij|, the

@@ -40,7 +40,9 @@ def _key(h: "nat.DeviceELL", R: int, doubling: bool) -> str:
     p = torch.cuda.get_device_properties(torch.cuda.current_device())
     dev = f"{getattr(p, 'gcnArchName', p.name)}|cu={p.multi_processor_count}"
     form = "|doubling" if doubling else ""
-    return f"{dev}|N={h.n}|K={h.k}|vtype={h.vtype}|R={R}{form}"
+    # the step kernels on the packed real values are other kernels
+    rv = "|real" if h.is_real() and getattr(h, "_use_real", True) else ""
+    return f"{dev}|N={h.n}|K={h.k}|vtype={h.vtype}|R={R}{rv}{form}"
 
 
 def _load():

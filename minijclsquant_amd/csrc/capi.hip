@@ -697,6 +697,7 @@ int jkpm_ell_create(const double *data, const int *indices, int n_rows, int len_
         if (idx_dev && !src_on_device) hipFree(idx_dev);
         if (bad) hipFree(bad);
         if (H.val) hipFree(H.val);
+        if (H.rval) hipFree(H.rval);
         if (H.col) hipFree(H.col);
         delete h;
         return code;
@@ -732,6 +733,8 @@ int jkpm_ell_create(const double *data, const int *indices, int n_rows, int len_
     }
     rc = upload_values(&H, data, scale, idx_dev, src_on_device, s);
     if (rc) return fail(rc);
+    if ((rc = update_real_values(&H, s))) return fail(rc);
+    h->values_real = H.real_values;
     if ((e = hipStreamSynchronize(s)) != hipSuccess)
         return fail(hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__));
     if (!src_on_device) hipFree(idx_dev);
@@ -787,6 +790,7 @@ int jkpm_ell_create_csr(const int *indptr, const int *indices, const double *dat
         if (stats) hipFree(stats);
         if (h) {
             if (h->e.val) hipFree(h->e.val);
+            if (h->e.rval) hipFree(h->e.rval);
             if (h->e.col) hipFree(h->e.col);
             delete h;
         }
@@ -854,6 +858,8 @@ int jkpm_ell_create_csr(const int *indptr, const int *indices, const double *dat
         return cleanup(hip_fail(e, "hipMalloc(cols)", __FILE__, __LINE__));
     rc = launch_csr_to_ell(d_ptr, d_idx, d_val, scale, &H, s);
     if (rc) return cleanup(rc);
+    if ((rc = update_real_values(&H, s))) return cleanup(rc);
+    h->values_real = H.real_values;
     if ((e = hipStreamSynchronize(s)) != hipSuccess)
         return cleanup(hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__));
     jkpm_ell *done = h;
@@ -915,12 +921,15 @@ int jkpm_ell_create_like(const jkpm_ell *pattern, const double *data, double sca
         return hip_fail(e, "hipMalloc(values)", __FILE__, __LINE__);
     }
     rc = upload_values(&H, data, scale, nullptr, src_on_device, (hipStream_t)stream);
+    if (!rc) rc = update_real_values(&H, (hipStream_t)stream);
+    h->values_real = H.real_values;
     if (!rc) {
         e = hipStreamSynchronize((hipStream_t)stream);
         if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
     }
     if (rc) {
         hipFree(H.val);
+        if (H.rval) hipFree(H.rval);
         delete h;
         return rc;
     }
@@ -932,6 +941,7 @@ int jkpm_ell_destroy(jkpm_ell *h) {
     clear_error();
     if (!h) return JKPM_OK;
     hipFree(h->e.val);
+    if (h->e.rval) hipFree(h->e.rval);
     delete h;  // releases this handle's share of the column array
     return JKPM_OK;
 }
@@ -951,9 +961,28 @@ int jkpm_ell_set_values(jkpm_ell *h, const double *data, double scale, int src_o
     if (!h || !data) return arg_error("handle/data is NULL");
     // Values only: the slot-major column array stays as uploaded.
     h->herm_err = h->herm_max = -1.0;
+    h->e.real_values = h->values_real = false;
     int rc = upload_values(&h->e, data, scale, nullptr, src_on_device, (hipStream_t)stream);
     if (rc) return rc;
+    if ((rc = update_real_values(&h->e, (hipStream_t)stream))) return rc;
+    h->values_real = h->e.real_values;
+    h->e.real_values = h->values_real && h->use_real;
     JKPM_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return JKPM_OK;
+}
+
+int jkpm_ell_is_real(const jkpm_ell *h, int *is_real) {
+    clear_error();
+    if (!h || !is_real) return arg_error("NULL argument");
+    *is_real = h->values_real ? 1 : 0;
+    return JKPM_OK;
+}
+
+int jkpm_ell_use_real_values(jkpm_ell *h, int on) {
+    clear_error();
+    if (!h) return arg_error("handle is NULL");
+    h->use_real = on != 0;
+    h->e.real_values = h->values_real && h->use_real && h->e.rval;
     return JKPM_OK;
 }
 
@@ -1003,6 +1032,7 @@ int jkpm_ell_peierls(jkpm_ell *dst, const jkpm_ell *src, const jkpm_ell *bonds, 
         return arg_error("dst, src and bonds must share one slot pattern (jkpm_ell_create_like)");
     if (dst->e.vtype != src->e.vtype) return arg_error("dst and src must have one value type");
     dst->herm_err = dst->herm_max = -1.0;
+    dst->e.real_values = dst->values_real = false;  // Peierls phases make the values complex
     int rc = launch_peierls(src->e, bonds->e, &dst->e, make_double2(a_re, a_im),
                             make_double2(ax_re, ax_im), make_double2(ay_re, ay_im),
                             (hipStream_t)stream);

@@ -227,12 +227,15 @@ int launch_step(const StepArgs &a, hipStream_t s) {
     const int nl = (a.vl1 || a.doubling) ? 2 : 1;
     const size_t cb = cplx_bytes(H.vtype);
     const bool wide = (a.ld % 2 == 0) && (a.r0 % 2 == 0);
+    // a real matrix: the kernels that read its packed real values, where the
+    // shape has them
+    const bool rv = H.real_values && H.rval;
     const step::StepEntry e =
         H.vtype == JKPM_C64
             ? step::pick_step_c64(a.rb, wide, nl, H.k, a.doubling ? H.dvariant : H.variant,
-                                  a.doubling, a.first)
+                                  a.doubling, a.first, rv)
             : step::pick_step_c128(a.rb, nl, H.k, a.doubling ? H.dvariant : H.variant, a.doubling,
-                                   a.first);
+                                   a.first, rv);
     if (!e.fn) return arg_error("unsupported vector group size " + std::to_string(a.rb));
     const int rpb = kWaves * (64 / e.lpr) * e.unr;
     long rpc = ((long)H.n + a.nchunks - 1) / a.nchunks;
@@ -240,7 +243,7 @@ int launch_step(const StepArgs &a, hipStream_t s) {
     const size_t off = (size_t)a.r0 * cb;
     // Arguments in the kernel's parameter order; every pointer is passed as
     // an untyped pointer of the same size as the kernel's typed one.
-    const void *p_val = H.val;
+    const void *p_val = e.rv ? H.rval : H.val;
     const int *p_col = H.col;
     int p_n = H.n, p_k = H.k;
     const void *p_xcur = (const char *)a.xcur + off;

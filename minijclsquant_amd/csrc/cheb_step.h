@@ -2,6 +2,8 @@
 // units cheb_c64.hip and cheb_c128.hip (see cheb.hip for the design notes).
 #pragma once
 
+#include <type_traits>
+
 #include "common.h"
 
 namespace jkpm {
@@ -41,8 +43,11 @@ __device__ __forceinline__ V load_stream(const V *p) {
 // but of the doubling-form kernels only the base kernel of each shape has
 // them, and the first step of a doubling recursion runs that one.  A kernel
 // without F0 treats first as 0.
+// RV (real values): `val` holds one real per slot (a matrix whose imaginary
+// parts are all 0, packed by the handle), so the matrix stream is 4 (8)
+// bytes per slot instead of 8 (16) and each product takes two multiplies.
 template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool DBL = false,
-          bool NS = false, bool NTM = false, bool F0 = false>
+          bool NS = false, bool NTM = false, bool F0 = false, bool RV = false>
 __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const Real *__restrict__ val, const int *__restrict__ col, int n, int k_rt,
     const Real *__restrict__ xcur, const Real *xprev, Real *xout,
@@ -52,9 +57,26 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const bool first = F0 && first_arg;
     using V = vec_t<Real, 2 * VPL>;
     using C2 = vec_t<Real, 2>;
+    using HV = std::conditional_t<RV, Real, C2>;  // one stored matrix value
     constexpr int RPW = 64 / LPR;
     constexpr int RB = VPL * LPR;
     const int K = KT > 0 ? KT : k_rt;
+    auto load_h = [&](long e) -> HV {
+        if constexpr (RV)
+            return load_stream<NTM>(val + e);
+        else
+            return load_stream<NTM>(reinterpret_cast<const C2 *>(val + 2 * e));
+    };
+    // s + h x for one vector of a lane (complex or real h)
+    auto mac = [](const HV &h, Real xr, Real xi, Real &sr_, Real &si_) {
+        if constexpr (RV) {
+            sr_ += h * xr;
+            si_ += h * xi;
+        } else {
+            sr_ += h.x * xr - h.y * xi;
+            si_ += h.x * xi + h.y * xr;
+        }
+    };
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -111,14 +133,14 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
 
         if constexpr (KT > 0) {
             int j[UNR][KT];
-            C2 h[UNR][KT];
+            HV h[UNR][KT];
 #pragma unroll
             for (int u = 0; u < UNR; ++u)
 #pragma unroll
                 for (int s = 0; s < KT; ++s) {
                     const long e = (long)s * n + rows[u];
                     j[u][s] = load_stream<NTM>(col + e);
-                    h[u][s] = load_stream<NTM>(reinterpret_cast<const C2 *>(val + 2 * e));
+                    h[u][s] = load_h(e);
                 }
             V x[UNR][KT];
 #pragma unroll
@@ -141,23 +163,18 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
 #pragma unroll
                 for (int s = 0; s < KT; ++s)
 #pragma unroll
-                    for (int v = 0; v < VPL; ++v) {
-                        sr[u][v] += h[u][s].x * x[u][s][2 * v] - h[u][s].y * x[u][s][2 * v + 1];
-                        si[u][v] += h[u][s].x * x[u][s][2 * v + 1] + h[u][s].y * x[u][s][2 * v];
-                    }
+                    for (int v = 0; v < VPL; ++v)
+                        mac(h[u][s], x[u][s][2 * v], x[u][s][2 * v + 1], sr[u][v], si[u][v]);
         } else {
 #pragma unroll
             for (int u = 0; u < UNR; ++u)
                 for (int s = 0; s < K; ++s) {
                     const long e = (long)s * n + rows[u];
                     const int j = load_stream<NTM>(col + e);
-                    const C2 h = load_stream<NTM>(reinterpret_cast<const C2 *>(val + 2 * e));
+                    const HV h = load_h(e);
                     const V x = *reinterpret_cast<const V *>(xcur + (long)j * ld * 2 + voff);
 #pragma unroll
-                    for (int v = 0; v < VPL; ++v) {
-                        sr[u][v] += h.x * x[2 * v] - h.y * x[2 * v + 1];
-                        si[u][v] += h.x * x[2 * v + 1] + h.y * x[2 * v];
-                    }
+                    for (int v = 0; v < VPL; ++v) mac(h, x[2 * v], x[2 * v + 1], sr[u][v], si[u][v]);
                 }
         }
 
@@ -292,6 +309,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
 struct StepEntry {
     const void *fn;
     int vpl, lpr, unr;
+    bool rv = false;  // the kernel reads the handle's packed real values
 };
 
 // variant v = unroll | non-temporal << 1 | lane-shape << 2 (0..11), plus 12
@@ -302,10 +320,20 @@ struct StepEntry {
 // dbl selects the doubling form (two partial sets, see DBL above); first
 // selects the base kernel of the shape with the first-step accumulators (F0).
 // variant 0..47: 24..47 are 0..23 with non-temporal matrix loads (NTM).
+// rv: the handle's values are real; a kernel reading the packed real copy
+// is returned where one exists (entry.rv tells which was picked).
 StepEntry pick_step_c64(int rb, bool wide, int nl, int k, int variant, bool dbl = false,
-                        bool first = false);
+                        bool first = false, bool rv = false);
 StepEntry pick_step_c128(int rb, int nl, int k, int variant, bool dbl = false,
-                         bool first = false);
+                         bool first = false, bool rv = false);
+// The real-value kernels of the tuned shapes (variants 0..23, widths 4 and 5;
+// fn is NULL for others)  Defined in cheb_c64_rv.hip and cheb_c128_rv.hip.
+StepEntry pick_step_c64_rv(int rb, bool wide, int nl, int k, int variant, bool dbl, bool first);
+StepEntry pick_step_c128_rv(int rb, int nl, int k, int variant, bool dbl, bool first);
+// ... and of variants 24..47 (variant - 24 given); in cheb_c64_rv_ntm.hip and
+// cheb_c128_rv_ntm.hip.
+StepEntry pick_step_c64_rv_ntm(int rb, bool wide, int nl, int k, int variant, bool dbl);
+StepEntry pick_step_c128_rv_ntm(int rb, int nl, int k, int variant, bool dbl);
 // The NTM kernels of the tuned shapes (variant 0..23 here); fn is NULL for
 // shapes that have none.  Defined in their own translation units.
 StepEntry pick_step_c64_ntm(int rb, bool wide, int nl, int k, int variant, bool dbl);

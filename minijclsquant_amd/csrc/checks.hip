@@ -11,7 +11,9 @@
 // Cost O(N K^2) reads, one launch and one synchronisation per value change.
 #include "common.h"
 
+#include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace jkpm {
@@ -89,7 +91,60 @@ __global__ void __launch_bounds__(kBlock) nonfinite_kernel(const double *__restr
     }
 }
 
+// rval[e] = Re val[e]; *cplx != 0 when some Im val[e] != 0 (NaN included).
+template <typename Real>
+__global__ void __launch_bounds__(kBlock) real_pack_kernel(const Real *__restrict__ val, long tot,
+                                                           Real *__restrict__ rval,
+                                                           unsigned *__restrict__ cplx) {
+    bool any = false;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+         e += (long)gridDim.x * blockDim.x) {
+        rval[e] = val[2 * e];
+        any |= !(val[2 * e + 1] == Real(0));
+    }
+    if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(cplx, 1u);
+}
+
 }  // namespace
+
+int update_real_values(Ell *H, hipStream_t s) {
+    static const bool enabled = [] {
+        const char *e = getenv("JKPM_REAL_VALUES");
+        return !(e && atoi(e) == 0);
+    }();
+    H->real_values = false;
+    if (!enabled) return JKPM_OK;
+    const long tot = (long)H->n * H->k;
+    const size_t rb = H->vtype == JKPM_C64 ? sizeof(float) : sizeof(double);
+    if (!H->rval) JKPM_HIP(hipMalloc(&H->rval, (size_t)tot * rb));
+    unsigned *d = nullptr;
+    JKPM_HIP(hipMalloc(&d, sizeof(unsigned)));
+    unsigned host = 0;
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned), s);
+    if (e == hipSuccess) {
+        DeviceInfo dev;
+        int rc = device_info(&dev);
+        if (rc) {
+            hipFree(d);
+            return rc;
+        }
+        const long want = (tot + kBlock - 1) / kBlock;
+        const int grid = (int)std::max(1L, std::min(want, (long)dev.cu_count * 8));
+        if (H->vtype == JKPM_C64)
+            hipLaunchKernelGGL(real_pack_kernel<float>, dim3(grid), dim3(kBlock), 0, s,
+                               (const float *)H->val, tot, (float *)H->rval, d);
+        else
+            hipLaunchKernelGGL(real_pack_kernel<double>, dim3(grid), dim3(kBlock), 0, s,
+                               (const double *)H->val, tot, (double *)H->rval, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&host, d, sizeof(host), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "real-value check", __FILE__, __LINE__);
+    H->real_values = host == 0;
+    return JKPM_OK;
+}
 
 int count_nonfinite(const double *x, long n, long long *first_bad, long long *count,
                     hipStream_t s) {

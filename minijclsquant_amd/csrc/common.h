@@ -68,6 +68,11 @@ struct Ell {
     void *val = nullptr;   // [k][n] complex of vtype
     int *col = nullptr;    // [k][n]
     int device = 0;
+    // Real parts of val, [k][n] reals of vtype's precision, owned by the
+    // handle; valid while real_values (every stored imaginary part is exactly
+    // 0, checked at each value upload), when the step kernels read these.
+    void *rval = nullptr;
+    bool real_values = false;
 };
 
 // Bytes of one complex element of a value type.
@@ -269,6 +274,11 @@ int count_nonfinite(const double *x, long n, long long *first_bad, long long *co
 // JKPM_FAULT=alloc in the environment: handle creation reports an
 // allocation failure (a test hook for the callers' error paths).
 bool inject_alloc_failure();
+// After a value upload: packs the real parts of H's values into H->rval
+// (allocated on first use) and sets H->real_values when every imaginary part
+// is exactly 0.  Synchronises s.  JKPM_REAL_VALUES=0 keeps every handle on
+// the complex values.
+int update_real_values(Ell *H, hipStream_t s);
 
 }  // namespace jkpm
 
@@ -281,6 +291,10 @@ struct jkpm_ell {
     // Hermiticity of the stored values (checks.hip): max |a_ij - conj(a_ji)|
     // and max |a_ij|, or -1 while unknown (reset whenever the values change).
     mutable double herm_err = -1.0, herm_max = -1.0;
+    // values_real: every stored imaginary part is 0 (checked at each value
+    // upload); use_real: the caller lets the step kernels read the packed
+    // real copy (jkpm_ell_use_real_values); e.real_values = both.
+    bool values_real = false, use_real = true;
 };
 
 // A tile plan of a handle: rows renumbered into patches, the matrix in that
