@@ -1032,7 +1032,12 @@ int jkpm_ell_peierls(jkpm_ell *dst, const jkpm_ell *src, const jkpm_ell *bonds, 
         return arg_error("dst, src and bonds must share one slot pattern (jkpm_ell_create_like)");
     if (dst->e.vtype != src->e.vtype) return arg_error("dst and src must have one value type");
     dst->herm_err = dst->herm_max = -1.0;
-    dst->e.real_values = dst->values_real = false;  // Peierls phases make the values complex
+    // Peierls phases make the values complex (see poison_real_copy)
+    if (dst->values_real) {
+        int rc0 = poison_real_copy(&dst->e, (hipStream_t)stream);
+        if (rc0) return rc0;
+    }
+    dst->e.real_values = dst->values_real = false;
     int rc = launch_peierls(src->e, bonds->e, &dst->e, make_double2(a_re, a_im),
                             make_double2(ax_re, ax_im), make_double2(ay_re, ay_im),
                             (hipStream_t)stream);

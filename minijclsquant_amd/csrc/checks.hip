@@ -105,7 +105,32 @@ __global__ void __launch_bounds__(kBlock) real_pack_kernel(const Real *__restric
     if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(cplx, 1u);
 }
 
+template <typename Real>
+__global__ void __launch_bounds__(kBlock) fill_nan_kernel(Real *__restrict__ p, long n) {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x)
+        p[e] = __builtin_nan("");
+}
+
 }  // namespace
+
+int poison_real_copy(Ell *H, hipStream_t s) {
+    if (!H->rval) return JKPM_OK;
+    const long tot = (long)H->n * H->k;
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const long want = (tot + kBlock - 1) / kBlock;
+    const int grid = (int)std::max(1L, std::min(want, (long)dev.cu_count * 8));
+    if (H->vtype == JKPM_C64)
+        hipLaunchKernelGGL(fill_nan_kernel<float>, dim3(grid), dim3(kBlock), 0, s,
+                           (float *)H->rval, tot);
+    else
+        hipLaunchKernelGGL(fill_nan_kernel<double>, dim3(grid), dim3(kBlock), 0, s,
+                           (double *)H->rval, tot);
+    JKPM_CHECK_LAUNCH("fill_nan_kernel");
+    return JKPM_OK;
+}
 
 int update_real_values(Ell *H, hipStream_t s) {
     static const bool enabled = [] {
@@ -143,6 +168,9 @@ int update_real_values(Ell *H, hipStream_t s) {
     hipFree(d);
     if (e != hipSuccess) return hip_fail(e, "real-value check", __FILE__, __LINE__);
     H->real_values = host == 0;
+    // complex values: a graph captured while they were real would replay the
+    // real-value kernels on this copy; NaN makes that visible
+    if (!H->real_values) return poison_real_copy(H, s);
     return JKPM_OK;
 }
 

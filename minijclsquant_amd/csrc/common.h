@@ -279,6 +279,10 @@ bool inject_alloc_failure();
 // is exactly 0.  Synchronises s.  JKPM_REAL_VALUES=0 keeps every handle on
 // the complex values.
 int update_real_values(Ell *H, hipStream_t s);
+// Fills H->rval with NaN (asynchronous on s): the values stopped being real,
+// and a HIP graph captured while they were replays the real-value kernels,
+// which then yield NaN moments instead of moments of the stale real parts.
+int poison_real_copy(Ell *H, hipStream_t s);
 
 }  // namespace jkpm
 

@@ -100,3 +100,37 @@ def test_use_real_values_switch(torch_gpu, dtype, R, tol):
         b = nat.moments_ex(h, None, vr, 200, dbl).cpu().numpy()
         assert np.max(np.abs(a - b)) <= tol * N, dbl
     h.use_real_values(True)
+
+
+def test_stale_graph_after_values_turn_complex(torch_gpu):
+    """A graph captured while the batch's values were real replays the
+    real-value kernels; once set_values makes them complex, the packed copy
+    holds NaN, so such a replay yields NaN moments (never the moments of
+    stale real parts).  A new capture runs the complex kernels and equals
+    the eager run."""
+    torch = torch_gpu
+    blocks, vecs = [], []
+    for g, L in enumerate((9, 14)):
+        d, i = jcl.square_lattice_ell(L, t=-1.0, W=1.0, seed=20 + g)
+        blocks.append((d, i, L * L, 5, 1.0 / 4.6))
+        vecs.append(np.stack([jcl.random_vector_generator(L * L, 50 + 10 * g + s) for s in range(4)]))
+    bk = jcl.BatchKPM(blocks, vecs, 40, "c128")
+    assert bk.H.is_real()
+    eager = bk.run().clone()
+    torch.cuda.synchronize()
+    got = bk.replay().clone()
+    torch.cuda.synchronize()
+    assert torch.equal(got, eager)
+    vals = bk.H.values() * complex(np.exp(0.4j))
+    bk.H.set_values(vals)
+    assert not bk.H.is_real()
+    stale = bk.replay().clone()
+    torch.cuda.synchronize()
+    assert torch.isnan(stale[:, 1:].real).all()
+    fresh_eager = bk.run().clone()
+    torch.cuda.synchronize()
+    assert torch.isfinite(fresh_eager.real).all()
+    bk.capture()
+    again = bk.replay().clone()
+    torch.cuda.synchronize()
+    assert torch.equal(again, fresh_eager)
