@@ -143,6 +143,26 @@ int jkpm_ell_is_real(const jkpm_ell *h, int *is_real);
  * comparison); on = 1 (the default) lets them read the packed real copy. */
 int jkpm_ell_use_real_values(jkpm_ell *h, int on);
 
+/* Left vectors as phases.  A complex64 handle's plain recursion with one
+ * left set (jkpm_moments with v_l1 = NULL, jkpm_moments_ex, the batch and
+ * autotune paths) packs v_l into 32-bit phases (2 pi = 2^32) in its
+ * workspace at the start of each call, on the device.  When every element
+ * is reproduced within 2^-21 per component by its phase (random-phase
+ * vectors e^{i phi}, jclsquant/observables.py:19-21), the step kernels read
+ * the phases, 4 bytes per element instead of 8, and rebuild e^{i phi} in
+ * registers; otherwise they read v_l as given.  The choice is made on the
+ * device, so a captured graph follows the vectors of each replay.  The
+ * workspace sizes include the phase buffer (N * R * 4 bytes) while this is
+ * on (jkpm_moments_noalloc keeps it in its scratch).  on = 0 turns it off
+ * for the handle (query the workspace size again afterwards).  No reference
+ * counterpart (the reference reads v_l as complex128 in every dot,
+ * jclsquant/recurrence_functions.py:31-38). */
+int jkpm_ell_use_left_phases(jkpm_ell *h, int on);
+/* The packing step on its own: phases[i] of complex64 v[i], i < n, and *bad
+ * (device int) = 0 when every element is reproduced within 2^-21 per
+ * component, else nonzero.  Asynchronous on stream. */
+int jkpm_phase_pack(const void *v, long long n, unsigned *phases, int *bad, void *stream);
+
 /* Non-finite entries (NaN, +-Inf) among the n doubles of device array x
  * (e.g. the 2*M*R doubles of a moment array): *first_bad = the smallest
  * such index or -1, *count = how many.  Synchronises `stream`; refused under
@@ -226,8 +246,9 @@ int jkpm_moments(const jkpm_ell *h, const void *v_r, const void *v_l0, const voi
 /* jkpm_moments without a workspace argument; it allocates nothing.  No
  * counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
- * forbid hipMalloc).  Its scratch (the two rotating vector batches and the
- * moment-partial ring) is a library-owned device array of
+ * forbid hipMalloc).  Its scratch (the two rotating vector batches, the
+ * moment-partial ring and, for complex64, the left vectors' phases; the
+ * layout of jkpm_moments_workspace) is a library-owned device array of
  * JKPM_NOALLOC_SCRATCH_BYTES (128 MiB) that every GPU the library runs on
  * holds until the process ends; a matrix and R whose layout needs more
  * (jkpm_moments_workspace) get JKPM_ERR_WORKSPACE.  Nothing in the scratch

@@ -62,6 +62,8 @@ SIGNATURES = {
     "jkpm_ell_hermiticity": (_int, [_vp, _dp, _dp, _vp]),
     "jkpm_ell_is_real": (_int, [_vp, _ip]),
     "jkpm_ell_use_real_values": (_int, [_vp, _int]),
+    "jkpm_ell_use_left_phases": (_int, [_vp, _int]),
+    "jkpm_phase_pack": (_int, [_vp, _c.c_longlong, _vp, _vp, _vp]),
     "jkpm_check_finite": (_int, [_vp, _c.c_longlong, _c.POINTER(_c.c_longlong),
                                  _c.POINTER(_c.c_longlong), _vp]),
     "jkpm_zvec": (_int, [_int, _c.c_longlong, _dp, _dp, _vp, _vp, _vp, _vp, _vp]),
@@ -490,6 +492,14 @@ class DeviceELL:
         are all real (jkpm_ell_use_real_values)."""
         check(lib().jkpm_ell_use_real_values(self._h, 1 if on else 0))
         self._use_real = bool(on)
+
+    def use_left_phases(self, on: bool = True) -> None:
+        """on=False: complex64 plain recursions read the left vectors as
+        complex numbers even when they are unit complex numbers, whose 32-bit
+        phases they read otherwise (jkpm_ell_use_left_phases).  Workspace
+        sizes change with it."""
+        check(lib().jkpm_ell_use_left_phases(self._h, 1 if on else 0))
+        self._left_phases = bool(on)
 
     def is_hermitian(self, stream=None) -> bool:
         """True when max |a_ij - conj(a_ji)| <= HERMITIAN_RTOL * max |a_ij|, the
@@ -999,6 +1009,22 @@ def profile_read(reset: bool = True, kind: int = JKPM_PROF_STEP):
 
 def copy(dst, src, nbytes: int, stream=None) -> None:
     check(lib().jkpm_copy(dst.data_ptr(), src.data_ptr(), int(nbytes), stream_ptr(stream)))
+
+
+def phase_pack(v, stream=None):
+    """(phases, ok) of a complex64 device tensor: int32 phases (2 pi = 2^32,
+    as the step kernels read them; the bits are those of the unsigned phase)
+    and whether every element is reproduced within 2^-21 per component
+    (jkpm_phase_pack)."""
+    torch = _torch()
+    if v.dtype != torch.complex64 or not v.is_cuda:
+        raise ValueError("phase_pack takes a complex64 device tensor")
+    v = v.contiguous()
+    ph = torch.empty(v.shape, dtype=torch.int32, device=v.device)
+    bad = torch.empty(1, dtype=torch.int32, device=v.device)
+    check(lib().jkpm_phase_pack(v.data_ptr(), v.numel(), ph.data_ptr(), bad.data_ptr(),
+                                stream_ptr(stream)))
+    return ph, int(bad.item()) == 0
 
 
 def rec_A_tab_host(data, indices, len_row: int, v, M: int, v_r=None):

@@ -42,7 +42,10 @@ def _key(h: "nat.DeviceELL", R: int, doubling: bool) -> str:
     form = "|doubling" if doubling else ""
     # the step kernels on the packed real values are other kernels
     rv = "|real" if h.is_real() and getattr(h, "_use_real", True) else ""
-    return f"{dev}|N={h.n}|K={h.k}|vtype={h.vtype}|R={R}{rv}{form}"
+    # complex64 plain recursions with the left-vector phases turned off
+    lph = ("|complex-left" if h.vtype == nat.JKPM_C64 and not doubling
+           and not getattr(h, "_left_phases", True) else "")
+    return f"{dev}|N={h.n}|K={h.k}|vtype={h.vtype}|R={R}{rv}{lph}{form}"
 
 
 def _load():

@@ -88,6 +88,8 @@ bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 struct MomentsLayout {
     size_t b0 = 0, b1 = 0, ring = 0, raw = 0, total = 0;
+    size_t ph = 0, ph_flag = 0;  // left-vector phases and their flag (when has_ph)
+    bool has_ph = false;
     int nchunks = 1;
 };
 
@@ -105,7 +107,9 @@ struct Groups {
 };
 
 // doubling_M > 0: the doubling form for that many moments (two partial sets
-// and a raw-dot buffer [2][G][S+1][R]).
+// and a raw-dot buffer [2][G][S+1][R]).  A complex64 plain recursion with one
+// left set also gets a buffer for the left vectors' phases (4 bytes per
+// element) and its flag, unless the handle turned them off.
 MomentsLayout moments_layout(const Ell &H, int R, int n_left, const DeviceInfo &dev,
                              const Groups *grp = nullptr, int doubling_M = 0) {
     MomentsLayout L;
@@ -120,6 +124,12 @@ MomentsLayout moments_layout(const Ell &H, int R, int n_left, const DeviceInfo &
     if (doubling_M > 0) {
         const size_t G = grp ? grp->G : 1;
         L.total += align_up(2 * G * (doubling_steps(doubling_M) + 1) * R * sizeof(double2), 256);
+    }
+    if (H.left_phases && H.vtype == JKPM_C64 && doubling_M == 0 && n_left == 1) {
+        L.has_ph = true;
+        L.ph = L.total;
+        L.ph_flag = L.ph + align_up((size_t)H.n * R * sizeof(unsigned), 256);
+        L.total = L.ph_flag + 256;
     }
     return L;
 }
@@ -165,10 +175,11 @@ int device_info(DeviceInfo *out) {
 // stack (plain form only): T_n is written to stack + n * (N*R elements)
 // instead of the two rotating buffers, so every T_1 .. T_{M-1} stays (the
 // caller puts T_0 in slot 0).
+// no_alloc: the caller may not allocate, so no profiling events are taken.
 static int run_moments(const Ell &H, const void *vr, const void *vl0, const void *vl1, int R,
                        int M, double2 *mu, void *ws, size_t ws_bytes, hipStream_t s,
                        const Groups *grp = nullptr, bool doubling = false,
-                       void *stack = nullptr, bool no_profile = false) {
+                       void *stack = nullptr, bool no_alloc = false) {
     DeviceInfo dev;
     int rc = device_info(&dev);
     if (rc) return rc;
@@ -199,8 +210,8 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
     Prof &P = prof();
     std::unique_lock<std::mutex> plk(P.mu);
     const int every = P.every;
-    // (no_profile: the caller may not allocate, and recording takes events)
-    const bool profiling = every > 0 && cap == hipStreamCaptureStatusNone && !no_profile;
+    // (no_alloc: the caller may not allocate, and recording takes events)
+    const bool profiling = every > 0 && cap == hipStreamCaptureStatusNone && !no_alloc;
     if (!profiling) plk.unlock();
 
     // Even steps sweep their chunks backwards: a step then starts on the rows
@@ -231,6 +242,10 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
         }
         a.vl0 = doubling ? nullptr : vl0;
         a.vl1 = doubling ? nullptr : vl1;
+        if (L.has_ph) {
+            a.vph = (const unsigned *)(w + L.ph);
+            a.ph_bad = (const int *)(w + L.ph_flag);
+        }
         a.doubling = doubling;
         a.ld = R;
         a.part = ring + (size_t)(n % kRing) * slot;
@@ -263,6 +278,11 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
     // cost of the recursion, inter-kernel gaps included.
     ProfRun run;
     bool run_open = false;
+    if (L.has_ph) {
+        rc = launch_phase_pack(vl0, (long)H.n * R, (unsigned *)(w + L.ph), (int *)(w + L.ph_flag),
+                               s);
+        if (rc) return rc;
+    }
     for (int n = 1; n <= nsteps; ++n) {
         if (profiling && !run_open && n > 1 && P.counter % every == 0) {
             run.e0 = take_event(P);
@@ -984,6 +1004,21 @@ int jkpm_ell_use_real_values(jkpm_ell *h, int on) {
     h->use_real = on != 0;
     h->e.real_values = h->values_real && h->use_real && h->e.rval;
     return JKPM_OK;
+}
+
+int jkpm_ell_use_left_phases(jkpm_ell *h, int on) {
+    clear_error();
+    if (!h) return arg_error("handle is NULL");
+    h->e.left_phases = on != 0;
+    return JKPM_OK;
+}
+
+int jkpm_phase_pack(const void *v, long long n, unsigned *phases, int *bad, void *stream) {
+    clear_error();
+    if (!bad || (n > 0 && (!v || !phases))) return arg_error("NULL argument");
+    if (n < 0) return arg_error("n must be >= 0");
+    if (!aligned16(v) || !aligned16(phases)) return arg_error("device pointers must be 16-byte aligned");
+    return launch_phase_pack(v, (long)n, phases, bad, (hipStream_t)stream);
 }
 
 int jkpm_check_finite(const double *x, long long n, long long *first_bad, long long *count,

@@ -18,6 +18,9 @@
 //                 gets chunk (b % 8) * (nchunks / 8) + b / 8 and every XCD
 //                 sweeps one contiguous eighth of the lattice -- the gather
 //                 window of neighbouring lattice rows stays in that XCD's L2.
+//   left vectors  complex64 random-phase vectors are read as 32-bit phases
+//                 (4 bytes per element) and rebuilt in registers
+//                 (phase_pack_kernel writes them once per recursion).
 //   moments       each lane accumulates conj(v_l) * T_{n+1} in FP64; a
 //                 butterfly over the lanes of equal vector index, then a fixed
 //                 wave order in LDS, gives one partial per (chunk, vector).
@@ -27,6 +30,8 @@
 // The step kernel template lives in cheb_step.h and is instantiated per
 // precision in cheb_c64.hip / cheb_c128.hip; this file holds its launcher and
 // the other small kernels.
+#include <algorithm>
+
 #include "cheb_step.h"
 
 namespace jkpm {
@@ -206,6 +211,30 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(const i32x4 *__restrict__ 
     }
 }
 
+// Phases of complex64 numbers for the step kernel's left-vector stream:
+// u = round(atan2(y, x) / (2 pi) * 2^32) mod 2^32, then a check that
+// step::phase_cs(u) gives (x, y) back within 2^-21 per component.  An
+// element off the unit circle, or a NaN, fails the check.
+__global__ __launch_bounds__(kBlock) void phase_pack_kernel(const float2 *__restrict__ v, long n,
+                                                             unsigned *__restrict__ ph,
+                                                             int *__restrict__ bad) {
+    constexpr double kTurns = 4294967296.0 / 6.283185307179586476925;  // 2^32 / (2 pi)
+    constexpr float kTol = 4.76837158203125e-07f;                       // 2^-21
+    bool fail = false;
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+        const float2 z = v[i];
+        const double t = rint(atan2((double)z.y, (double)z.x) * kTurns);  // [-2^31, 2^31]
+        const unsigned u = t == t ? (unsigned)(long long)t : 0u;
+        float c, sn;
+        step::phase_cs(u, c, sn);
+        fail |= !(fabsf(c - z.x) <= kTol && fabsf(sn - z.y) <= kTol);
+        ph[i] = u;
+    }
+    if (__any(fail) && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
+}
+
+__global__ void clear_flag_kernel(int *p) { *p = 0; }
+
 }  // namespace
 
 int max_group(int) { return 32; }
@@ -252,14 +281,17 @@ int launch_step(const StepArgs &a, hipStream_t s) {
     // (the doubling form reads no left vectors; its pointers stay unused)
     const void *p_vl0 = a.vl0 ? (const char *)a.vl0 + off : nullptr;
     const void *p_vl1 = a.vl1 ? (const char *)a.vl1 + off : nullptr;
+    const unsigned *p_vph = a.vph ? a.vph + a.r0 : nullptr;
+    const int *p_phb = a.ph_bad;
     int p_ld = a.ld;
     double2 *p_part = a.part + a.r0;
     double2 *p_part0 = a.part0 ? a.part0 + a.r0 : nullptr;
     int p_first = a.first ? 1 : 0, p_rpc = (int)rpc, p_nch = a.nchunks;
     const long *p_rows = a.chunk_rows;
     int p_rev = a.reverse ? 1 : 0;
-    void *args[] = {&p_val,  &p_col,  &p_n,    &p_k,     &p_xcur,  &p_xprev, &p_xout, &p_vl0,
-                    &p_vl1,  &p_ld,   &p_part, &p_part0, &p_first, &p_rpc,   &p_nch,  &p_rows,
+    void *args[] = {&p_val,   &p_col,  &p_n,   &p_k,    &p_xcur,  &p_xprev,
+                    &p_xout,  &p_vl0,  &p_vl1, &p_vph,  &p_phb,   &p_ld,
+                    &p_part,  &p_part0, &p_first, &p_rpc, &p_nch, &p_rows,
                     &p_rev};
     JKPM_HIP(hipLaunchKernel(e.fn, dim3(a.nchunks), dim3(kBlock), args, 0, s));
     JKPM_CHECK_LAUNCH("cheb_step_kernel");
@@ -321,6 +353,21 @@ int launch_doubling_finish(const double2 *raw, int G, int S1, int R, int M, doub
     hipLaunchKernelGGL(doubling_finish_kernel, dim3((unsigned)((tot + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, s, raw, G, S1, R, M, mu);
     JKPM_CHECK_LAUNCH("doubling_finish_kernel");
+    return JKPM_OK;
+}
+
+int launch_phase_pack(const void *v, long n, unsigned *ph, int *bad, hipStream_t s) {
+    hipLaunchKernelGGL(clear_flag_kernel, dim3(1), dim3(1), 0, s, bad);
+    JKPM_CHECK_LAUNCH("clear_flag_kernel");
+    if (n <= 0) return JKPM_OK;
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const long want = (n + kBlock - 1) / kBlock;
+    const int grid = (int)std::min(want, (long)dev.cu_count * 8);
+    hipLaunchKernelGGL(phase_pack_kernel, dim3(grid), dim3(kBlock), 0, s, (const float2 *)v, n,
+                       ph, bad);
+    JKPM_CHECK_LAUNCH("phase_pack_kernel");
     return JKPM_OK;
 }
 

@@ -27,6 +27,29 @@ __device__ __forceinline__ V load_stream(const V *p) {
         return *p;
 }
 
+// Unit complex number e^{i phi} from a 32-bit phase u = phi / (2 pi) * 2^32
+// (mod 2^32).  The angle is split into the nearest quarter turn q and a rest
+// a in [-pi/4, pi/4); sin a and cos a are their Taylor series (truncation
+// below 2e-9 at |a| = pi/4), then rotated by q quarter turns.  Each
+// component is within 1.2e-7 (2^-23) of the exact value (every 7th u
+// compared with double precision).
+constexpr float kPhaseUnit = 1.4629180792671596e-09f;  // 2 pi / 2^32
+__device__ __forceinline__ void phase_cs(unsigned u, float &c, float &s) {
+    const unsigned q = (u + (1u << 29)) >> 30;
+    const float a = (float)(int)(u - (q << 30)) * kPhaseUnit;
+    const float a2 = a * a;
+    const float sn =
+        a + a * a2 * (-1.f / 6 + a2 * (1.f / 120 + a2 * (-1.f / 5040 + a2 * (1.f / 362880))));
+    const float cs =
+        1.f + a2 * (-0.5f + a2 * (1.f / 24 + a2 * (-1.f / 720 + a2 * (1.f / 40320 +
+                                                                   a2 * (-1.f / 3628800)))));
+    // rotation by q quarter turns: (cs, sn) -> (-sn, cs) per turn
+    const bool sw = q & 1, neg_c = (q + 1) & 2, neg_s = q & 2;
+    const float c0 = sw ? sn : cs, s0 = sw ? cs : sn;
+    c = neg_c ? -c0 : c0;
+    s = neg_s ? -s0 : s0;
+}
+
 // UNR: rows per lane per loop iteration (all their loads are issued before
 // any is used, for more bytes in flight per wave).
 // DBL (doubling form, NL = 2, Hermitian A and v_l = v_r = T_0): instead of
@@ -46,12 +69,17 @@ __device__ __forceinline__ V load_stream(const V *p) {
 // RV (real values): `val` holds one real per slot (a matrix whose imaginary
 // parts are all 0, packed by the handle), so the matrix stream is 4 (8)
 // bytes per slot instead of 8 (16) and each product takes two multiplies.
+// Left phases (complex64, one left set, plain form): when vph is non-NULL
+// and *ph_bad is 0, v_l is read as 32-bit phases (phase_cs) from vph [n][ld]
+// instead of complex64 pairs, 4 bytes per element instead of 8.  The flag is
+// read on the device, so a captured graph follows the vectors of each replay.
 template <typename Real, int VPL, int LPR, int NL, int KT, int UNR, bool NT, bool DBL = false,
           bool NS = false, bool NTM = false, bool F0 = false, bool RV = false>
 __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const Real *__restrict__ val, const int *__restrict__ col, int n, int k_rt,
     const Real *__restrict__ xcur, const Real *xprev, Real *xout,
-    const Real *__restrict__ vl0, const Real *__restrict__ vl1, int ld,
+    const Real *__restrict__ vl0, const Real *__restrict__ vl1,
+    const unsigned *__restrict__ vph, const int *__restrict__ ph_bad, int ld,
     double2 *__restrict__ part, double2 *__restrict__ part0, int first_arg,
     int rows_per_chunk, int nchunks, const long *__restrict__ chunk_rows, int reverse) {
     const bool first = F0 && first_arg;
@@ -61,6 +89,8 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     constexpr int RPW = 64 / LPR;
     constexpr int RB = VPL * LPR;
     const int K = KT > 0 ? KT : k_rt;
+    constexpr bool kPH = !DBL && NL == 1 && std::is_same_v<Real, float>;
+    const bool ph = kPH && vph && *ph_bad == 0;
     auto load_h = [&](long e) -> HV {
         if constexpr (RV)
             return load_stream<NTM>(val + e);
@@ -180,6 +210,19 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
 
         // T_{n-1} and v_l of every row first, then the arithmetic.
         V p[UNR], q[UNR][NL], t0[UNR];
+        using U = vec_t<unsigned, VPL>;
+        U wph[UNR];  // left phases, turned into q[u][0] after T_{n+1} is stored
+        auto phase_q = [&](int u) {
+            if (kPH && ph) {
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    float c, sn;
+                    phase_cs(wph[u][v], c, sn);
+                    q[u][0][2 * v] = c;
+                    q[u][0][2 * v + 1] = sn;
+                }
+            }
+        };
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             const long vo = rows[u] * ld * 2 + voff;
@@ -195,9 +238,14 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
                         q[u][1] = *reinterpret_cast<const V *>(xcur + vo);  // T_n, cached
                 }
             } else {
+                if (kPH && ph) {
+                    wph[u] = load_stream<NT>(
+                        reinterpret_cast<const U *>(vph + rows[u] * ld + lv * VPL));
+                } else {
 #pragma unroll
-                for (int l = 0; l < NL; ++l)
-                    q[u][l] = load_stream<NT>(reinterpret_cast<const V *>(vls[l] + vo));
+                    for (int l = 0; l < NL; ++l)
+                        q[u][l] = load_stream<NT>(reinterpret_cast<const V *>(vls[l] + vo));
+                }
             }
         }
 #pragma unroll
@@ -212,6 +260,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
                 }
             }
             if (first) {
+                phase_q(u);
 #pragma unroll
                 for (int v = 0; v < VPL; ++v) {
                     y[2 * v] = sr[u][v];
@@ -240,6 +289,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
             else
                 *reinterpret_cast<V *>(xout + vo) = y;
             if constexpr (DBL) q[u][0] = y;
+            if (!first) phase_q(u);
 #pragma unroll
             for (int l = 0; l < NL; ++l)
 #pragma unroll

@@ -73,6 +73,10 @@ struct Ell {
     // 0, checked at each value upload), when the step kernels read these.
     void *rval = nullptr;
     bool real_values = false;
+    // complex64 plain recursions with one left set keep the left vectors'
+    // phases in the workspace and read them instead when every element is a
+    // unit complex number (jkpm_ell_use_left_phases turns this off)
+    bool left_phases = true;
 };
 
 // Bytes of one complex element of a value type.
@@ -87,6 +91,10 @@ struct StepArgs {
     void *xout;           // T_{n+1}      [n][ld] (may equal xprev)
     const void *vl0;      // left vectors [n][ld]
     const void *vl1;      // second left set or nullptr
+    // complex64, one left set: vl0 as 32-bit phases [n][ld] and the device
+    // flag that is 0 when every phase reproduces vl0 (else vl0 is read)
+    const unsigned *vph = nullptr;
+    const int *ph_bad = nullptr;
     int ld;
     int r0;
     int rb;
@@ -110,6 +118,11 @@ int step_chunks(const Ell &H, int rb, const DeviceInfo &dev, bool doubling = fal
 
 // Largest vector group one launch handles for a value type.
 int max_group(int vtype);
+
+// Phases of n complex64 numbers (32-bit, 2 pi = 2^32, as read by the step
+// kernel) into ph; *bad is cleared first and becomes nonzero when some
+// element is not reproduced within 2^-21 per component by its phase.
+int launch_phase_pack(const void *v, long n, unsigned *ph, int *bad, hipStream_t s);
 
 int launch_step(const StepArgs &a, hipStream_t s);
 

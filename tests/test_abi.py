@@ -61,6 +61,8 @@ def test_symbols_are_c_linkage():
 def test_calls_that_need_no_device():
     lib = nat.load_library()
     assert lib.jkpm_version() >= 100
+    # a call that succeeds clears the error an earlier call left
+    assert lib.jkpm_profile_enable(0) == nat.JKPM_OK
     assert lib.jkpm_last_error() == b""
     # argument errors are reported before any device work
     rc = lib.jkpm_moments(None, None, None, None, 1, 1, None, None, 0, None)
