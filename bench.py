@@ -455,7 +455,9 @@ def bench_batch(args, torch, jcl, nat, _kpm):
     diff = float((mu_d - bk.mu).abs().max() / bk.mu[:, 0].abs().max())
     units = N * M * R
     mv = 4 if bk.H.is_real() else 8  # matrix value bytes per slot (packed real copy)
-    bytes_launch = N * K * (mv + 4) + R * N * 4 * 8
+    # the random-phase left vectors (v_l = v_r) are read as 32-bit phases
+    s_l = 4 if nat.phase_pack(bk.vr)[1] else 8
+    bytes_launch = N * K * (mv + 4) + R * N * (3 * 8 + s_l)
     bytes_d = N * K * (mv + 4) + R * N * 3 * 8
     ach = bytes_launch * (M - 1) / (ms / 1e3) / 1e9
     ach_d = bytes_d * max(1, M // 2) / (ms_d / 1e3) / 1e9
@@ -490,7 +492,8 @@ def bench_batch(args, torch, jcl, nat, _kpm):
                      "G": G, "sum_N": N, "K": K, "M": M, "R": R})
     out["roofline"] = hbm_roofline(bytes_launch, launch_ms, n_prof, "cheb_step_kernel",
                                    whole_replay_GBs=round(ach, 1),
-                                   whole_replay_frac=round(ach / HBM_PEAK_GBS, 4))
+                                   whole_replay_frac=round(ach / HBM_PEAK_GBS, 4),
+                                   left_vector_bytes_per_element=s_l)
     out["roofline"]["traffic"] = profile_traffic(
         PROFILES / "cheb_step_c64_r8_batch64_square_trace.md", "traffic_bytes_per_launch",
         N=N, K=K, R=R)
@@ -894,6 +897,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dos-check", action="store_true")
     ap.add_argument("--no-doubling", action="store_true")
+    ap.add_argument("--no-complex-left", action="store_true",
+                    help="skip the plain recursion with v_l read as complex numbers")
     ap.add_argument("--no-tiles", action="store_true",
                     help="skip the two-steps-per-launch (LDS patch) path")
     ap.add_argument("--no-wave", action="store_true",
@@ -1018,7 +1023,10 @@ def main():
     value = units / (ms_per_step / 1e3)
     s_v = 8 if args.dtype == "c64" else 16
     mv = s_v // 2 if h.is_real() else s_v  # matrix value bytes per slot (packed real copy)
-    bytes_launch = N * K * (mv + 4) + R * N * 4 * s_v
+    # complex64: the random-phase left vectors (v_l = v_r) are read as 32-bit phases
+    phases = args.dtype == "c64" and nat.phase_pack(vr)[1]
+    s_l = 4 if phases else s_v
+    bytes_launch = N * K * (mv + 4) + R * N * (3 * s_v + s_l)
     achieved = bytes_launch / (launch_ms / 1e3) / 1e9
     roofline = {
         "bound": "hbm",
@@ -1030,6 +1038,7 @@ def main():
         "kernel": "cheb_step_kernel",
         "algorithmic_bytes_per_launch": bytes_launch,
         "matrix_value_bytes_per_slot": mv,
+        "left_vector_bytes_per_element": s_l,
         "mean_launch_us": round(launch_ms * 1e3, 2),
         "launches_timed": n_prof,
         "copy_kernel_GBs": round(copy_gbs, 1),
@@ -1072,6 +1081,34 @@ def main():
                                 float(np.max(np.abs(dos_d.cpu().numpy() - d128))
                                       / np.max(np.abs(d128)))),
         }
+
+    complex_left = None
+    if phases and not args.no_complex_left:
+        # the same plain recursion reading v_l as complex64 (its own tuned launch)
+        h.use_left_phases(False)
+        if not args.no_tune:
+            autotune.tuned(h, vr)
+        h.moments(vr, M, workspace=ws, out=mu)
+        torch.cuda.synchronize()
+        ms_c, launch_c, n_c, _ = timed(False)
+        shape_c = list(h.get_launch())
+        h.use_left_phases(True)
+        if not args.no_tune:
+            autotune.tuned(h, vr)
+        bytes_c = N * K * (mv + 4) + R * N * 4 * s_v
+        ach_c = bytes_c / (launch_c / 1e3) / 1e9
+        complex_left = {
+            "value": units / (ms_c / 1e3), "ms_per_step": ms_c,
+            "launch": shape_c,
+            "roofline": {"bound": "hbm", "achieved": round(ach_c, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach_c / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes_per_launch": bytes_c,
+                         "mean_launch_us": round(launch_c * 1e3, 2), "launches_timed": n_c},
+            "headline_over_it": value / (units / (ms_c / 1e3)),
+            "note": "the plain recursion with v_l read as complex64 pairs (8 bytes per "
+                    "element) instead of 32-bit phases",
+        }
+        log(f"complex left vectors: {ms_c:.2f} ms per step")
 
     tiles = None
     if not args.no_tiles:
@@ -1122,6 +1159,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "doubling": dbl,
+        "complex_left": complex_left,
         "plain_c128": plain,
         "two_step_tiles": tiles,
         "two_step_wave": wave,

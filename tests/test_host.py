@@ -651,9 +651,10 @@ def test_bench_reads_traffic_from_committed_profiles():
     }
     # algorithmic bytes per launch of each kernel (the step kernels of these
     # real-valued lattices read 4 or 8 bytes per slot value, plus a 4-byte
-    # column); counters sit 0-7 % above
-    alg = {"headline": 2097152 * (4 * 8 + 16 * 32), "headline_dbl": 2097152 * (4 * 8 + 16 * 24),
-           "batch": 24054938 * (5 * 8 + 8 * 32), "config1": 65536 * (5 * 12 + 4 * 64),
+    # column; the complex64 plain forms read their random-phase left vectors
+    # as 4-byte phases); counters sit 0-7 % above
+    alg = {"headline": 2097152 * (4 * 8 + 16 * 28), "headline_dbl": 2097152 * (4 * 8 + 16 * 24),
+           "batch": 24054938 * (5 * 8 + 8 * 28), "config1": 65536 * (5 * 12 + 4 * 64),
            "grad": 262144 * (5 * 12 + 8 * 64), "kubo_gemm": 2 * 512 * 2097152 * 8,
            "kubo_step": 524288 * 4 * 20 + 4 * 524288 * 8 * 7}
     for k, v in got.items():
