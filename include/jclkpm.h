@@ -282,6 +282,19 @@ int jkpm_moments_noalloc(const jkpm_ell *h, const void *v_r, const void *v_l0, c
 int jkpm_ell_set_launch(jkpm_ell *h, int variant, int chunk_mult, int flags);
 int jkpm_ell_get_launch(const jkpm_ell *h, int *variant, int *chunk_mult, int flags);
 
+/* Vectors per recursion pass (0, the default: all R at once).  With
+ * 0 < vectors < R, a moments call runs the whole recursion for each group of
+ * `vectors` columns in turn (copied to a contiguous [N][vectors] array in the
+ * workspace, the moments written to their columns of mu), so that one pass's
+ * vectors and the matrix fit in the 256 MB Infinity Cache; the workspace
+ * sizes cover both the passes and all R vectors at once (jkpm_moments_keep
+ * never runs in passes).  flags = JKPM_MOMENTS_DOUBLING addresses the
+ * doubling form, 0 the plain recursion.  autotune.tuned chooses it with the
+ * launch shape.  No reference counterpart (the reference runs one vector per
+ * recursion, jclsquant/recurrence_functions.py:9-38). */
+int jkpm_ell_set_pass(jkpm_ell *h, int vectors, int flags);
+int jkpm_ell_get_pass(const jkpm_ell *h, int *vectors, int flags);
+
 /* Times `steps` fused steps of jkpm_moments (flags = 0) or of the doubling
  * form of jkpm_moments_ex (flags = JKPM_MOMENTS_DOUBLING) on v_r (device
  * [N][R]) for each variant x chunk_mult in {2, 4, 8, 16, 32}, keeps the

@@ -119,6 +119,8 @@ SIGNATURES = {
     "jkpm_graph_destroy": (_int, [_vp]),
     "jkpm_ell_set_launch": (_int, [_vp, _int, _int, _int]),
     "jkpm_ell_get_launch": (_int, [_vp, _ip, _ip, _int]),
+    "jkpm_ell_set_pass": (_int, [_vp, _int, _int]),
+    "jkpm_ell_get_pass": (_int, [_vp, _ip, _int]),
     "jkpm_autotune_workspace": (_int, [_vp, _int, _int, _int, _c.POINTER(_size)]),
     "jkpm_autotune": (_int, [_vp, _vp, _int, _int, _int, _vp, _size, _vp, _dp]),
     "jkpm_moments_ex_workspace": (_int, [_vp, _vp, _int, _int, _int, _c.POINTER(_size)]),
@@ -538,6 +540,18 @@ class DeviceELL:
         doubling form with doubling=True)."""
         check(lib().jkpm_ell_set_launch(self._h, int(variant), int(chunk_mult),
                                         JKPM_MOMENTS_DOUBLING if doubling else 0))
+
+    def set_pass(self, vectors: int, doubling: bool = False) -> None:
+        """Vectors per recursion pass of the plain recursion (or of the
+        doubling form); 0 runs all vectors at once (jkpm_ell_set_pass)."""
+        check(lib().jkpm_ell_set_pass(self._h, int(vectors),
+                                      JKPM_MOMENTS_DOUBLING if doubling else 0))
+
+    def get_pass(self, doubling: bool = False) -> int:
+        v = _c.c_int()
+        check(lib().jkpm_ell_get_pass(self._h, _c.byref(v),
+                                      JKPM_MOMENTS_DOUBLING if doubling else 0))
+        return int(v.value)
 
     def get_launch(self, doubling: bool = False):
         v, m = _c.c_int(), _c.c_int()

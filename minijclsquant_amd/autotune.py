@@ -4,10 +4,12 @@ The step kernel has 48 variants (rows per lane x non-temporal vector loads x
 vectors per lane x non-temporal stores x non-temporal matrix loads) and a
 work-group count per CU, set separately for the
 plain recursion and for the doubling form; jkpm_autotune times them on the
-caller's matrix and vectors.  Results are cached in a JSON table keyed by
-device, matrix shape (N, K), value type, batch size R and form, so a shape
-is tuned once per machine.  ``JKPM_LAUNCH="variant,chunk_mult"`` overrides
-the table (for both forms).
+caller's matrix and vectors.  ``tuned`` also chooses how many of the R
+vectors one recursion pass carries (all of them, or halves down to 4), so a
+pass's vectors and the matrix can stay in the 256 MB Infinity Cache.
+Results are cached in a JSON table keyed by device, matrix shape (N, K),
+value type, batch size R and form, so a shape is tuned once per machine.
+``JKPM_LAUNCH="variant,chunk_mult"`` overrides the table (for both forms).
 
 The reference has a single fixed launch (<<<ceil(N/1024), 1024>>>,
 jclsquant/cuda_cython/cuda/recursions.cu:19-20).
@@ -21,7 +23,7 @@ from pathlib import Path
 
 from . import _native as nat
 
-__all__ = ["tuned", "tuned_batch", "table_path"]
+__all__ = ["tuned", "tuned_batch", "table_path", "pass_candidates"]
 
 _lock = threading.Lock()
 
@@ -125,33 +127,99 @@ def _finalists(h, v_r, v, m, us, doubling):
     return best[1], best[0] * 1e3 / steps
 
 
+def pass_candidates(R: int):
+    """Vectors per recursion pass the tuner tries for R vectors: all of
+    them, then halves down to 4 (0 stands for all at once)."""
+    out, g = [0], R // 2
+    while g >= 4:
+        out.append(g)
+        g //= 2
+    return out
+
+
+def _tune_shape(h, v_r, steps, doubling):
+    us = h.autotune(v_r, steps, doubling=doubling)
+    # Short steps: time again over about 2 ms per candidate, so that
+    # shapes a few percent apart are told apart.
+    if us * steps < _MIN_TIMED_US:
+        steps2 = min(_MAX_TUNE_STEPS, int(_MIN_TIMED_US / max(us, 1e-3)) + 1)
+        if steps2 > steps:
+            us = h.autotune(v_r, steps2, doubling=doubling)
+    v, m = h.get_launch(doubling)
+    v, us = _finalists(h, v_r, v, m, us, doubling)
+    return v, m, us
+
+
+_SLOPE_US = 4000.0  # timed span of the shorter run of the pass comparison
+_PASS_MARGIN = 0.02  # a pass size below R must beat all-at-once by this much
+
+
+def _step_slope(h, v_r, doubling, us):
+    """Per-step time (us) of the handle's current setting, as the slope
+    between recursions of a and 3a steps (best of two each): the fixed cost
+    of a call -- per pass a column copy, the left-phase packing and the
+    first step -- cancels, as it does in a recursion of thousands of steps."""
+    import torch
+
+    R = int(v_r.shape[1])
+    a = max(16, int(_SLOPE_US / max(us, 1e-3)))
+    times = []
+    for steps in (a, 3 * a):
+        M = steps + 1 if not doubling else 2 * steps
+        need = nat.moments_ex_workspace_bytes(h, None, R, M, doubling)
+        ws = torch.empty(need, dtype=torch.uint8, device=v_r.device)
+        nat.moments_ex(h, None, v_r, M, doubling, workspace=ws)
+        t = float("inf")
+        for _ in range(2):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            nat.moments_ex(h, None, v_r, M, doubling, workspace=ws)
+            e1.record()
+            e1.synchronize()
+            t = min(t, e0.elapsed_time(e1))
+        times.append(t)
+    return (times[1] - times[0]) * 1e3 / (2 * a)
+
+
 def tuned(h: "nat.DeviceELL", v_r, steps: int = 8, retune: bool = False,
-          doubling: bool = False):
-    """Applies the cached (or freshly tuned) launch shape to handle h for the
-    batch v_r [N, R] and the chosen form; returns (variant, chunk_mult)."""
+          doubling: bool = False, passes: bool = True):
+    """Applies the cached (or freshly tuned) launch shape and vectors per
+    pass to handle h for the batch v_r [N, R] and the chosen form; returns
+    (variant, chunk_mult).  Each pass size of pass_candidates(R) is tuned
+    on its own and the fastest whole step (all passes) wins, a pass size
+    below R only when it is at least 2 % faster per step.  passes=False
+    tunes all R vectors at once only (for the moments-with-stack form, which
+    never runs in passes)."""
     env = os.environ.get("JKPM_LAUNCH")
     if env:
         v, m = (int(x) for x in env.split(","))
         h.set_launch(v, m, doubling)
         return v, m
     R = int(v_r.shape[1])
-    key = _key(h, R, doubling)
+    key = _key(h, R, doubling) + ("" if passes else "|at-once")
     with _lock:
         tab = _load()
         if not retune and key in tab:
             v, m = tab[key]["variant"], tab[key]["chunk_mult"]
             h.set_launch(v, m, doubling)
+            h.set_pass(tab[key].get("pass", 0), doubling)
             return v, m
-        us = h.autotune(v_r, steps, doubling=doubling)
-        # Short steps: time again over about 2 ms per candidate, so that
-        # shapes a few percent apart are told apart.
-        if us * steps < _MIN_TIMED_US:
-            steps2 = min(_MAX_TUNE_STEPS, int(_MIN_TIMED_US / max(us, 1e-3)) + 1)
-            if steps2 > steps:
-                us = h.autotune(v_r, steps2, doubling=doubling)
-        v, m = h.get_launch(doubling)
-        v, us = _finalists(h, v_r, v, m, us, doubling)
-        tab[key] = {"variant": v, "chunk_mult": m, "step_us": us}
+        best = None
+        cands = pass_candidates(R) if passes else [0]
+        for g in cands:
+            h.set_pass(g, doubling)
+            v, m, us = _tune_shape(h, v_r, steps, doubling)
+            if len(cands) > 1:
+                us = _step_slope(h, v_r, doubling, us)
+            # passes add copies and a schedule to the call: they must win clearly
+            lim = us if g == 0 else us * (1 + _PASS_MARGIN)
+            if best is None or lim < best[3]:
+                best = (g, v, m, us)
+        g, v, m, us = best
+        h.set_pass(g, doubling)
+        h.set_launch(v, m, doubling)
+        tab[key] = {"variant": v, "chunk_mult": m, "pass": g, "step_us": us}
         _store(tab)
         return v, m
 

@@ -110,8 +110,8 @@ struct Groups {
 // and a raw-dot buffer [2][G][S+1][R]).  A complex64 plain recursion with one
 // left set also gets a buffer for the left vectors' phases (4 bytes per
 // element) and its flag, unless the handle turned them off.
-MomentsLayout moments_layout(const Ell &H, int R, int n_left, const DeviceInfo &dev,
-                             const Groups *grp = nullptr, int doubling_M = 0) {
+MomentsLayout moments_layout_once(const Ell &H, int R, int n_left, const DeviceInfo &dev,
+                                  const Groups *grp = nullptr, int doubling_M = 0) {
     MomentsLayout L;
     const size_t vec = align_up((size_t)H.n * R * cplx_bytes(H.vtype), 256);
     if (doubling_M > 0) n_left = 2;
@@ -132,6 +132,42 @@ MomentsLayout moments_layout(const Ell &H, int R, int n_left, const DeviceInfo &
         L.total = L.ph_flag + 256;
     }
     return L;
+}
+
+// Vectors per recursion pass for a call with R vectors (R: all at once).
+int pass_vectors(const Ell &H, int R, bool doubling) {
+    const int p = doubling ? H.dpass_r : H.pass_r;
+    return (p > 0 && p < R) ? p : R;
+}
+
+// Workspace of a call that runs its R vectors in passes of g < R: the pass's
+// copies of v_r (and of distinct left sets, plain form) as [N][g], then the
+// layout of one pass of g vectors at offset `inner`.
+struct PassLayout {
+    int g = 0;
+    size_t vec = 0, inner = 0, total = 0;
+    MomentsLayout once;
+};
+
+PassLayout pass_layout(const Ell &H, int R, int n_left, const DeviceInfo &dev, const Groups *grp,
+                       int doubling_M) {
+    PassLayout P;
+    P.g = pass_vectors(H, R, doubling_M > 0);
+    P.once = moments_layout_once(H, P.g, n_left, dev, grp, doubling_M);
+    if (P.g < R) {
+        P.vec = align_up((size_t)H.n * P.g * cplx_bytes(H.vtype), 256);
+        P.inner = (size_t)(doubling_M > 0 ? 1 : n_left + 1) * P.vec;
+    }
+    P.total = P.inner + P.once.total;
+    return P;
+}
+
+// Workspace bytes of a moments call: enough for the handle's passes and for
+// all R vectors at once (the stack form never runs in passes).
+size_t moments_bytes(const Ell &H, int R, int n_left, const DeviceInfo &dev,
+                     const Groups *grp = nullptr, int doubling_M = 0) {
+    return std::max(pass_layout(H, R, n_left, dev, grp, doubling_M).total,
+                    moments_layout_once(H, R, n_left, dev, grp, doubling_M).total);
 }
 
 }  // namespace
@@ -176,15 +212,18 @@ int device_info(DeviceInfo *out) {
 // instead of the two rotating buffers, so every T_1 .. T_{M-1} stays (the
 // caller puts T_0 in slot 0).
 // no_alloc: the caller may not allocate, so no profiling events are taken.
-static int run_moments(const Ell &H, const void *vr, const void *vl0, const void *vl1, int R,
-                       int M, double2 *mu, void *ws, size_t ws_bytes, hipStream_t s,
-                       const Groups *grp = nullptr, bool doubling = false,
-                       void *stack = nullptr, bool no_alloc = false) {
+// mu_ld: row length of mu (R when 0), so that one pass of a call can write
+// its column slice of the caller's moments.
+static int run_moments_once(const Ell &H, const void *vr, const void *vl0, const void *vl1,
+                            int R, int M, double2 *mu, void *ws, size_t ws_bytes, hipStream_t s,
+                            const Groups *grp = nullptr, bool doubling = false,
+                            void *stack = nullptr, bool no_alloc = false, int mu_ld = 0) {
     DeviceInfo dev;
     int rc = device_info(&dev);
     if (rc) return rc;
+    if (mu_ld <= 0) mu_ld = R;
     const int n_left = (vl1 || doubling) ? 2 : 1;
-    const MomentsLayout L = moments_layout(H, R, n_left, dev, grp, doubling ? M : 0);
+    const MomentsLayout L = moments_layout_once(H, R, n_left, dev, grp, doubling ? M : 0);
     if (ws_bytes < L.total) {
         set_error("workspace too small: need " + std::to_string(L.total) + " bytes, got " +
                   std::to_string(ws_bytes));
@@ -309,12 +348,58 @@ static int run_moments(const Ell &H, const void *vr, const void *vl0, const void
         }
         if (reduce_now) {
             rc = launch_reduce(ring, kRing, pending_lo, last - pending_lo + 1, n_left, L.nchunks,
-                               R, target, target_M, grp ? grp->group_chunks : nullptr, G, s);
+                               R, target, target_M, grp ? grp->group_chunks : nullptr, G, s,
+                               doubling ? R : mu_ld);
             if (rc) return rc;
             pending_lo = n + 1;
         }
     }
-    if (doubling) return launch_doubling_finish(target, G, target_M, R, M, mu, s);
+    if (doubling) return launch_doubling_finish(target, G, target_M, R, M, mu, s, mu_ld);
+    return JKPM_OK;
+}
+
+// The whole recursion for R vectors: at once, or, when the handle sets fewer
+// vectors per pass for this form (jkpm_ell_set_pass), one pass per group of
+// g vectors.  A pass copies its columns of v_r (and of left sets other than
+// v_r) to contiguous [N][g] arrays and writes its column slice of mu.  The
+// stack form always runs at once.
+static int run_moments(const Ell &H, const void *vr, const void *vl0, const void *vl1, int R,
+                       int M, double2 *mu, void *ws, size_t ws_bytes, hipStream_t s,
+                       const Groups *grp = nullptr, bool doubling = false,
+                       void *stack = nullptr, bool no_alloc = false) {
+    if (stack || pass_vectors(H, R, doubling) >= R)
+        return run_moments_once(H, vr, vl0, vl1, R, M, mu, ws, ws_bytes, s, grp, doubling, stack,
+                                no_alloc);
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const int n_left = (vl1 || doubling) ? 2 : 1;
+    const PassLayout P = pass_layout(H, R, n_left, dev, grp, doubling ? M : 0);
+    if (ws_bytes < P.total) {
+        set_error("workspace too small: need " + std::to_string(P.total) + " bytes, got " +
+                  std::to_string(ws_bytes));
+        return JKPM_ERR_WORKSPACE;
+    }
+    char *w = (char *)ws;
+    const int cb = (int)cplx_bytes(H.vtype);
+    void *pv = w, *pl0 = w + P.vec, *pl1 = w + 2 * P.vec;
+    for (int p0 = 0; p0 < R; p0 += P.g) {
+        const int gc = std::min(P.g, R - p0);
+        rc = launch_take_columns(vr, H.n, R, p0, gc, cb, pv, s);
+        const void *l0 = pv, *l1 = nullptr;
+        if (!rc && !doubling && vl0 != vr) {
+            rc = launch_take_columns(vl0, H.n, R, p0, gc, cb, pl0, s);
+            l0 = pl0;
+        }
+        if (!rc && !doubling && vl1) {
+            rc = launch_take_columns(vl1, H.n, R, p0, gc, cb, pl1, s);
+            l1 = pl1;
+        }
+        if (!rc)
+            rc = run_moments_once(H, pv, l0, l1, gc, M, mu + p0, w + P.inner, ws_bytes - P.inner,
+                                  s, grp, doubling, nullptr, no_alloc, R);
+        if (rc) return rc;
+    }
     return JKPM_OK;
 }
 
@@ -1142,7 +1227,7 @@ int jkpm_moments_workspace(const jkpm_ell *h, int R, int n_left, size_t *bytes) 
     DeviceInfo dev;
     int rc = device_info(&dev);
     if (rc) return rc;
-    *bytes = moments_layout(h->e, R, n_left, dev).total;
+    *bytes = moments_bytes(h->e, R, n_left, dev);
     return JKPM_OK;
 }
 
@@ -1178,9 +1263,9 @@ int jkpm_moments_noalloc(const jkpm_ell *h, const void *v_r, const void *v_l0, c
     DeviceInfo dev;
     int rc = device_info(&dev);
     if (rc) return rc;
-    const MomentsLayout L = moments_layout(h->e, R, v_l1 ? 2 : 1, dev);
-    if (L.total > JKPM_NOALLOC_SCRATCH_BYTES) {
-        set_error("jkpm_moments_noalloc: this matrix and R need " + std::to_string(L.total) +
+    const size_t need = moments_bytes(h->e, R, v_l1 ? 2 : 1, dev);
+    if (need > JKPM_NOALLOC_SCRATCH_BYTES) {
+        set_error("jkpm_moments_noalloc: this matrix and R need " + std::to_string(need) +
                   " bytes of scratch, more than the " +
                   std::to_string((size_t)JKPM_NOALLOC_SCRATCH_BYTES) +
                   " the library holds per GPU; use jkpm_moments with a workspace");
@@ -1237,6 +1322,23 @@ int jkpm_ell_set_launch(jkpm_ell *h, int variant, int chunk_mult, int flags) {
     return JKPM_OK;
 }
 
+int jkpm_ell_set_pass(jkpm_ell *h, int vectors, int flags) {
+    clear_error();
+    if (!h) return arg_error("handle is NULL");
+    if (vectors < 0) return arg_error("vectors per pass must be >= 0 (0: all at once)");
+    if (flags & ~JKPM_MOMENTS_DOUBLING) return arg_error("unknown flags");
+    (flags & JKPM_MOMENTS_DOUBLING ? h->e.dpass_r : h->e.pass_r) = vectors;
+    return JKPM_OK;
+}
+
+int jkpm_ell_get_pass(const jkpm_ell *h, int *vectors, int flags) {
+    clear_error();
+    if (!h || !vectors) return arg_error("NULL argument");
+    if (flags & ~JKPM_MOMENTS_DOUBLING) return arg_error("unknown flags");
+    *vectors = flags & JKPM_MOMENTS_DOUBLING ? h->e.dpass_r : h->e.pass_r;
+    return JKPM_OK;
+}
+
 int jkpm_ell_get_launch(const jkpm_ell *h, int *variant, int *chunk_mult, int flags) {
     clear_error();
     if (!h) return arg_error("handle is NULL");
@@ -1269,7 +1371,7 @@ int jkpm_autotune_workspace(const jkpm_ell *h, int R, int steps, int flags, size
     for (int m : kTuneMult) {
         probe.chunk_mult = m;
         probe.dchunk_mult = m;
-        need = std::max(need, moments_layout(probe, R, 1, dev, nullptr, dbl ? M : 0).total);
+        need = std::max(need, moments_bytes(probe, R, 1, dev, nullptr, dbl ? M : 0));
     }
     *bytes = need + align_up((size_t)M * R * sizeof(double2), 256);
     return JKPM_OK;
@@ -1458,7 +1560,7 @@ int jkpm_batch_workspace(const jkpm_batch *b, int R, int n_left, size_t *bytes) 
     int rc = device_info(&dev);
     if (rc) return rc;
     const Groups g = b->grp();
-    *bytes = moments_layout(b->h->e, R, n_left, dev, &g).total;
+    *bytes = moments_bytes(b->h->e, R, n_left, dev, &g);
     return JKPM_OK;
 }
 
@@ -1526,9 +1628,9 @@ int jkpm_moments_ex_workspace(const jkpm_ell *h, const jkpm_batch *b, int R, int
     const bool dbl = (flags & JKPM_MOMENTS_DOUBLING) != 0;
     if (b) {
         const Groups g = b->grp(dbl);
-        *bytes = moments_layout(b->h->e, R, 1, dev, &g, dbl ? M : 0).total;
+        *bytes = moments_bytes(b->h->e, R, 1, dev, &g, dbl ? M : 0);
     } else {
-        *bytes = moments_layout(h->e, R, 1, dev, nullptr, dbl ? M : 0).total;
+        *bytes = moments_bytes(h->e, R, 1, dev, nullptr, dbl ? M : 0);
     }
     return JKPM_OK;
 }

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(const double2 *__restric
                                                          int n_left, int nchunks, int ld,
                                                          double2 *__restrict__ mu, int M,
                                                          const int *__restrict__ group_chunks,
-                                                         int G) {
+                                                         int G, int mu_ld) {
     const int r = blockIdx.x, l = blockIdx.y / G, g = blockIdx.y % G;
     const int step = step_first + blockIdx.z;
     const double2 *p =
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(const double2 *__restric
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        mu[(((long)l * G + g) * M + step) * ld + r] = make_double2(shr[0], shi[0]);
+        mu[(((long)l * G + g) * M + step) * mu_ld + r] = make_double2(shr[0], shi[0]);
 }
 
 // The same sums with one work-group per (left set x group, step) for all
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const double2 *__re
                                                               int n_left, int nchunks, int ld,
                                                               double2 *__restrict__ mu, int M,
                                                               const int *__restrict__ group_chunks,
-                                                              int G) {
+                                                              int G, int mu_ld) {
     const int l = blockIdx.x / G, g = blockIdx.x % G;
     const int step = step_first + blockIdx.y;
     const double2 *p = ring + ((long)(step % ring_slots) * n_left + l) * (long)nchunks * ld;
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const double2 *__re
             ar += shr[c * ld + t];
             ai += shi[c * ld + t];
         }
-        mu[(((long)l * G + g) * M + step) * ld + t] = make_double2(ar, ai);
+        mu[(((long)l * G + g) * M + step) * mu_ld + t] = make_double2(ar, ai);
     }
 }
 
@@ -161,7 +161,7 @@ __global__ void aaxby_kernel(const Real *__restrict__ val, const int *__restrict
 
 // Moments from the doubling form's dots (see launch_doubling_finish).
 __global__ void doubling_finish_kernel(const double2 *__restrict__ raw, int G, int S1, int R,
-                                       int M, double2 *__restrict__ mu) {
+                                       int M, double2 *__restrict__ mu, int mu_ld) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)G * M * R) return;
     const int r = (int)(t % R);
@@ -180,7 +180,21 @@ __global__ void doubling_finish_kernel(const double2 *__restrict__ raw, int G, i
         const double2 b = at(1, (m - 1) / 2 + 1), b0 = at(1, 1);
         v = make_double2(2.0 * b.x - b0.x, 2.0 * b.y - b0.y);
     }
-    mu[t] = v;
+    mu[((long)g * M + m) * mu_ld + r] = v;
+}
+
+// Columns c0 .. c0+cols-1 of an [n][ld] array of `u` 8-byte words per
+// element, made contiguous ([n][cols]).
+__global__ __launch_bounds__(kBlock) void take_columns_kernel(const unsigned long long *__restrict__ src,
+                                                              long n, int ld, int c0, int cols,
+                                                              int u,
+                                                              unsigned long long *__restrict__ dst) {
+    const long w = (long)cols * u;  // words per destination row
+    const long tot = n * w;
+    for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < tot; t += (long)gridDim.x * kBlock) {
+        const long i = t / w, j = t - i * w;
+        dst[t] = src[i * ld * u + (long)c0 * u + j];
+    }
 }
 
 // Plain copy with kCopyUnroll independent 16-byte loads in flight per lane
@@ -300,17 +314,19 @@ int launch_step(const StepArgs &a, hipStream_t s) {
 
 int launch_reduce(const double2 *ring, int ring_slots, int step_first, int nsteps, int n_left,
                   int nchunks, int ld, double2 *mu, int M, const int *group_chunks, int G,
-                  hipStream_t s) {
+                  hipStream_t s, int mu_ld) {
     if (nsteps <= 0) return JKPM_OK;
     if (G < 1) G = 1;
+    if (mu_ld <= 0) mu_ld = ld;
     if (ld <= kBlock) {
         hipLaunchKernelGGL(reduce_rows_kernel, dim3(n_left * G, nsteps), dim3(kBlock), 0, s, ring,
-                           ring_slots, step_first, n_left, nchunks, ld, mu, M, group_chunks, G);
+                           ring_slots, step_first, n_left, nchunks, ld, mu, M, group_chunks, G,
+                           mu_ld);
         JKPM_CHECK_LAUNCH("reduce_rows_kernel");
         return JKPM_OK;
     }
     hipLaunchKernelGGL(reduce_kernel, dim3(ld, n_left * G, nsteps), dim3(kBlock), 0, s, ring,
-                       ring_slots, step_first, n_left, nchunks, ld, mu, M, group_chunks, G);
+                       ring_slots, step_first, n_left, nchunks, ld, mu, M, group_chunks, G, mu_ld);
     JKPM_CHECK_LAUNCH("reduce_kernel");
     return JKPM_OK;
 }
@@ -347,12 +363,30 @@ int launch_aaxby(const Ell &H, double2 a, double2 b, const void *x, void *y, int
 }
 
 int launch_doubling_finish(const double2 *raw, int G, int S1, int R, int M, double2 *mu,
-                           hipStream_t s) {
+                           hipStream_t s, int mu_ld) {
     const long tot = (long)G * M * R;
     if (tot == 0) return JKPM_OK;
+    if (mu_ld <= 0) mu_ld = R;
     hipLaunchKernelGGL(doubling_finish_kernel, dim3((unsigned)((tot + kBlock - 1) / kBlock)),
-                       dim3(kBlock), 0, s, raw, G, S1, R, M, mu);
+                       dim3(kBlock), 0, s, raw, G, S1, R, M, mu, mu_ld);
     JKPM_CHECK_LAUNCH("doubling_finish_kernel");
+    return JKPM_OK;
+}
+
+int launch_take_columns(const void *src, long n, int ld, int c0, int cols, int elem_bytes,
+                        void *dst, hipStream_t s) {
+    const int u = elem_bytes / 8;
+    const long tot = n * cols * u;
+    if (tot == 0) return JKPM_OK;
+    DeviceInfo dev;
+    int rc = device_info(&dev);
+    if (rc) return rc;
+    const long want = (tot + kBlock - 1) / kBlock;
+    const int grid = (int)std::min(want, (long)dev.cu_count * 16);
+    hipLaunchKernelGGL(take_columns_kernel, dim3(grid), dim3(kBlock), 0, s,
+                       (const unsigned long long *)src, n, ld, c0, cols, u,
+                       (unsigned long long *)dst);
+    JKPM_CHECK_LAUNCH("take_columns_kernel");
     return JKPM_OK;
 }
 

@@ -65,6 +65,12 @@ struct Ell {
     // the same for the doubling form of the recursion
     int dvariant = 0;
     int dchunk_mult = 8;
+    // Vectors per recursion pass (jkpm_ell_set_pass; 0 = all R at once): a
+    // call with more vectors runs the whole recursion for each group of
+    // pass_r of them in turn, so one pass's vectors and the matrix can stay
+    // in the 256 MB Infinity Cache.  Plain form and doubling form.
+    int pass_r = 0;
+    int dpass_r = 0;
     void *val = nullptr;   // [k][n] complex of vtype
     int *col = nullptr;    // [k][n]
     int device = 0;
@@ -110,8 +116,10 @@ struct StepArgs {
 // mu[g][m][r] from the doubling form's raw dots raw[l][g][s][r] (s < S1):
 // mu_0 = a_0, mu_1 = b_0, mu_2k = 2 a_k - a_0, mu_2k+1 = 2 b_k - b_0 with
 // a_k at (l=0, s=k) and b_k at (l=1, s=k+1).
+// mu_ld: the row length of mu (R when 0; larger when mu is a column slice
+// of a wider moment array).
 int launch_doubling_finish(const double2 *raw, int G, int S1, int R, int M, double2 *mu,
-                           hipStream_t s);
+                           hipStream_t s, int mu_ld = 0);
 
 // Number of row chunks (work-groups) used for a matrix of n rows.
 int step_chunks(const Ell &H, int rb, const DeviceInfo &dev, bool doubling = false);
@@ -128,10 +136,16 @@ int launch_step(const StepArgs &a, hipStream_t s);
 
 // mu[l][g][step][r] = sum over the chunks of group g (group_chunks[g] ..
 // group_chunks[g+1]; all chunks and G = 1 when group_chunks is null) of the
-// ring partials, for steps [step_first, step_first + nsteps).
+// ring partials, for steps [step_first, step_first + nsteps).  mu rows are
+// mu_ld long (ld when 0).
 int launch_reduce(const double2 *ring, int ring_slots, int step_first, int nsteps,
                   int n_left, int nchunks, int ld, double2 *mu, int M,
-                  const int *group_chunks, int G, hipStream_t s);
+                  const int *group_chunks, int G, hipStream_t s, int mu_ld = 0);
+
+// dst[i][c] = src[i][c0 + c] for i < n, c < cols: columns c0 .. c0+cols-1 of
+// an [n][ld] array of elem_bytes-sized (8 or 16) elements, made contiguous.
+int launch_take_columns(const void *src, long n, int ld, int c0, int cols, int elem_bytes,
+                        void *dst, hipStream_t s);
 
 int launch_aaxby(const Ell &H, double2 a, double2 b, const void *x, void *y, int R,
                  hipStream_t s);

@@ -461,6 +461,9 @@ def test_tuned_times_short_steps_longer(tmp_path, monkeypatch):
         def set_launch(self, v, m, doubling=False):
             self.launch = (v, m)
 
+        def set_pass(self, g, doubling=False):
+            pass
+
     class Vecs:
         shape = (100, 4)
 
@@ -476,6 +479,61 @@ def test_tuned_times_short_steps_longer(tmp_path, monkeypatch):
     assert h.calls == [8, 256]
     h = Handle(25.0)
     assert autotune.tuned(h, Vecs()) == (5, 4) and h.calls == []
+
+
+def test_tuned_picks_vectors_per_pass(tmp_path, monkeypatch):
+    """autotune.tuned tries all R vectors at once and passes of R/2, R/4, ...
+    down to 4 vectors, tunes the launch shape for each, keeps the fastest
+    whole step, stores the pass size in the table and applies it again from
+    the table."""
+    from minijclsquant_amd import autotune
+
+    assert autotune.pass_candidates(16) == [0, 8, 4]
+    assert autotune.pass_candidates(8) == [0, 4]
+    assert autotune.pass_candidates(4) == [0]
+    assert autotune.pass_candidates(1) == [0]
+    assert autotune.pass_candidates(40) == [0, 20, 10, 5]
+    monkeypatch.setenv("JKPM_TUNE_TABLE", str(tmp_path / "table.json"))
+    monkeypatch.delenv("JKPM_LAUNCH", raising=False)
+    monkeypatch.setattr(autotune, "_key", lambda h, R, d: f"k|R={R}|{d}")
+    monkeypatch.setattr(autotune, "_finalists", lambda h, v_r, v, m, us, d: (v, us))
+    # the passes are compared by their per-step slope, not the sweep's time
+    monkeypatch.setattr(autotune, "_step_slope", lambda h, v_r, d, us: slope[h.g])
+    step_us = {0: 180.0, 8: 190.0, 4: 185.0}
+    slope = {0: 180.0, 8: 190.0, 4: 170.0}
+
+    class Handle:
+        def __init__(self):
+            self.g, self.launch, self.passes = 0, (0, 8), []
+
+        def set_pass(self, g, doubling=False):
+            self.g = g
+            self.passes.append(g)
+
+        def autotune(self, v_r, steps, doubling=False):
+            self.launch = (10 + self.g, 4)
+            return step_us[self.g]
+
+        def get_launch(self, doubling=False):
+            return self.launch
+
+        def set_launch(self, v, m, doubling=False):
+            self.launch = (v, m)
+
+    class Vecs:
+        shape = (100, 16)
+
+    h = Handle()
+    assert autotune.tuned(h, Vecs()) == (14, 4)
+    assert h.passes == [0, 8, 4, 4] and h.g == 4
+    h2 = Handle()
+    assert autotune.tuned(h2, Vecs()) == (14, 4)
+    assert h2.passes == [4]
+    # less than 2 % faster per step: all vectors at once stay
+    slope[4] = 177.0
+    h3 = Handle()
+    assert autotune.tuned(h3, Vecs(), retune=True) == (10, 4)
+    assert h3.g == 0
 
 
 def test_finalist_neighbours_flip_one_choice():
@@ -638,6 +696,9 @@ def test_bench_reads_traffic_from_committed_profiles():
         "headline": bench.traffic_from_profile(2097152, 4, 16),
         "headline_dbl": bench.traffic_from_profile(2097152, 4, 16,
                                                    "doubling_traffic_bytes_per_launch"),
+        "headline_pass4": bench.traffic_from_profile(2097152, 4, 16, g=4),
+        "headline_dbl_pass4": bench.traffic_from_profile(2097152, 4, 16,
+                                                         "doubling_traffic_bytes_per_launch", g=4),
         "batch": bench.profile_traffic(P / "cheb_step_c64_r8_batch64_square_trace.md",
                                        "traffic_bytes_per_launch", N=24054938, K=5, R=8),
         "config1": bench.profile_traffic(P / "cheb_step_c128_r4_square256_pmc.md",
@@ -654,6 +715,8 @@ def test_bench_reads_traffic_from_committed_profiles():
     # column; the complex64 plain forms read their random-phase left vectors
     # as 4-byte phases); counters sit 0-7 % above
     alg = {"headline": 2097152 * (4 * 8 + 16 * 28), "headline_dbl": 2097152 * (4 * 8 + 16 * 24),
+           "headline_pass4": 2097152 * (4 * 8 + 4 * 28),
+           "headline_dbl_pass4": 2097152 * (4 * 8 + 4 * 24),
            "batch": 24054938 * (5 * 8 + 8 * 28), "config1": 65536 * (5 * 12 + 4 * 64),
            "grad": 262144 * (5 * 12 + 8 * 64), "kubo_gemm": 2 * 512 * 2097152 * 8,
            "kubo_step": 524288 * 4 * 20 + 4 * 524288 * 8 * 7}
