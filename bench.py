@@ -91,9 +91,37 @@ def profile_traffic(path, key, **shape):
     return float(m.group(1))
 
 
-def traffic_from_profile(N, K, R, key="traffic_bytes_per_launch"):
-    """The headline workload's traffic figure (see profile_traffic)."""
+def traffic_from_profile(N, K, R, key="traffic_bytes_per_launch", g=None):
+    """The headline workload's traffic figure (see profile_traffic); for
+    passes of g < R vectors, the figure of one such pass's launch."""
+    if g is not None and g < R:
+        key = f"{key}_pass{g}"
     return profile_traffic(TRAFFIC_FILE, key, N=N, K=K, R=R)
+
+
+def pass_note(N, K, R, g, mv, s_v, s_l, launch_ms):
+    """Roofline fields for a recursion run in passes of g < R vectors: the
+    bytes one step of all R vectors moves (the matrix once per pass) and
+    their rate, and the bytes of the same step at R vectors per pass (the
+    matrix once), over the same time."""
+    per_vec = 3 * s_v + (s_l if s_l else 0)
+    passes = -(-R // g)
+    step_s = launch_ms / 1e3 * R / g
+    all_bytes = passes * N * K * (mv + 4) + R * N * per_vec
+    one_bytes = N * K * (mv + 4) + R * N * per_vec
+    return {
+        "passes_per_step": passes,
+        "step_bytes_all_passes": all_bytes,
+        "step_GBs_all_passes": round(all_bytes / step_s / 1e9, 1),
+        "single_pass_bytes": one_bytes,
+        "single_pass_equiv_GBs": round(one_bytes / step_s / 1e9, 1),
+        "single_pass_equiv_frac": round(one_bytes / step_s / 1e9 / HBM_PEAK_GBS, 4),
+        "pass_note": (f"the {R} vectors run as {passes} recursions of {g}: one pass's "
+                      f"vectors and the matrix ({(N * K * (mv + 4) + 2 * g * N * s_v + g * N * (s_l or 0)) / 1e6:.0f} MB) "
+                      "stay in the 256 MB Infinity Cache, so a launch reads much of its "
+                      "algorithmic bytes from there; single_pass_equiv_* counts the bytes of "
+                      "one pass of all R vectors over the same time"),
+    }
 
 
 def copy_ceiling(torch, nat, nbytes=2 << 30, reps=5):
@@ -1026,7 +1054,10 @@ def main():
     # complex64: the random-phase left vectors (v_l = v_r) are read as 32-bit phases
     phases = args.dtype == "c64" and nat.phase_pack(vr)[1]
     s_l = 4 if phases else s_v
-    bytes_launch = N * K * (mv + 4) + R * N * (3 * s_v + s_l)
+    # vectors per recursion pass (the tuner may run the R vectors in passes)
+    g = h.get_pass() or R
+    g = min(g, R)
+    bytes_launch = N * K * (mv + 4) + g * N * (3 * s_v + s_l)
     achieved = bytes_launch / (launch_ms / 1e3) / 1e9
     roofline = {
         "bound": "hbm",
@@ -1034,16 +1065,19 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_from_profile(N, K, R),
+        "traffic": traffic_from_profile(N, K, R, g=g),
         "kernel": "cheb_step_kernel",
         "algorithmic_bytes_per_launch": bytes_launch,
         "matrix_value_bytes_per_slot": mv,
         "left_vector_bytes_per_element": s_l,
+        "vectors_per_pass": g,
         "mean_launch_us": round(launch_ms * 1e3, 2),
         "launches_timed": n_prof,
         "copy_kernel_GBs": round(copy_gbs, 1),
         "frac_of_copy": round(achieved / copy_gbs, 4),
     }
+    if g < R:
+        roofline.update(pass_note(N, K, R, g, mv, s_v, s_l, launch_ms))
 
     d128 = None
     dos_err = None
@@ -1063,7 +1097,8 @@ def main():
         torch.cuda.synchronize()
         ms_d, launch_d, n_d, dos_d = timed(True)
         log(f"doubling form: {ms_d:.2f} ms per step")
-        bytes_d = N * K * (mv + 4) + R * N * 3 * s_v
+        gd = min(h.get_pass(doubling=True) or R, R)
+        bytes_d = N * K * (mv + 4) + gd * N * 3 * s_v
         ach_d = bytes_d / (launch_d / 1e3) / 1e9
         dbl = {
             "value": units / (ms_d / 1e3),
@@ -1074,13 +1109,17 @@ def main():
             "roofline": {"bound": "hbm", "achieved": round(ach_d, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(ach_d / HBM_PEAK_GBS, 4),
                          "traffic": traffic_from_profile(N, K, R,
-                                                         "doubling_traffic_bytes_per_launch"),
+                                                         "doubling_traffic_bytes_per_launch",
+                                                         g=gd),
                          "algorithmic_bytes_per_launch": bytes_d,
+                         "vectors_per_pass": gd,
                          "mean_launch_us": round(launch_d * 1e3, 2), "launches_timed": n_d},
             "dos_err_vs_c128": (None if d128 is None else
                                 float(np.max(np.abs(dos_d.cpu().numpy() - d128))
                                       / np.max(np.abs(d128)))),
         }
+        if gd < R:
+            dbl["roofline"].update(pass_note(N, K, R, gd, mv, s_v, 0, launch_d))
 
     complex_left = None
     if phases and not args.no_complex_left:
@@ -1092,17 +1131,18 @@ def main():
         torch.cuda.synchronize()
         ms_c, launch_c, n_c, _ = timed(False)
         shape_c = list(h.get_launch())
+        g_c = min(h.get_pass() or R, R)
         h.use_left_phases(True)
         if not args.no_tune:
             autotune.tuned(h, vr)
-        bytes_c = N * K * (mv + 4) + R * N * 4 * s_v
+        bytes_c = N * K * (mv + 4) + g_c * N * 4 * s_v
         ach_c = bytes_c / (launch_c / 1e3) / 1e9
         complex_left = {
             "value": units / (ms_c / 1e3), "ms_per_step": ms_c,
             "launch": shape_c,
             "roofline": {"bound": "hbm", "achieved": round(ach_c, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(ach_c / HBM_PEAK_GBS, 4),
-                         "algorithmic_bytes_per_launch": bytes_c,
+                         "algorithmic_bytes_per_launch": bytes_c, "vectors_per_pass": g_c,
                          "mean_launch_us": round(launch_c * 1e3, 2), "launches_timed": n_c},
             "headline_over_it": value / (units / (ms_c / 1e3)),
             "note": "the plain recursion with v_l read as complex64 pairs (8 bytes per "
