@@ -57,6 +57,8 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(const double2 *__restric
     const int c_begin = group_chunks ? group_chunks[g] : 0;
     const int c_end = group_chunks ? group_chunks[g + 1] : nchunks;
     double sr = 0.0, si = 0.0;
+    // unrolled: several partials in flight per thread, added in the same order
+#pragma unroll 8
     for (int c = c_begin + threadIdx.x; c < c_end; c += kBlock) {
         const double2 q = p[(long)c * ld];
         sr += q.x;
@@ -98,6 +100,8 @@ __global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const double2 *__re
     const int r = t % ld, c0 = t / ld;
     double sr = 0.0, si = 0.0;
     if (c0 < rows) {
+        // unrolled: several partials in flight per thread, added in the same order
+#pragma unroll 8
         for (int c = c_begin + c0; c < c_end; c += rows) {
             const double2 q = p[(long)c * ld + r];
             sr += q.x;
