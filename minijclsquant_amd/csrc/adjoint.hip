@@ -36,7 +36,9 @@ __device__ __forceinline__ int xcd_chunk(int b, int nchunks) {
     return (nchunks % kXcd == 0) ? (b % kXcd) * (nchunks / kXcd) + b / kXcd : b;
 }
 
-template <typename Real, int VPL, int LPR>
+// RV: val holds one real per slot (the handle's packed real copy of a matrix
+// whose imaginary parts are all 0), so each product takes two multiplies.
+template <typename Real, int VPL, int LPR, bool RV = false>
 __global__ __launch_bounds__(kBlock) void plain_step_kernel(
     const Real *__restrict__ val, const int *__restrict__ col, int n, int k,
     const Real *__restrict__ x, const Real *prev, Real *out, const double2 *__restrict__ coef,
@@ -63,12 +65,21 @@ __global__ __launch_bounds__(kBlock) void plain_step_kernel(
         for (int s = 0; s < k; ++s) {
             const long e = (long)s * n + row;
             const int j = col[e];
-            const C2 h = *reinterpret_cast<const C2 *>(val + 2 * e);
             const V xv = *reinterpret_cast<const V *>(x + (long)j * ld * 2 + voff);
+            if constexpr (RV) {
+                const Real h = val[e];
 #pragma unroll
-            for (int v = 0; v < VPL; ++v) {
-                sr[v] += h.x * xv[2 * v] - h.y * xv[2 * v + 1];
-                si[v] += h.x * xv[2 * v + 1] + h.y * xv[2 * v];
+                for (int v = 0; v < VPL; ++v) {
+                    sr[v] += h * xv[2 * v];
+                    si[v] += h * xv[2 * v + 1];
+                }
+            } else {
+                const C2 h = *reinterpret_cast<const C2 *>(val + 2 * e);
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    sr[v] += h.x * xv[2 * v] - h.y * xv[2 * v + 1];
+                    si[v] += h.x * xv[2 * v + 1] + h.y * xv[2 * v];
+                }
             }
         }
         const long vo = row * ld * 2 + voff;
@@ -232,6 +243,9 @@ __global__ void zero_tail_kernel(double2 *__restrict__ p, long cnt, int tail8) {
 template <typename Real, int VPL, int LPR>
 struct Fns {
     static const void *step() { return reinterpret_cast<const void *>(&plain_step_kernel<Real, VPL, LPR>); }
+    static const void *step_rv() {
+        return reinterpret_cast<const void *>(&plain_step_kernel<Real, VPL, LPR, true>);
+    }
     static const void *fold(int k) {
         switch (k) {
             case 4: return reinterpret_cast<const void *>(&grad_fold_kernel<Real, VPL, LPR, 4>);
@@ -244,6 +258,7 @@ struct Fns {
 struct Pick {
     const void *step, *fold;
     int lpr;
+    const void *step_rv = nullptr;  // the step on the packed real values
 };
 
 // wide (an even batch R, so rows and groups start on two-vector
@@ -254,21 +269,21 @@ template <typename Real>
 Pick pick_real(int rb, int k, bool wide) {
     if (wide) {
         switch (rb) {
-            case 2: return {Fns<Real, 2, 1>::step(), Fns<Real, 2, 1>::fold(k), 1};
-            case 4: return {Fns<Real, 2, 2>::step(), Fns<Real, 2, 2>::fold(k), 2};
-            case 8: return {Fns<Real, 2, 4>::step(), Fns<Real, 2, 4>::fold(k), 4};
-            case 16: return {Fns<Real, 2, 8>::step(), Fns<Real, 2, 8>::fold(k), 8};
-            case 32: return {Fns<Real, 2, 16>::step(), Fns<Real, 2, 16>::fold(k), 16};
+            case 2: return {Fns<Real, 2, 1>::step(), Fns<Real, 2, 1>::fold(k), 1, Fns<Real, 2, 1>::step_rv()};
+            case 4: return {Fns<Real, 2, 2>::step(), Fns<Real, 2, 2>::fold(k), 2, Fns<Real, 2, 2>::step_rv()};
+            case 8: return {Fns<Real, 2, 4>::step(), Fns<Real, 2, 4>::fold(k), 4, Fns<Real, 2, 4>::step_rv()};
+            case 16: return {Fns<Real, 2, 8>::step(), Fns<Real, 2, 8>::fold(k), 8, Fns<Real, 2, 8>::step_rv()};
+            case 32: return {Fns<Real, 2, 16>::step(), Fns<Real, 2, 16>::fold(k), 16, Fns<Real, 2, 16>::step_rv()};
             default: break;
         }
     }
     switch (rb) {
-        case 1: return {Fns<Real, 1, 1>::step(), Fns<Real, 1, 1>::fold(k), 1};
-        case 2: return {Fns<Real, 1, 2>::step(), Fns<Real, 1, 2>::fold(k), 2};
-        case 4: return {Fns<Real, 1, 4>::step(), Fns<Real, 1, 4>::fold(k), 4};
-        case 8: return {Fns<Real, 1, 8>::step(), Fns<Real, 1, 8>::fold(k), 8};
-        case 16: return {Fns<Real, 1, 16>::step(), Fns<Real, 1, 16>::fold(k), 16};
-        case 32: return {Fns<Real, 1, 32>::step(), Fns<Real, 1, 32>::fold(k), 32};
+        case 1: return {Fns<Real, 1, 1>::step(), Fns<Real, 1, 1>::fold(k), 1, Fns<Real, 1, 1>::step_rv()};
+        case 2: return {Fns<Real, 1, 2>::step(), Fns<Real, 1, 2>::fold(k), 2, Fns<Real, 1, 2>::step_rv()};
+        case 4: return {Fns<Real, 1, 4>::step(), Fns<Real, 1, 4>::fold(k), 4, Fns<Real, 1, 4>::step_rv()};
+        case 8: return {Fns<Real, 1, 8>::step(), Fns<Real, 1, 8>::fold(k), 8, Fns<Real, 1, 8>::step_rv()};
+        case 16: return {Fns<Real, 1, 16>::step(), Fns<Real, 1, 16>::fold(k), 16, Fns<Real, 1, 16>::step_rv()};
+        case 32: return {Fns<Real, 1, 32>::step(), Fns<Real, 1, 32>::fold(k), 32, Fns<Real, 1, 32>::step_rv()};
         default: return {nullptr, nullptr, 0};
     }
 }
@@ -294,8 +309,10 @@ int launch_plain_step(const Ell &H, const void *x, const void *prev, void *out,
         while (rb > R - r0) rb >>= 1;
         Pick p = pick(H.vtype, rb, H.k, R % 2 == 0);
         if (!p.step) return arg_error("unsupported vector group");
+        // a real matrix: the step on its packed real values
+        const bool rv = H.real_values && H.rval && p.step_rv;
         const size_t off = (size_t)r0 * cb;
-        const void *a_val = H.val;
+        const void *a_val = rv ? H.rval : H.val;
         const int *a_col = H.col;
         int a_n = H.n, a_k = H.k;
         const void *a_x = (const char *)x + off;
@@ -311,7 +328,7 @@ int launch_plain_step(const Ell &H, const void *x, const void *prev, void *out,
                           &a_u,   &a_ld,  &fa,  &fb,  &a_rpc, &a_nch};
         void *args_d[] = {&a_val, &a_col, &a_n, &a_k, &a_x, &a_prev, &a_out, &a_coef,
                           &a_u,   &a_ld,  &da,  &db,  &a_rpc, &a_nch};
-        JKPM_HIP(hipLaunchKernel(p.step, dim3(nchunks), dim3(kBlock),
+        JKPM_HIP(hipLaunchKernel(rv ? p.step_rv : p.step, dim3(nchunks), dim3(kBlock),
                                  H.vtype == JKPM_C64 ? args_f : args_d, 0, s));
         r0 += rb;
     }

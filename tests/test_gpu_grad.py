@@ -1,7 +1,9 @@
 """Adjoint Chebyshev gradients on the GPU.
 
 * jkpm_moments_grad vs the oracle's FP64 reverse-mode restatement
-  (oracle.grad_values, itself pinned by finite differences): <= 1e-10 relative.
+  (oracle.grad_values, itself pinned by finite differences): <= 1e-10
+  relative, with the step kernels on the packed real values and on the
+  complex values.
 * Independence of the checkpoint spacing (sqrt(M) default, 1, 3, M).
 * torch.autograd through chebyshev_moments + kpm_dos_torch on real onsite
   energies and bond hoppings vs central finite differences (h = 1e-6,
@@ -41,11 +43,15 @@ def test_grad_values_vs_oracle(torch_gpu):
     gd = torch.from_numpy(g).cuda()
     want = sum(O.grad_values(M, A.data * (1 / B), A.indices, K, r[k], np.conj(g[:, k]))
                for k in range(R)) * (1 / B)
-    # checkpointed (0, 1, 3, M) and keep-every-T (-1) forms
-    for seg in (0, 1, 3, M, -1):
-        got = h.grad_values(rd, M, gd, seg=seg).cpu().numpy()
-        err = np.max(np.abs(got - want)) / np.max(np.abs(want))
-        assert err <= 1e-10, (seg, err)
+    # checkpointed (0, 1, 3, M) and keep-every-T (-1) forms, on the packed
+    # real values (the default for this real matrix) and on the complex ones
+    assert h.is_real()
+    for use_real in (True, False):
+        h.use_real_values(use_real)
+        for seg in (0, 1, 3, M, -1):
+            got = h.grad_values(rd, M, gd, seg=seg).cpu().numpy()
+            err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+            assert err <= 1e-10, (use_real, seg, err)
 
 
 @pytest.mark.parametrize("M", [1, 2, 3, 17])
