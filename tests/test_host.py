@@ -727,3 +727,27 @@ def test_bench_reads_traffic_from_committed_profiles():
     assert bench.traffic_from_profile(2097152, 4, 8) is None
     assert bench.profile_traffic(P / "cheb_step_c128_r4_square256_pmc.md",
                                  "traffic_bytes_per_launch", N=65536, K=5, R=8) is None
+
+
+def test_bench_pass_roofline_fields():
+    """bench.pass_note for R vectors in passes of g: a step moves the matrix
+    once per pass plus every vector stream once, and the single-pass figure
+    counts the matrix once over the same step time (R/g launches)."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("bench_mod2", ROOT / "bench.py")
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    N, K, R, g, mv, s_v, s_l = 2097152, 4, 16, 4, 4, 8, 4
+    launch_ms = 0.04274
+    f = bench.pass_note(N, K, R, g, mv, s_v, s_l, launch_ms)
+    assert f["passes_per_step"] == 4
+    assert f["step_bytes_all_passes"] == 4 * N * K * 8 + R * N * 28
+    assert f["single_pass_bytes"] == N * K * 8 + R * N * 28 == 1006632960
+    step_s = 4 * launch_ms / 1e3
+    assert abs(f["single_pass_equiv_GBs"] - 1006632960 / step_s / 1e9) < 0.1
+    assert abs(f["single_pass_equiv_frac"] - 1006632960 / step_s / 8e12) < 1e-4
+    # doubling form: no left stream; a last pass shorter than g counts as a pass
+    d = bench.pass_note(N, K, 10, 4, mv, s_v, 0, 0.03)
+    assert d["passes_per_step"] == 3
+    assert d["step_bytes_all_passes"] == 3 * N * K * 8 + 10 * N * 24
