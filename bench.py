@@ -486,12 +486,12 @@ def bench_batch(args, torch, jcl, nat, _kpm):
     # the random-phase left vectors (v_l = v_r) are read as 32-bit phases
     s_l = 4 if nat.phase_pack(bk.vr)[1] else 8
     # vectors per recursion pass of each form (all R unless the tuner split them)
-    g = min(bk.H.get_pass() or R, R)
-    gd = min(bk.H.get_pass(doubling=True) or R, R)
-    bytes_launch = N * K * (mv + 4) + g * N * (3 * 8 + s_l)
-    bytes_d = N * K * (mv + 4) + gd * N * 3 * 8
-    ach = bytes_launch * (M - 1) * (-(-R // g)) / (ms / 1e3) / 1e9
-    ach_d = bytes_d * max(1, M // 2) * (-(-R // gd)) / (ms_d / 1e3) / 1e9
+    g_pass = min(bk.H.get_pass() or R, R)
+    gd_pass = min(bk.H.get_pass(doubling=True) or R, R)
+    bytes_launch = N * K * (mv + 4) + g_pass * N * (3 * 8 + s_l)
+    bytes_d = N * K * (mv + 4) + gd_pass * N * 3 * 8
+    ach = bytes_launch * (M - 1) * (-(-R // g_pass)) / (ms / 1e3) / 1e9
+    ach_d = bytes_d * max(1, M // 2) * (-(-R // gd_pass)) / (ms_d / 1e3) / 1e9
     # the step kernel alone: event-timed runs of launches of one eager pass
     # (a graph replay records no events)
     launch_ms, n_prof = profiled(nat, bk.run, every=16)
@@ -524,11 +524,12 @@ def bench_batch(args, torch, jcl, nat, _kpm):
     out["roofline"] = hbm_roofline(bytes_launch, launch_ms, n_prof, "cheb_step_kernel",
                                    whole_replay_GBs=round(ach, 1),
                                    whole_replay_frac=round(ach / HBM_PEAK_GBS, 4),
-                                   left_vector_bytes_per_element=s_l, vectors_per_pass=g)
+                                   left_vector_bytes_per_element=s_l,
+                                   vectors_per_pass=g_pass)
     # (the committed counters are those of launches carrying all R vectors)
     out["roofline"]["traffic"] = profile_traffic(
         PROFILES / "cheb_step_c64_r8_batch64_square_trace.md", "traffic_bytes_per_launch",
-        N=N, K=K, R=R) if g == R else None
+        N=N, K=K, R=R) if g_pass == R else None
     out["mu_err_vs_c128"] = mu_err
     out["dos_err_vs_c128"] = dos_err
     out["accuracy_note"] = ("max over the 64 realisations and all n of |mu_c64 - mu_c128| / mu_0, "
