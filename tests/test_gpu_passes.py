@@ -141,3 +141,28 @@ def test_noalloc_and_stack_with_passes(torch_gpu):
     need = ctypes.c_size_t()
     assert nat.lib().jkpm_ell_set_pass(h.handle, 2, 4) == nat.JKPM_ERR_ARG
     assert nat.lib().jkpm_moments_workspace(h.handle, R, 1, ctypes.byref(need)) == nat.JKPM_OK
+
+
+@pytest.mark.parametrize("L,M", [(4, 1), (4, 2), (6, 3), (3, 5)])
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_short_recursions_and_tiny_lattices_in_passes(torch_gpu, L, M, dtype):
+    """M = 1..5 on lattices of 9..36 rows: passes of 4 (and the doubling form)
+    give the single pass's moments, and complex128 gives the oracle's."""
+    Ht = square(L, 30 + L)
+    N, R = Ht.shape[0], 8
+    r = vectors(N, R, 900 + M)
+    vr = _kpm.as_batch(r, N, dtype)
+    h = nat.DeviceELL(Ht.data, Ht.indices, N, Ht.len_row, 1.0, dtype)
+    one = h.moments(vr, M)[0].cpu().numpy()
+    one_d = nat.moments_ex(h, None, vr, M, True).cpu().numpy()
+    h.set_pass(4)
+    h.set_pass(4, doubling=True)
+    got = h.moments(vr, M)[0].cpu().numpy()
+    got_d = nat.moments_ex(h, None, vr, M, True).cpu().numpy()
+    assert np.max(np.abs(got - one)) <= 1e-12 * N
+    assert np.max(np.abs(got_d - one_d)) <= 1e-12 * N
+    if dtype == "c128":
+        for k in range(R):
+            o = O.rec_A_tab(M, Ht.data, Ht.indices, Ht.len_row, r[k])
+            assert np.max(np.abs(got[:, k] - o)) <= 1e-10 * N, k
+            assert np.max(np.abs(got_d[:, k] - o)) <= 1e-10 * N, k
