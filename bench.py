@@ -854,7 +854,11 @@ def bench_kubo(args, torch, jcl, nat, _kpm):
                                  "v_x on H's slots, M = 512, R = 4, complex64 stacks, FP32 "
                                  "MFMA accumulate, FP64 combine, sigma on 512 energies",
                      "N": N, "M": M, "R": R})
-    kstep_bytes = N * 4 * (8 + 8 + 4) + R * N * 8 * 7
+    # per slot: H's value (4 bytes packed when real), v's value (4 when purely
+    # imaginary), the column; per row: 7 complex64 vector streams
+    h_b = 4 if Hd.is_real() else 8
+    v_b = 4 if Vd.is_imag() else 8
+    kstep_bytes = N * 4 * (h_b + v_b + 4) + R * N * 8 * 7
     kubo_pmc = PROFILES / "kubo_step_and_gemm_honeycomb512_pmc.md"
     out["roofline"] = {"bound": "mfma", "achieved": round(tf_gemm, 2), "peak": 157.3,
                        "unit": "TFLOP/s", "frac": round(tf_gemm / 157.3, 4),
@@ -867,7 +871,8 @@ def bench_kubo(args, torch, jcl, nat, _kpm):
                        "whole_step_TFLOPs": round(tf, 2),
                        "whole_step_frac": round(tf / 157.3, 4)}
     out["stack_step"] = hbm_roofline(kstep_bytes, kstep_ms, n_kstep, "kubo_step_kernel",
-                                     steps_per_call=M - 1)
+                                     steps_per_call=M - 1, h_value_bytes=h_b,
+                                     v_value_bytes=v_b)
     out["stack_step"]["traffic"] = profile_traffic(kubo_pmc, "step_traffic_bytes_per_launch",
                                                    N=N, M=M, R=R)
     out["mu_nm_err_vs_fp64"] = mu_err

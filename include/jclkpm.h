@@ -138,9 +138,15 @@ int jkpm_ell_hermiticity(const jkpm_ell *h, double *abs_err, double *max_abs, vo
  * counterpart (the reference stores complex128 values always,
  * jclsquant/ell_matrix.py:187-200). */
 int jkpm_ell_is_real(const jkpm_ell *h, int *is_real);
-/* on = 0: the step kernels of this handle read its complex values even when
- * they are all real (the complex128 storage of the reference, or an A/B
- * comparison); on = 1 (the default) lets them read the packed real copy. */
+/* *is_imag = 1 when every stored real part is exactly 0 and some imaginary
+ * part is not (checked at the same uploads): the velocity -i d H of a real
+ * H.  The handle's packed copy then holds the imaginary parts, which the
+ * Kubo stack step reads for v (4 bytes per slot instead of 8).  No reference
+ * counterpart. */
+int jkpm_ell_is_imag(const jkpm_ell *h, int *is_imag);
+/* on = 0: the kernels of this handle read its complex values even when they
+ * are all real or all imaginary (the complex128 storage of the reference, or
+ * an A/B comparison); on = 1 (the default) lets them read the packed copy. */
 int jkpm_ell_use_real_values(jkpm_ell *h, int on);
 
 /* Left vectors as phases.  A complex64 handle's plain recursion with one

@@ -61,6 +61,7 @@ SIGNATURES = {
     "jkpm_ell_get_values": (_int, [_vp, _vp, _int, _vp]),
     "jkpm_ell_hermiticity": (_int, [_vp, _dp, _dp, _vp]),
     "jkpm_ell_is_real": (_int, [_vp, _ip]),
+    "jkpm_ell_is_imag": (_int, [_vp, _ip]),
     "jkpm_ell_use_real_values": (_int, [_vp, _int]),
     "jkpm_ell_use_left_phases": (_int, [_vp, _int]),
     "jkpm_phase_pack": (_int, [_vp, _c.c_longlong, _vp, _vp, _vp]),
@@ -487,6 +488,13 @@ class DeviceELL:
         kernels then read a packed real copy (jkpm_ell_is_real)."""
         v = _c.c_int()
         check(lib().jkpm_ell_is_real(self._h, _c.byref(v)))
+        return bool(v.value)
+
+    def is_imag(self) -> bool:
+        """True when every stored real part is zero and some imaginary part is
+        not; the Kubo step then reads the packed imaginary parts (jkpm_ell_is_imag)."""
+        v = _c.c_int()
+        check(lib().jkpm_ell_is_imag(self._h, _c.byref(v)))
         return bool(v.value)
 
     def use_real_values(self, on: bool = True) -> None:

@@ -840,6 +840,7 @@ int jkpm_ell_create(const double *data, const int *indices, int n_rows, int len_
     if (rc) return fail(rc);
     if ((rc = update_real_values(&H, s))) return fail(rc);
     h->values_real = H.real_values;
+    h->values_imag = H.imag_values;
     if ((e = hipStreamSynchronize(s)) != hipSuccess)
         return fail(hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__));
     if (!src_on_device) hipFree(idx_dev);
@@ -965,6 +966,7 @@ int jkpm_ell_create_csr(const int *indptr, const int *indices, const double *dat
     if (rc) return cleanup(rc);
     if ((rc = update_real_values(&H, s))) return cleanup(rc);
     h->values_real = H.real_values;
+    h->values_imag = H.imag_values;
     if ((e = hipStreamSynchronize(s)) != hipSuccess)
         return cleanup(hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__));
     jkpm_ell *done = h;
@@ -1028,6 +1030,7 @@ int jkpm_ell_create_like(const jkpm_ell *pattern, const double *data, double sca
     rc = upload_values(&H, data, scale, nullptr, src_on_device, (hipStream_t)stream);
     if (!rc) rc = update_real_values(&H, (hipStream_t)stream);
     h->values_real = H.real_values;
+    h->values_imag = H.imag_values;
     if (!rc) {
         e = hipStreamSynchronize((hipStream_t)stream);
         if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
@@ -1067,11 +1070,14 @@ int jkpm_ell_set_values(jkpm_ell *h, const double *data, double scale, int src_o
     // Values only: the slot-major column array stays as uploaded.
     h->herm_err = h->herm_max = -1.0;
     h->e.real_values = h->values_real = false;
+    h->e.imag_values = h->values_imag = false;
     int rc = upload_values(&h->e, data, scale, nullptr, src_on_device, (hipStream_t)stream);
     if (rc) return rc;
     if ((rc = update_real_values(&h->e, (hipStream_t)stream))) return rc;
     h->values_real = h->e.real_values;
+    h->values_imag = h->e.imag_values;
     h->e.real_values = h->values_real && h->use_real;
+    h->e.imag_values = h->values_imag && h->use_real;
     JKPM_HIP(hipStreamSynchronize((hipStream_t)stream));
     return JKPM_OK;
 }
@@ -1083,11 +1089,19 @@ int jkpm_ell_is_real(const jkpm_ell *h, int *is_real) {
     return JKPM_OK;
 }
 
+int jkpm_ell_is_imag(const jkpm_ell *h, int *is_imag) {
+    clear_error();
+    if (!h || !is_imag) return arg_error("NULL argument");
+    *is_imag = h->values_imag ? 1 : 0;
+    return JKPM_OK;
+}
+
 int jkpm_ell_use_real_values(jkpm_ell *h, int on) {
     clear_error();
     if (!h) return arg_error("handle is NULL");
     h->use_real = on != 0;
     h->e.real_values = h->values_real && h->use_real && h->e.rval;
+    h->e.imag_values = h->values_imag && h->use_real && h->e.rval;
     return JKPM_OK;
 }
 
@@ -1153,11 +1167,12 @@ int jkpm_ell_peierls(jkpm_ell *dst, const jkpm_ell *src, const jkpm_ell *bonds, 
     if (dst->e.vtype != src->e.vtype) return arg_error("dst and src must have one value type");
     dst->herm_err = dst->herm_max = -1.0;
     // Peierls phases make the values complex (see poison_real_copy)
-    if (dst->values_real) {
+    if (dst->values_real || dst->values_imag) {
         int rc0 = poison_real_copy(&dst->e, (hipStream_t)stream);
         if (rc0) return rc0;
     }
     dst->e.real_values = dst->values_real = false;
+    dst->e.imag_values = dst->values_imag = false;
     int rc = launch_peierls(src->e, bonds->e, &dst->e, make_double2(a_re, a_im),
                             make_double2(ax_re, ax_im), make_double2(ay_re, ay_im),
                             (hipStream_t)stream);

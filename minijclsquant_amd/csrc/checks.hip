@@ -91,18 +91,30 @@ __global__ void __launch_bounds__(kBlock) nonfinite_kernel(const double *__restr
     }
 }
 
-// rval[e] = Re val[e]; *cplx != 0 when some Im val[e] != 0 (NaN included).
+// rval[e] = Re val[e]; bit 0 of *flags is set when some Im val[e] != 0, bit
+// 1 when some Re val[e] != 0 (NaN counts as nonzero for both).
 template <typename Real>
 __global__ void __launch_bounds__(kBlock) real_pack_kernel(const Real *__restrict__ val, long tot,
                                                            Real *__restrict__ rval,
-                                                           unsigned *__restrict__ cplx) {
-    bool any = false;
+                                                           unsigned *__restrict__ flags) {
+    bool im = false, re = false;
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
          e += (long)gridDim.x * blockDim.x) {
         rval[e] = val[2 * e];
-        any |= !(val[2 * e + 1] == Real(0));
+        im |= !(val[2 * e + 1] == Real(0));
+        re |= !(val[2 * e] == Real(0));
     }
-    if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(cplx, 1u);
+    const unsigned bits = (__any(im) ? 1u : 0u) | (__any(re) ? 2u : 0u);
+    if (bits && (threadIdx.x & 63) == 0) atomicOr(flags, bits);
+}
+
+// rval[e] = Im val[e] (a purely imaginary matrix).
+template <typename Real>
+__global__ void __launch_bounds__(kBlock) imag_pack_kernel(const Real *__restrict__ val, long tot,
+                                                           Real *__restrict__ rval) {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+         e += (long)gridDim.x * blockDim.x)
+        rval[e] = val[2 * e + 1];
 }
 
 template <typename Real>
@@ -138,6 +150,7 @@ int update_real_values(Ell *H, hipStream_t s) {
         return !(e && atoi(e) == 0);
     }();
     H->real_values = false;
+    H->imag_values = false;
     if (!enabled) return JKPM_OK;
     const long tot = (long)H->n * H->k;
     const size_t rb = H->vtype == JKPM_C64 ? sizeof(float) : sizeof(double);
@@ -167,11 +180,28 @@ int update_real_values(Ell *H, hipStream_t s) {
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     hipFree(d);
     if (e != hipSuccess) return hip_fail(e, "real-value check", __FILE__, __LINE__);
-    H->real_values = host == 0;
-    // complex values: a graph captured while they were real would replay the
-    // real-value kernels on this copy; NaN makes that visible
-    if (!H->real_values) return poison_real_copy(H, s);
-    return JKPM_OK;
+    H->real_values = (host & 1u) == 0;
+    if (H->real_values) return JKPM_OK;
+    if ((host & 2u) == 0) {
+        // purely imaginary values: the copy holds the imaginary parts
+        DeviceInfo dev;
+        int rc = device_info(&dev);
+        if (rc) return rc;
+        const long want = (tot + kBlock - 1) / kBlock;
+        const int grid = (int)std::max(1L, std::min(want, (long)dev.cu_count * 8));
+        if (H->vtype == JKPM_C64)
+            hipLaunchKernelGGL(imag_pack_kernel<float>, dim3(grid), dim3(kBlock), 0, s,
+                               (const float *)H->val, tot, (float *)H->rval);
+        else
+            hipLaunchKernelGGL(imag_pack_kernel<double>, dim3(grid), dim3(kBlock), 0, s,
+                               (const double *)H->val, tot, (double *)H->rval);
+        JKPM_CHECK_LAUNCH("imag_pack_kernel");
+        H->imag_values = true;
+        return JKPM_OK;
+    }
+    // complex values: a graph captured while they were real or imaginary would
+    // replay the packed-value kernels on this copy; NaN makes that visible
+    return poison_real_copy(H, s);
 }
 
 int count_nonfinite(const double *x, long n, long long *first_bad, long long *count,

@@ -77,8 +77,12 @@ struct Ell {
     // Real parts of val, [k][n] reals of vtype's precision, owned by the
     // handle; valid while real_values (every stored imaginary part is exactly
     // 0, checked at each value upload), when the step kernels read these.
+    // While imag_values it holds the imaginary parts instead.
     void *rval = nullptr;
     bool real_values = false;
+    // every stored real part is exactly 0 and some imaginary part is not: rval
+    // then holds the imaginary parts instead (read by the Kubo step for v)
+    bool imag_values = false;
     // complex64 plain recursions with one left set keep the left vectors'
     // phases in the workspace and read them instead when every element is a
     // unit complex number (jkpm_ell_use_left_phases turns this off)
@@ -326,6 +330,7 @@ struct jkpm_ell {
     // upload); use_real: the caller lets the step kernels read the packed
     // real copy (jkpm_ell_use_real_values); e.real_values = both.
     bool values_real = false, use_real = true;
+    bool values_imag = false;  // the values are purely imaginary (Ell::imag_values)
 };
 
 // A tile plan of a handle: rows renumbered into patches, the matrix in that

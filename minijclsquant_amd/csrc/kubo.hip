@@ -48,7 +48,10 @@ using floatx4 = vec_t<float, 4>;
 // beta_l == 0).  The Kubo driver runs the right recursion (T_n, v T_n) and the
 // left one (L_m) side by side this way, so the slots are read once per step
 // for both.
-template <int VPL, int LPR, bool DUAL, bool LEFT>
+// HR: hval holds one real per slot (the packed real copy of a real H).
+// VI: vval holds one real w per slot with v = i w (the packed imaginary parts
+// of a purely imaginary v, such as the velocity of a real H).
+template <int VPL, int LPR, bool DUAL, bool LEFT, bool HR = false, bool VI = false>
 __global__ __launch_bounds__(kBlock) void kubo_step_kernel(
     const float *__restrict__ hval, const float *__restrict__ vval, const int *__restrict__ col,
     int n, int k, const float *__restrict__ x, const float *prev, float *out,
@@ -73,23 +76,51 @@ __global__ __launch_bounds__(kBlock) void kubo_step_kernel(
         for (int s = 0; s < k; ++s) {
             const long e = (long)s * n + row;
             const int j = col[e];
-            const C2 h = *reinterpret_cast<const C2 *>(hval + 2 * e);
+            // (h, w) as complex numbers; the packed forms load one real
+            C2 h, w;
+            if constexpr (HR) {
+                h.x = hval[e];
+                h.y = 0.f;
+            } else {
+                h = *reinterpret_cast<const C2 *>(hval + 2 * e);
+            }
+            if constexpr (DUAL) {
+                if constexpr (VI) {
+                    w.x = 0.f;
+                    w.y = vval[e];
+                } else {
+                    w = *reinterpret_cast<const C2 *>(vval + 2 * e);
+                }
+            }
             const V xv = *reinterpret_cast<const V *>(x + (long)j * ld * 2 + voff);
-            C2 w;
-            if constexpr (DUAL) w = *reinterpret_cast<const C2 *>(vval + 2 * e);
             V xq;
             if constexpr (LEFT) xq = *reinterpret_cast<const V *>(xl + (long)j * ld * 2 + voff);
 #pragma unroll
             for (int v = 0; v < VPL; ++v) {
-                sr[v] += h.x * xv[2 * v] - h.y * xv[2 * v + 1];
-                si[v] += h.x * xv[2 * v + 1] + h.y * xv[2 * v];
+                if constexpr (HR) {
+                    sr[v] += h.x * xv[2 * v];
+                    si[v] += h.x * xv[2 * v + 1];
+                } else {
+                    sr[v] += h.x * xv[2 * v] - h.y * xv[2 * v + 1];
+                    si[v] += h.x * xv[2 * v + 1] + h.y * xv[2 * v];
+                }
                 if constexpr (DUAL) {
-                    wr[v] += w.x * xv[2 * v] - w.y * xv[2 * v + 1];
-                    wi[v] += w.x * xv[2 * v + 1] + w.y * xv[2 * v];
+                    if constexpr (VI) {
+                        wr[v] -= w.y * xv[2 * v + 1];
+                        wi[v] += w.y * xv[2 * v];
+                    } else {
+                        wr[v] += w.x * xv[2 * v] - w.y * xv[2 * v + 1];
+                        wi[v] += w.x * xv[2 * v + 1] + w.y * xv[2 * v];
+                    }
                 }
                 if constexpr (LEFT) {
-                    qr[v] += h.x * xq[2 * v] - h.y * xq[2 * v + 1];
-                    qi[v] += h.x * xq[2 * v + 1] + h.y * xq[2 * v];
+                    if constexpr (HR) {
+                        qr[v] += h.x * xq[2 * v];
+                        qi[v] += h.x * xq[2 * v + 1];
+                    } else {
+                        qr[v] += h.x * xq[2 * v] - h.y * xq[2 * v + 1];
+                        qi[v] += h.x * xq[2 * v + 1] + h.y * xq[2 * v];
+                    }
                 }
             }
         }
@@ -140,11 +171,21 @@ __global__ __launch_bounds__(kBlock) void kubo_step_kernel(
     }
 }
 
+template <int VPL, int LPR, bool HR, bool VI>
+const void *step_fn_v(bool dual, bool left) {
+    if (left) return reinterpret_cast<const void *>(&kubo_step_kernel<VPL, LPR, true, true, HR, VI>);
+    return dual ? reinterpret_cast<const void *>(&kubo_step_kernel<VPL, LPR, true, false, HR, VI>)
+                : reinterpret_cast<const void *>(&kubo_step_kernel<VPL, LPR, false, false, HR>);
+}
+
+// hr: H's packed real values; vi: v's packed imaginary parts (dual only)
 template <int VPL, int LPR>
-const void *step_fn(bool dual, bool left) {
-    if (left) return reinterpret_cast<const void *>(&kubo_step_kernel<VPL, LPR, true, true>);
-    return dual ? reinterpret_cast<const void *>(&kubo_step_kernel<VPL, LPR, true, false>)
-                : reinterpret_cast<const void *>(&kubo_step_kernel<VPL, LPR, false, false>);
+const void *step_fn(bool dual, bool left, bool hr, bool vi) {
+    vi = vi && dual;
+    if (hr) return vi ? step_fn_v<VPL, LPR, true, true>(dual, left)
+                      : step_fn_v<VPL, LPR, true, false>(dual, left);
+    return vi ? step_fn_v<VPL, LPR, false, true>(dual, left)
+              : step_fn_v<VPL, LPR, false, false>(dual, left);
 }
 
 struct KStep {
@@ -152,24 +193,24 @@ struct KStep {
     int lpr;
 };
 
-KStep pick(int rb, bool wide, bool dual, bool left) {
+KStep pick(int rb, bool wide, bool dual, bool left, bool hr, bool vi) {
     if (wide) {
         switch (rb) {
-            case 2: return {step_fn<2, 1>(dual, left), 1};
-            case 4: return {step_fn<2, 2>(dual, left), 2};
-            case 8: return {step_fn<2, 4>(dual, left), 4};
-            case 16: return {step_fn<2, 8>(dual, left), 8};
-            case 32: return {step_fn<2, 16>(dual, left), 16};
+            case 2: return {step_fn<2, 1>(dual, left, hr, vi), 1};
+            case 4: return {step_fn<2, 2>(dual, left, hr, vi), 2};
+            case 8: return {step_fn<2, 4>(dual, left, hr, vi), 4};
+            case 16: return {step_fn<2, 8>(dual, left, hr, vi), 8};
+            case 32: return {step_fn<2, 16>(dual, left, hr, vi), 16};
             default: break;
         }
     }
     switch (rb) {
-        case 1: return {step_fn<1, 1>(dual, left), 1};
-        case 2: return {step_fn<1, 2>(dual, left), 2};
-        case 4: return {step_fn<1, 4>(dual, left), 4};
-        case 8: return {step_fn<1, 8>(dual, left), 8};
-        case 16: return {step_fn<1, 16>(dual, left), 16};
-        case 32: return {step_fn<1, 32>(dual, left), 32};
+        case 1: return {step_fn<1, 1>(dual, left, hr, vi), 1};
+        case 2: return {step_fn<1, 2>(dual, left, hr, vi), 2};
+        case 4: return {step_fn<1, 4>(dual, left, hr, vi), 4};
+        case 8: return {step_fn<1, 8>(dual, left, hr, vi), 8};
+        case 16: return {step_fn<1, 16>(dual, left, hr, vi), 16};
+        case 32: return {step_fn<1, 32>(dual, left, hr, vi), 32};
         default: break;
     }
     return {nullptr, 0};
@@ -603,14 +644,17 @@ int launch_kubo_step(const Ell &H, const Ell *Vop, const void *x, const void *pr
         int rb = 32;
         while (rb > R - r0) rb >>= 1;
         const bool wide = (R % 2 == 0) && (r0 % 2 == 0);
-        KStep e = pick(rb, wide, dual, left != nullptr);
+        // packed values where the handles have them: a real H, an imaginary v
+        const bool hr = H.real_values && H.rval;
+        const bool vi = dual && Vop->imag_values && Vop->rval;
+        KStep e = pick(rb, wide, dual, left != nullptr, hr, vi);
         if (!e.fn) return arg_error("unsupported vector group");
         const int rpb = kWaves * (64 / e.lpr);
         long rpc = ((long)H.n + nchunks - 1) / nchunks;
         rpc = (rpc + rpb - 1) / rpb * rpb;
         const size_t off = (size_t)r0 * 8;
-        const void *p_h = H.val;
-        const void *p_v = dual ? Vop->val : nullptr;
+        const void *p_h = hr ? H.rval : H.val;
+        const void *p_v = dual ? (vi ? Vop->rval : Vop->val) : nullptr;
         const int *p_c = H.col;
         int p_n = H.n, p_k = H.k;
         const void *p_x = (const char *)x + off;

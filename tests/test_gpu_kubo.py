@@ -47,6 +47,35 @@ def test_mu_nm_vs_oracle(torch_gpu, L, M, R):
     assert err <= TOL_MU, err
 
 
+def test_packed_values_match_complex_values(torch_gpu):
+    """A real H and its velocity -i d H (purely imaginary) are detected at
+    upload; the Kubo step then reads one float per slot of each.  With the
+    packed copies switched off the same moments come from the complex
+    values: equal to the complex64 bar, and both against the oracle."""
+    from minijclsquant_amd.kubo import velocity_handle
+    from minijclsquant_amd import _kpm
+
+    A, B = lattice(10, seed=4)
+    r = vectors(A.shape[0], 3, 77)
+    Hd, Vd = velocity_handle(A, "x")
+    assert Hd.is_real() and not Hd.is_imag()
+    assert Vd.is_imag() and not Vd.is_real()
+    rb = _kpm.as_batch(r, int(A.shape[0]), "c64")
+    M = 48
+    packed = nat.kubo_mu_nm(Hd, Vd, rb, M).cpu().numpy()
+    Hd.use_real_values(False)
+    Vd.use_real_values(False)
+    cplx = nat.kubo_mu_nm(Hd, Vd, rb, M).cpu().numpy()
+    ref = oracle_mu(A, B, jcl.modifier_velocity(A, "x"), r, M)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(packed - cplx)) <= TOL_MU * scale
+    for mu in (packed, cplx):
+        assert np.max(np.abs(mu - ref)) <= TOL_MU * scale
+    # a value upload that is neither real nor imaginary clears both kinds
+    Vd.set_values(jcl.modifier_velocity(A, "x") + 0.5, 1.0)
+    assert not Vd.is_imag() and not Vd.is_real()
+
+
 def test_mu_nm_y_direction_and_single_vector(torch_gpu):
     A, B = lattice(8, seed=3)
     r = jcl.random_vector_generator(A.shape[0], 5)

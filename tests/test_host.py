@@ -719,7 +719,7 @@ def test_bench_reads_traffic_from_committed_profiles():
            "headline_dbl_pass4": 2097152 * (4 * 8 + 4 * 24),
            "batch": 24054938 * (5 * 8 + 8 * 28), "config1": 65536 * (5 * 12 + 4 * 64),
            "grad": 262144 * (5 * 12 + 8 * 64), "kubo_gemm": 2 * 512 * 2097152 * 8,
-           "kubo_step": 524288 * 4 * 20 + 4 * 524288 * 8 * 7}
+           "kubo_step": 524288 * 4 * 12 + 4 * 524288 * 8 * 7}
     for k, v in got.items():
         assert v is not None, k
         assert 1.0 <= v / alg[k] <= 1.07, (k, v / alg[k])
