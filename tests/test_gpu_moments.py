@@ -324,7 +324,7 @@ def test_precision_form_full_size(torch_gpu):
     assert np.max(np.abs(m128)) <= N * (1 + 1e-10)
     # one vector's complex128 moments against the oracle at full size (the
     # C restatement runs this in a few seconds on the host), and every
-    # complex64 moment against complex128 (the north star's per-mu_n error)
+    # complex64 moment against complex128 (the error of every mu_n)
     want = O.rec_A_tab(M, A.data * (1 / B), A.indices, A.len_row, r[3])
     assert np.max(np.abs(m128[3] - want)) / N <= 1e-10
     assert np.max(np.abs(m64 - m128)) / N <= 1e-6
