@@ -34,14 +34,30 @@ def as_batch(v, n: int, dtype) -> "torch.Tensor":
     return t.to(device="cuda", dtype=cd).transpose(0, 1).contiguous()
 
 
+def apply_launch(h, vr, doubling: bool, tune):
+    """Launch shape of a fresh handle: tune None applies the tuner's table
+    entry for this shape when there is one; True tunes it (once per shape
+    and machine, then from the table); False keeps the library defaults."""
+    if tune is False:
+        return
+    from . import autotune
+
+    if tune:
+        autotune.tuned(h, vr, doubling=doubling)
+    else:
+        autotune.apply_cached(h, vr, doubling)
+
+
 def moments(A, M: int, v_r, v_l=None, v_l2=None, dtype="c128", scale: float = 1.0,
-            doubling: bool = False):
+            doubling: bool = False, tune=None):
     """mu[l, r, m] = <v_l{l}[r] | T_m(scale*A) | v_r[r]> as a device tensor
     complex128 [n_left, R, M]; A is an ``ell_matrix``.  doubling (v_l = v_r,
-    Hermitian A): two moments per Chebyshev step (jkpm_moments_ex)."""
+    Hermitian A): two moments per Chebyshev step (jkpm_moments_ex).  tune: see
+    apply_launch."""
     n = int(A.shape[0])
     h = A.device(dtype, scale)
     vr = as_batch(v_r, n, dtype)
+    apply_launch(h, vr, doubling, tune)
     if doubling:
         if v_l is not None or v_l2 is not None:
             raise ValueError("the doubling form needs v_l = v_r")

@@ -23,7 +23,7 @@ from pathlib import Path
 
 from . import _native as nat
 
-__all__ = ["tuned", "tuned_batch", "table_path", "pass_candidates"]
+__all__ = ["tuned", "tuned_batch", "table_path", "pass_candidates", "apply_cached"]
 
 _lock = threading.Lock()
 
@@ -180,6 +180,25 @@ def _step_slope(h, v_r, doubling, us):
             t = min(t, e0.elapsed_time(e1))
         times.append(t)
     return (times[1] - times[0]) * 1e3 / (2 * a)
+
+
+def apply_cached(h: "nat.DeviceELL", v_r, doubling: bool = False) -> bool:
+    """Applies the table's launch shape and vectors per pass for this shape
+    and form, when the table has one, without timing anything; returns
+    whether it did.  ``JKPM_LAUNCH`` applies as in tuned."""
+    env = os.environ.get("JKPM_LAUNCH")
+    if env:
+        v, m = (int(x) for x in env.split(","))
+        h.set_launch(v, m, doubling)
+        return True
+    key = _key(h, int(v_r.shape[1]), doubling)
+    with _lock:
+        ent = _load().get(key)
+    if ent is None:
+        return False
+    h.set_launch(ent["variant"], ent["chunk_mult"], doubling)
+    h.set_pass(ent.get("pass", 0), doubling)
+    return True
 
 
 def tuned(h: "nat.DeviceELL", v_r, steps: int = 8, retune: bool = False,

@@ -46,10 +46,13 @@ def _rescaled(H):
     return H, s
 
 
-def kpm_moments(H, M, random_vector, *, dtype="c128", bounds=True, doubling=False):
-    """Moments mu_m = <r|T_m(H~)|r> as a device tensor complex128 [R, M]."""
+def kpm_moments(H, M, random_vector, *, dtype="c128", bounds=True, doubling=False, tune=None):
+    """Moments mu_m = <r|T_m(H~)|r> as a device tensor complex128 [R, M].
+    tune: None applies a launch shape the tuner already has for this shape,
+    True tunes it (once per shape and machine), False keeps the defaults."""
     Hk, s = _rescaled(H) if bounds else (H, 1.0)
-    return _kpm.moments(Hk, int(M), random_vector, dtype=dtype, scale=s, doubling=doubling)[0]
+    return _kpm.moments(Hk, int(M), random_vector, dtype=dtype, scale=s, doubling=doubling,
+                        tune=tune)[0]
 
 
 def _dos_table(H, Ef_vec, mu_rm, kernel):
@@ -62,19 +65,20 @@ def _dos_table(H, Ef_vec, mu_rm, kernel):
 
 
 def kpm_dos(H, M=None, random_vector=None, *, energies=None, dtype="c128", kernel="jackson",
-            doubling=False):
-    """DOS [E, 2] = [energies, unnormalised DOS] (jclsquant/observables.py:26-54)."""
+            doubling=False, tune=None):
+    """DOS [E, 2] = [energies, unnormalised DOS] (jclsquant/observables.py:26-54).
+    tune: as in kpm_moments."""
     if M is None:
         M = int(np.sqrt(H.shape[0]))
     Ef_vec = _energy_grid(H, M, energies)
     if random_vector is None:
         random_vector = random_vector_generator(H.shape[0])
-    mu = kpm_moments(H, M, random_vector, dtype=dtype, doubling=doubling)
+    mu = kpm_moments(H, M, random_vector, dtype=dtype, doubling=doubling, tune=tune)
     return _dos_table(H, Ef_vec, mu, kernel)
 
 
 def kpm_n_dos_n(H, M=None, random_vector_l=None, random_vector_r=None, bounds=True,
-                proyector=None, *, energies=None, dtype="c128", kernel="jackson"):
+                proyector=None, *, energies=None, dtype="c128", kernel="jackson", tune=None):
     """DOS from mu_m = <v_l|T_m|v_r>, optionally with v_l projected by an ELL
     matrix first (jclsquant/observables.py:59-120)."""
     if M is None:
@@ -86,7 +90,8 @@ def kpm_n_dos_n(H, M=None, random_vector_l=None, random_vector_r=None, bounds=Tr
     if proyector is not None:
         random_vector_l = _project(proyector, random_vector_l)
     Hk, s = _rescaled(H) if bounds else (H, 1.0)
-    mu = _kpm.moments(Hk, int(M), random_vector_r, v_l=random_vector_l, dtype=dtype, scale=s)[0]
+    mu = _kpm.moments(Hk, int(M), random_vector_r, v_l=random_vector_l, dtype=dtype, scale=s,
+                      tune=tune)[0]
     return _dos_table(H, Ef_vec, mu, kernel)
 
 

@@ -22,30 +22,30 @@ __all__ = ["rec_A_tab", "rec_A_tab2v", "rec_A_tab2v_2", "rec_A_vec", "rec_com_A_
            "rec_com_A_vec_tab"]
 
 
-def rec_A_tab(M, A, v=None, *, dtype="c128", doubling=False):
+def rec_A_tab(M, A, v=None, *, dtype="c128", doubling=False, tune=None):
     """mu_m = <v|T_m(A)|v>, m < M (jclsquant/recurrence_functions.py:9-40)."""
     if v is None:
         print('You must introduce a vector.')
         return
-    mu = _kpm.moments(A, int(M), v, dtype=dtype, doubling=doubling)[0]
+    mu = _kpm.moments(A, int(M), v, dtype=dtype, doubling=doubling, tune=tune)[0]
     return _kpm.to_host(mu, _kpm.is_single(v))
 
 
-def rec_A_tab2v(M, A, v_l=None, v_r=None, *, dtype="c128"):
+def rec_A_tab2v(M, A, v_l=None, v_r=None, *, dtype="c128", tune=None):
     """mu_m = <v_l|T_m(A)|v_r> (jclsquant/recurrence_functions.py:44-74)."""
     if v_r is None or v_l is None:
         print('You must introduce a right and left vector.')
         return
-    mu = _kpm.moments(A, int(M), v_r, v_l=v_l, dtype=dtype)[0]
+    mu = _kpm.moments(A, int(M), v_r, v_l=v_l, dtype=dtype, tune=tune)[0]
     return _kpm.to_host(mu, _kpm.is_single(v_r))
 
 
-def rec_A_tab2v_2(M, A, v_l=None, v_r=None, *, dtype="c128"):
+def rec_A_tab2v_2(M, A, v_l=None, v_r=None, *, dtype="c128", tune=None):
     """(<v_l|T_m v_r>, <v_r|T_m v_r>) (jclsquant/recurrence_functions.py:76-111)."""
     if v_r is None or v_l is None:
         print('You must introduce a right and left vector.')
         return
-    mu = _kpm.moments(A, int(M), v_r, v_l=v_l, v_l2=v_r, dtype=dtype)
+    mu = _kpm.moments(A, int(M), v_r, v_l=v_l, v_l2=v_r, dtype=dtype, tune=tune)
     single = _kpm.is_single(v_r)
     return _kpm.to_host(mu[0], single), _kpm.to_host(mu[1], single)
 

@@ -198,3 +198,23 @@ def test_hermiticity_guard_on_reference_graphene(torch_gpu):
     m1 = jcl.rec_A_tab(64, Ah.modifier(jcl.modifier_bounds), rr, doubling=True)
     m0 = jcl.rec_A_tab(64, Ah.modifier(jcl.modifier_bounds), rr)
     assert np.max(np.abs(m1 - m0)) <= 1e-10 * Hh.shape[0]
+
+
+def test_dropin_dos_with_tuned_launch(torch_gpu, tmp_path, monkeypatch):
+    """kpm_dos(tune=True) tunes this shape once into the table; tune=None
+    (the default) then applies the table's launch shape and vectors per pass
+    and gives bitwise the same DOS; tune=False runs the library defaults,
+    equal to the complex64 bar."""
+    monkeypatch.setenv("JKPM_TUNE_TABLE", str(tmp_path / "table.json"))
+    monkeypatch.delenv("JKPM_LAUNCH", raising=False)
+    H, Dx, Dy = jcl.honeycomb_lattice(48, t=-2.7, W=1.0, seed=9)
+    A = jcl.ell_matrix(H, Dx, Dy)
+    N = A.shape[0]
+    r = np.stack([jcl.random_vector_generator(N, 600 + s) for s in range(16)])
+    M = 96
+    tuned = jcl.kpm_dos(A, M, r, dtype="c64", tune=True)
+    assert (tmp_path / "table.json").exists()
+    cached = jcl.kpm_dos(A, M, r, dtype="c64")
+    assert np.array_equal(tuned, cached)
+    plain = jcl.kpm_dos(A, M, r, dtype="c64", tune=False)
+    assert np.max(np.abs(plain[:, 1] - tuned[:, 1])) <= 1e-5 * np.max(np.abs(plain[:, 1]))

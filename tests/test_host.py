@@ -751,3 +751,41 @@ def test_bench_pass_roofline_fields():
     d = bench.pass_note(N, K, 10, 4, mv, s_v, 0, 0.03)
     assert d["passes_per_step"] == 3
     assert d["step_bytes_all_passes"] == 3 * N * K * 8 + 10 * N * 24
+
+
+def test_apply_cached_uses_the_table_only(tmp_path, monkeypatch):
+    """autotune.apply_cached applies a table entry (launch shape and vectors
+    per pass) without timing anything, and reports when there is none."""
+    import json
+
+    from minijclsquant_amd import autotune
+
+    monkeypatch.setenv("JKPM_TUNE_TABLE", str(tmp_path / "table.json"))
+    monkeypatch.delenv("JKPM_LAUNCH", raising=False)
+    monkeypatch.setattr(autotune, "_key", lambda h, R, d: f"k|R={R}|{d}")
+
+    class Handle:
+        def __init__(self):
+            self.launch, self.g = (0, 8), 0
+
+        def set_launch(self, v, m, doubling=False):
+            self.launch = (v, m)
+
+        def set_pass(self, g, doubling=False):
+            self.g = g
+
+        def autotune(self, *a, **k):
+            raise AssertionError("apply_cached must not time anything")
+
+    class Vecs:
+        shape = (100, 16)
+
+    h = Handle()
+    assert autotune.apply_cached(h, Vecs()) is False and h.launch == (0, 8)
+    (tmp_path / "table.json").write_text(json.dumps(
+        {"k|R=16|False": {"variant": 12, "chunk_mult": 4, "pass": 4, "step_us": 1.0}}))
+    assert autotune.apply_cached(h, Vecs()) is True
+    assert h.launch == (12, 4) and h.g == 4
+    assert autotune.apply_cached(h, Vecs(), doubling=True) is False
+    monkeypatch.setenv("JKPM_LAUNCH", "3,16")
+    assert autotune.apply_cached(h, Vecs(), doubling=True) is True and h.launch == (3, 16)
