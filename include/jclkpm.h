@@ -131,7 +131,7 @@ int jkpm_ell_hermiticity(const jkpm_ell *h, double *abs_err, double *max_abs, vo
  * bytes per slot instead of 8 or 16; results are those of the complex
  * arithmetic with zero imaginary parts.  JKPM_REAL_VALUES=0 in the
  * environment turns the check and the copy off (every handle then reports 0
- * and runs on its complex values).  A HIP graph captured while the values
+ * here and in jkpm_ell_is_imag, and runs on its complex values).  A HIP graph captured while the values
  * were real keeps the real-value kernels: after a value upload or Peierls
  * update that makes them complex, capture it again (the stale copy is
  * overwritten with NaN, so such a replay gives NaN moments).  No reference
