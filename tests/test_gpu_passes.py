@@ -103,20 +103,30 @@ def test_c64_phases_in_passes(torch_gpu):
         assert np.max(np.abs(got - one)) <= bar("c64", g) * N, g
 
 
-def test_batch_passes(torch_gpu):
+@pytest.mark.parametrize("doubling", [False, True])
+def test_batch_passes(torch_gpu, doubling):
+    """A realisation batch in passes of 4 vectors, eager and replayed from a
+    captured graph (the pass column copies are captured too), against the
+    batch in one pass."""
     torch = torch_gpu
     blocks, vecs = [], []
     for g, L in enumerate((10, 15, 7)):
         d, i = jcl.square_lattice_ell(L, t=-1.0, W=1.0, seed=60 + g)
         blocks.append((d, i, L * L, 5, 1.0 / 4.6))
         vecs.append(vectors(L * L, 8, 20 * g))
-    bk = jcl.BatchKPM(blocks, vecs, 50, "c128")
+    bk = jcl.BatchKPM(blocks, vecs, 50, "c128", doubling=doubling)
     one = bk.run().clone()
-    bk.H.set_pass(4)
+    bk.H.set_pass(4, doubling=doubling)
     bk.ws = None  # the workspace size follows the setting
     got = bk.run().clone()
     torch.cuda.synchronize()
     assert (got - one).abs().max().item() <= 1e-12 * 225
+    for _ in range(3):  # replays with eager work between them
+        rep = bk.replay().clone()
+        torch.cuda.synchronize()
+        assert torch.equal(rep, got)
+        bk.run()
+        torch.cuda.synchronize()
 
 
 def test_noalloc_and_stack_with_passes(torch_gpu):
