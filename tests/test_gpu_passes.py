@@ -103,18 +103,19 @@ def test_c64_phases_in_passes(torch_gpu):
         assert np.max(np.abs(got - one)) <= bar("c64", g) * N, g
 
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("doubling", [False, True])
-def test_batch_passes(torch_gpu, doubling):
+def test_batch_passes(torch_gpu, doubling, dtype):
     """A realisation batch in passes of 4 vectors, eager and replayed from a
-    captured graph (the pass column copies are captured too), against the
-    batch in one pass."""
+    captured graph (the pass column copies, and for complex64 the left-phase
+    packing, are captured too), against the batch in one pass."""
     torch = torch_gpu
     blocks, vecs = [], []
     for g, L in enumerate((10, 15, 7)):
         d, i = jcl.square_lattice_ell(L, t=-1.0, W=1.0, seed=60 + g)
         blocks.append((d, i, L * L, 5, 1.0 / 4.6))
         vecs.append(vectors(L * L, 8, 20 * g))
-    bk = jcl.BatchKPM(blocks, vecs, 50, "c128", doubling=doubling)
+    bk = jcl.BatchKPM(blocks, vecs, 50, dtype, doubling=doubling)
     one = bk.run().clone()
     bk.H.set_pass(4, doubling=doubling)
     bk.ws = None  # the workspace size follows the setting
